@@ -1,0 +1,140 @@
+/*
+ * C-ABI of libsoliton.so: the MI355X relaxation core of soliton_solver.
+ *
+ * The reference reaches its Numba-CUDA kernels from Python (Simulation in
+ * core/simulation.py, theory kernels/observables modules); this library is the
+ * native side of that same boundary.  Plain pointers and sizes only; every call
+ * returns 0 on success and a non-zero status on failure, with the message in
+ * ss_last_error(ctx) (or ss_global_error() when there is no context yet).
+ * No exception crosses the ABI.  One context = one relaxation on one GPU, one
+ * HIP stream; calls on one context must come from one host thread.
+ *
+ * Host arrays use the reference's flat layout (core/utils.py:16-42):
+ * idx(a, x, y) = a*xlen*ylen + x*ylen + y, FP64.
+ */
+#ifndef SOLITON_H
+#define SOLITON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ss_ctx ss_ctx;
+
+/* Theory ids (registry names of soliton_solver/theories/<name>/__init__.py). */
+enum {
+  SS_THEORY_CHIRAL_MAGNET = 1,
+  SS_THEORY_GINZBURG_LANDAU = 2,
+  SS_THEORY_BABY_SKYRME = 3,
+  SS_THEORY_ANISOTROPIC_SC = 4,
+  SS_THEORY_BEC = 5,
+  SS_THEORY_MAXWELL_CHERN_SIMONS_HIGGS = 6,
+  SS_THEORY_FERROMAGNETIC_SC = 7,
+  SS_THEORY_SPIN_TRIPLET = 8,
+  SS_THEORY_LIQUID_CRYSTAL = 9
+};
+
+/* Storage / stencil precision.  Energy and charge sums and the arrest test are
+ * FP64 in both. */
+enum { SS_PREC_FP64 = 0, SS_PREC_FP32 = 1 };
+
+/* Variation used by the flow and by ss_variation:
+ * REFERENCE: the reference's Euler-Lagrange expression (theory
+ *   kernels.do_gradient_step_point), masks and projections included;
+ * EXACT_ADJOINT: the exact gradient dE_h/dphi of the discrete energy sum. */
+enum { SS_VAR_REFERENCE = 0, SS_VAR_EXACT_ADJOINT = 1 };
+
+/* Buffers for ss_upload / ss_download. */
+enum { SS_BUF_FIELD = 0, SS_BUF_VELOCITY = 1 };
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ss_version(void);
+
+/* Message of the last failure that had no context (ss_ctx_create). */
+const char* ss_global_error(void);
+
+/* Number of GPUs visible to HIP (0 when none); never fails. */
+int ss_device_count(void);
+
+/* Replaces Simulation.__init__ buffer allocation (core/simulation.py:43-115)
+ * and set_params (:144-166).  p_i / p_f are the packed arrays of the theory's
+ * pack_device_params (e.g. chiral_magnet/params.py:333-378). */
+int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f, int n_f,
+                  int precision, int variation_mode, ss_ctx** out);
+int ss_ctx_destroy(ss_ctx* ctx);
+
+/* Simulation.set_params (core/simulation.py:144-166).  The lattice shape
+ * (xlen, ylen, number_total_fields) must not change. */
+int ss_set_params(ss_ctx* ctx, const int32_t* p_i, int n_i, const double* p_f, int n_f);
+
+/* Switch storage precision in place (FP32 relaxation -> FP64 polish). */
+int ss_set_precision(ss_ctx* ctx, int precision);
+int ss_set_variation_mode(ss_ctx* ctx, int variation_mode);
+
+/* Copies of Field / Velocity between the host (FP64, reference layout) and
+ * the device.  n must equal number_total_fields*xlen*ylen. */
+int ss_upload(ss_ctx* ctx, int buf, const double* host, size_t n);
+int ss_download(ss_ctx* ctx, int buf, double* host, size_t n);
+
+/* nsteps x Simulation.step (core/simulation.py:388-409 ->
+ * core/integrator.py:217-348): step i compares its new energy with the
+ * previous step's (the first with prev_energy) using strict '>', and kills the
+ * velocity when killkinen is set.  Returns the last step's energy, the signed
+ * max of its stage-4 energy gradient (the reference's err, utils.py:397-420)
+ * and max |stage-4 energy gradient|.  Any output pointer may be NULL. */
+int ss_step(ss_ctx* ctx, double prev_energy, int nsteps, double* e_new, double* err_signed_max,
+            double* err_abs_max);
+
+/* Per-step energies and errs of the last ss_step call (n <= its nsteps). */
+int ss_step_history(ss_ctx* ctx, double* energies, double* errs, int n);
+
+/* theory observables.compute_energy (e.g. chiral_magnet/observables.py:15-57). */
+int ss_energy(ss_ctx* ctx, double* e);
+
+/* Charges of the theory: kind indexes the theory's charge list (chiral magnet:
+ * 0 skyrmion number, 1 magnetic charge; Ginzburg-Landau: 0 vortex number ...). */
+int ss_charge(ss_ctx* ctx, int kind, double* q);
+int ss_num_charges(ss_ctx* ctx);
+
+/* Per-site density times grid volume (kind -1: energy, >= 0: charge kind),
+ * n = xlen*ylen values in idx_field(0, x, y) order
+ * (Simulation.compute_energy_density, core/simulation.py:194-209). */
+int ss_density(ss_ctx* ctx, int kind, double* host_out, size_t n);
+
+/* Energy gradient EG of the current Field (the gradient kernel with
+ * Temp = Field, core/integrator.py:15-48), FP64, reference layout. */
+int ss_variation(ss_ctx* ctx, int mode, double* host_eg, size_t n);
+
+/* Device-pointer variation for the autograd boundary: field_dev and grad_dev
+ * hold nf*xlen*ylen values of the context's precision in reference layout;
+ * runs on ``stream`` (hipStream_t, NULL = the context's stream). */
+int ss_variation_dev(ss_ctx* ctx, const void* field_dev, void* grad_dev, int mode, void* stream);
+
+/* Device-pointer energy: writes the FP64 total into energy_dev[0]. */
+int ss_energy_dev(ss_ctx* ctx, const void* field_dev, double* energy_dev, void* stream);
+
+/* Capture steps_per_graph steps (rounded up to even) into one hipGraph and
+ * replay it inside ss_step; 0 disables graphs. */
+int ss_use_graphs(ss_ctx* ctx, int steps_per_graph);
+
+/* Wait for all work of the context. */
+int ss_synchronize(ss_ctx* ctx);
+
+/* Eager steps with a HIP event around every launch (on the context's
+ * stream); avg_ms[0..5] = mean duration of pass 1, energy finalize, pass 2,
+ * pass 3, pass 4, error finalize.  State advances like ss_step. */
+int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
+
+/* Device memory held by the context, in bytes. */
+int ss_memory_bytes(ss_ctx* ctx, size_t* bytes);
+
+const char* ss_last_error(ss_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLITON_H */

@@ -1,0 +1,118 @@
+"""ctypes binding of ``libsoliton.so`` (declarations in include/soliton.h).
+
+The library is built in-tree by ``python -m soliton_solver_amd.build``.  There
+is no fallback: if the library is missing every entry point raises, so a run
+without the HIP path fails loudly instead of silently computing elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libsoliton.so"
+
+THEORY_CHIRAL_MAGNET = 1
+THEORY_GINZBURG_LANDAU = 2
+THEORY_BABY_SKYRME = 3
+THEORY_ANISOTROPIC_SC = 4
+THEORY_BEC = 5
+THEORY_MAXWELL_CHERN_SIMONS_HIGGS = 6
+THEORY_FERROMAGNETIC_SC = 7
+THEORY_SPIN_TRIPLET = 8
+THEORY_LIQUID_CRYSTAL = 9
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+VAR_REFERENCE = 0
+VAR_EXACT_ADJOINT = 1
+BUF_FIELD = 0
+BUF_VELOCITY = 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_dp = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); mirrors include/soliton.h one to one.
+SIGNATURES = {
+    "ss_version": (C.c_int, []),
+    "ss_global_error": (C.c_char_p, []),
+    "ss_device_count": (C.c_int, []),
+    "ss_ctx_create": (C.c_int, [C.c_int, _i32p, C.c_int, _f64p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "ss_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "ss_set_params": (C.c_int, [C.c_void_p, _i32p, C.c_int, _f64p, C.c_int]),
+    "ss_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_set_variation_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_upload": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t]),
+    "ss_download": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t]),
+    "ss_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _dp]),
+    "ss_step_history": (C.c_int, [C.c_void_p, _f64p, _f64p, C.c_int]),
+    "ss_energy": (C.c_int, [C.c_void_p, _dp]),
+    "ss_charge": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "ss_num_charges": (C.c_int, [C.c_void_p]),
+    "ss_density": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t]),
+    "ss_variation": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t]),
+    "ss_variation_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "ss_energy_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ss_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_synchronize": (C.c_int, [C.c_void_p]),
+    "ss_time_passes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _f64p]),
+    "ss_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "ss_last_error": (C.c_char_p, [C.c_void_p]),
+}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    """libsoliton.so is not built (or failed to load)."""
+
+
+def lib():
+    """Load the library once; raise ``LibraryMissing`` if it is not there."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LibraryMissing(
+            f"{LIB_PATH} is not built; run `python -m soliton_solver_amd.build` "
+            "(the HIP path has no CPU fallback)")
+    try:
+        L = C.CDLL(str(LIB_PATH))
+    except OSError as exc:
+        raise LibraryMissing(f"cannot load {LIB_PATH}: {exc}") from exc
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def error_text(ctx=None) -> str:
+    L = lib()
+    msg = L.ss_last_error(ctx) if ctx else L.ss_global_error()
+    return (msg or b"").decode(errors="replace")
+
+
+def check(rc: int, ctx=None) -> None:
+    """Map a non-zero status to the exception the reference raises for it.
+
+    -1 (bad argument or unsupported request) -> ValueError,
+    -2 (HIP runtime failure) -> RuntimeError.
+    """
+    if rc == 0:
+        return
+    msg = error_text(ctx)
+    if rc == -1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count() -> int:
+    return int(lib().ss_device_count())
+
+
+def out_double():
+    return C.c_double(0.0)

@@ -1,0 +1,285 @@
+"""``Simulation``: the driver object of the reference API, over libsoliton.so.
+
+Mirrors soliton_solver/core/simulation.py (``__init__`` :43-115,
+``initialize`` :117-142, ``set_params`` :144-166, ``observables`` :168-192,
+``step`` :388-409, ``compute_energy_density`` :194-209): same construction
+from ``(params, theory)``, same plugin attribute checks, same return values.
+The difference is where the state lives: the reference keeps a dozen FP64
+Numba device arrays per field and runs ~15 launches and 2 host round trips per
+step; here one native context (``ss_ctx``) owns the device state and a step is
+four fused HIP passes with the arrest decision taken on the device.
+
+Extras beyond the reference (no counterpart there): ``steps`` (many steps per
+call), ``minimize`` (the headless loop the reference documents at
+core/simulation.py:8,39 but does not define), ``precision`` ("fp64" or the
+"fp32" stencil / FP64-reduction form), ``energy_gradient`` and array
+injection ``initialize({"mode": "array", ...})``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+from .params import Params
+
+_PREC = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}
+_VAR = {"reference": _lib.VAR_REFERENCE, "exact_adjoint": _lib.VAR_EXACT_ADJOINT}
+
+
+class DeviceBuffer:
+    """Host view of one device buffer with the Numba-style copy methods.
+
+    The reference exposes ``sim.Field`` / ``sim.Velocity`` as Numba device
+    arrays; code that calls ``sim.Field.copy_to_host()`` keeps working.
+    Values are FP64 in the reference flat layout (core/utils.py:16-42).
+    """
+
+    def __init__(self, sim: "Simulation", buf: int):
+        self._sim = sim
+        self._buf = buf
+
+    @property
+    def size(self) -> int:
+        return int(self._sim.rp.dim_fields)
+
+    @property
+    def shape(self):
+        return (self.size,)
+
+    def copy_to_host(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.float64)
+        _lib.check(_lib.lib().ss_download(self._sim._ctx, self._buf, out, out.size), self._sim._ctx)
+        return out
+
+    def copy_to_device(self, host) -> None:
+        a = np.ascontiguousarray(np.asarray(host, dtype=np.float64).ravel())
+        if a.size != self.size:
+            raise ValueError(f"expected {self.size} values, got {a.size}")
+        _lib.check(_lib.lib().ss_upload(self._sim._ctx, self._buf, a, a.size), self._sim._ctx)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.copy_to_host()
+        return a if dtype is None else a.astype(dtype)
+
+
+class Simulation:
+    """Arrested-Newton-flow relaxation of one theory on one GPU."""
+
+    def __init__(self, params, theory, *, precision: str = "fp64", variation: str = "reference"):
+        self.theory = theory
+        self.kernels = theory.kernels
+        self.initial_config = theory.initial_config
+        self.observables_mod = theory.observables
+        self.threads2d = getattr(self.theory, "threads2d", (8, 8))
+        self.gradient_step_kernel = getattr(self.kernels, "do_gradient_step_kernel", None)
+        if self.gradient_step_kernel is None:
+            raise AttributeError("Theory kernels must provide do_gradient_step_kernel "
+                                 "(a soliton_solver_amd.core.integrator.NativeKernel).")
+        self.rk4_kernel = getattr(self.kernels, "do_rk4_kernel", None)
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        if variation not in _VAR:
+            raise ValueError(f"variation must be one of {sorted(_VAR)}")
+        self.precision = precision
+        self.variation = variation
+        self._ctx = None
+        self.set_params(params)
+        self.Field = DeviceBuffer(self, _lib.BUF_FIELD)
+        self.Velocity = DeviceBuffer(self, _lib.BUF_VELOCITY)
+        self.grid = None
+        self.last_history = None
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            _lib.lib().ss_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        _lib.check(rc, self._ctx)
+
+    # -- parameters ---------------------------------------------------------
+    def set_params(self, p) -> None:
+        """Resolve, pack and push parameters (core/simulation.py:144-166)."""
+        self.p = p
+        self.rp = p if not isinstance(p, Params) else p.resolved()
+        p_i, p_f = self.theory.params.pack_device_params(self.rp)
+        self.p_i_h = np.ascontiguousarray(p_i, dtype=np.int32)
+        self.p_f_h = np.ascontiguousarray(p_f, dtype=np.float64)
+        L = _lib.lib()
+        if self._ctx is None:
+            ctx = C.c_void_p()
+            rc = L.ss_ctx_create(int(self.kernels.THEORY_ID), self.p_i_h, self.p_i_h.size, self.p_f_h,
+                                 self.p_f_h.size, _PREC[self.precision], _VAR[self.variation], C.byref(ctx))
+            _lib.check(rc, None)
+            self._ctx = ctx
+        else:
+            self._check(L.ss_set_params(self._ctx, self.p_i_h, self.p_i_h.size, self.p_f_h, self.p_f_h.size))
+
+    def set_precision(self, precision: str) -> None:
+        """Switch storage precision in place (e.g. FP32 relaxation -> FP64 polish)."""
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        self._check(_lib.lib().ss_set_precision(self._ctx, _PREC[precision]))
+        self.precision = precision
+
+    def use_graphs(self, steps_per_graph: int) -> None:
+        """Replay ``steps_per_graph`` captured steps per hipGraph launch (0: off)."""
+        self._check(_lib.lib().ss_use_graphs(self._ctx, int(steps_per_graph)))
+
+    # -- state --------------------------------------------------------------
+    @property
+    def field_shape(self):
+        return (int(self.rp.number_total_fields), int(self.rp.xlen), int(self.rp.ylen))
+
+    def get_field(self) -> np.ndarray:
+        return self.Field.copy_to_host().reshape(self.field_shape)
+
+    def get_velocity(self) -> np.ndarray:
+        return self.Velocity.copy_to_host().reshape(self.field_shape)
+
+    def set_field(self, field) -> None:
+        self.Field.copy_to_device(field)
+
+    def set_velocity(self, velocity) -> None:
+        self.Velocity.copy_to_device(velocity)
+
+    def create_grid(self) -> np.ndarray:
+        """``grid[0] = lsx*x``, ``grid[1] = lsy*y`` (e.g. chiral_magnet/kernels.py:17-45)."""
+        X, Y = int(self.rp.xlen), int(self.rp.ylen)
+        g = np.empty((2, X, Y))
+        g[0] = self.rp.lsx * np.arange(X, dtype=np.float64)[:, None]
+        g[1] = self.rp.lsy * np.arange(Y, dtype=np.float64)[None, :]
+        return g
+
+    def initialize(self, init_config: dict | None = None) -> None:
+        """Build the initial state and upload it (core/simulation.py:117-142).
+
+        ``{"mode": "array", "field": F, "velocity": V}`` injects host arrays
+        (shape ``(nf, xlen, ylen)`` or flat); every other mode is passed to the
+        theory's ``initial_config.initialize``, which fills host arrays in place.
+        """
+        cfg = dict(init_config or {})
+        self.grid = self.create_grid()
+        F = np.zeros(self.field_shape)
+        V = np.zeros(self.field_shape)
+        mode = str(cfg.get("mode", "initial")).lower()
+        if mode == "array":
+            F[...] = np.asarray(cfg["field"], dtype=np.float64).reshape(self.field_shape)
+            if cfg.get("velocity") is not None:
+                V[...] = np.asarray(cfg["velocity"], dtype=np.float64).reshape(self.field_shape)
+        else:
+            self.initial_config.initialize(Field=F, Velocity=V, grid=self.grid, p_i_h=self.p_i_h,
+                                           p_f_h=self.p_f_h, config=cfg)
+        self.set_field(F)
+        self.set_velocity(V)
+
+    # -- the flow -----------------------------------------------------------
+    def step(self, prev_energy: float) -> tuple[float, float]:
+        """One arrested Newton flow step; returns ``(new_energy, err)``.
+
+        core/simulation.py:388-409 -> core/integrator.py:217-348.
+        """
+        e, err, _ = self.steps(prev_energy, 1)
+        return e, err
+
+    def steps(self, prev_energy: float, nsteps: int, history: bool = False):
+        """``nsteps`` steps in one native call, chained like the reference loop.
+
+        Equivalent to ``E = prev_energy; for _ in range(n): E, err = sim.step(E)``
+        (chiral_magnet/render_gl.py:663-668).  Returns ``(E, err, err_absmax)``
+        of the last step; with ``history`` the per-step energies and errs are
+        kept in ``self.last_history``.
+        """
+        e, em, ea = C.c_double(), C.c_double(), C.c_double()
+        L = _lib.lib()
+        self._check(L.ss_step(self._ctx, float(prev_energy), int(nsteps), C.byref(e), C.byref(em), C.byref(ea)))
+        if history and nsteps > 0:
+            hE = np.empty(nsteps)
+            hR = np.empty(nsteps)
+            self._check(L.ss_step_history(self._ctx, hE, hR, int(nsteps)))
+            self.last_history = (hE, hR)
+        return e.value, em.value, ea.value
+
+    def minimize(self, tol: float = 1e-6, max_steps: int = 100000, check_every: int = 100,
+                 prev_energy: float | None = None):
+        """Relax until max |EG| < tol (checked every ``check_every`` steps).
+
+        Returns ``(energy, err_absmax, steps_taken)``.
+        """
+        E = self.compute_energy() if prev_energy is None else float(prev_energy)
+        done = 0
+        ea = float("inf")
+        while done < max_steps:
+            n = min(check_every, max_steps - done)
+            E, _, ea = self.steps(E, n)
+            done += n
+            if ea < tol:
+                break
+        return E, ea, done
+
+    # -- observables --------------------------------------------------------
+    def compute_energy(self) -> float:
+        out = C.c_double()
+        self._check(_lib.lib().ss_energy(self._ctx, C.byref(out)))
+        return out.value
+
+    def compute_charge(self, kind: int) -> float:
+        out = C.c_double()
+        self._check(_lib.lib().ss_charge(self._ctx, int(kind), C.byref(out)))
+        return out.value
+
+    def observables(self) -> dict:
+        """Energy plus every charge the theory's observables module defines.
+
+        core/simulation.py:168-192 (same keys).
+        """
+        obs = {"energy": self.observables_mod.compute_energy(self)}
+        for key, fn in (("skyrmion_number", "compute_skyrmion_number"),
+                        ("vortex_number", "compute_vortex_number"),
+                        ("magnetic_charge", "compute_magnetic_charge"),
+                        ("electric_charge", "compute_electric_charge")):
+            f = getattr(self.observables_mod, fn, None)
+            if f is not None:
+                obs[key] = f(self)
+        return obs
+
+    def compute_energy_density(self) -> np.ndarray:
+        """Per-site energy times gv, shape (xlen, ylen) (core/simulation.py:194-209)."""
+        return self.density(-1)
+
+    def density(self, kind: int) -> np.ndarray:
+        out = np.empty(int(self.rp.dim_grid))
+        self._check(_lib.lib().ss_density(self._ctx, int(kind), out, out.size))
+        return out.reshape(int(self.rp.xlen), int(self.rp.ylen))
+
+    def energy_gradient(self, mode: str = "reference") -> np.ndarray:
+        """EG of the current field (the theory's gradient kernel with Temp = Field)."""
+        out = np.empty(int(self.rp.dim_fields))
+        self._check(_lib.lib().ss_variation(self._ctx, _VAR[mode], out, out.size))
+        return out.reshape(self.field_shape)
+
+    def time_passes(self, prev_energy: float, nsteps: int) -> np.ndarray:
+        """Mean ms of [pass1, energy finalize, pass2, pass3, pass4, err finalize]."""
+        out = np.zeros(6)
+        self._check(_lib.lib().ss_time_passes(self._ctx, float(prev_energy), int(nsteps), out))
+        return out
+
+    def synchronize(self) -> None:
+        self._check(_lib.lib().ss_synchronize(self._ctx))
+
+    def memory_bytes(self) -> int:
+        n = C.c_size_t()
+        self._check(_lib.lib().ss_memory_bytes(self._ctx, C.byref(n)))
+        return int(n.value)
+
+    def print_instructions(self) -> None:
+        print("Headless relaxation core: drive it with sim.step(E) / sim.steps(E, n) / sim.minimize().")

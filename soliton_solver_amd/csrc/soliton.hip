@@ -1,0 +1,701 @@
+// Context management and the extern "C" surface of libsoliton.so.
+//
+// One context owns the device state of one relaxation: ping-pong Field and
+// Velocity buffers, the three stage slopes l1..l3, per-tile partial sums, a
+// DevState block with the arrest bookkeeping, and per-step history.  The
+// theory-specific kernels are reached through a TheoryOps table.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/soliton.h"
+#include "relax_kernels.h"
+
+namespace ss {
+const TheoryOps* ops_chiral_magnet();
+const TheoryOps* ops_ginzburg_landau();
+}  // namespace ss
+
+using namespace ss;
+
+static thread_local std::string g_global_err;
+
+static const TheoryOps* theory_ops(int id) {
+  switch (id) {
+    case SS_THEORY_CHIRAL_MAGNET: return ops_chiral_magnet();
+    case SS_THEORY_GINZBURG_LANDAU: return ops_ginzburg_landau();
+    default: return nullptr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Theory-independent kernels.
+
+__global__ void k_begin_call(DevState* st, double prev_energy) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->prev_energy = prev_energy;
+    st->kill = 0;
+    st->step = 0;
+  }
+}
+
+// Sum of per-tile energy partials (fixed order: Kahan per thread, then a
+// shuffle tree), then the arrest decision of the previous step
+// (core/integrator.py:338-343): kill iff killkinen and E_new > E_prev.
+__global__ void __launch_bounds__(1024)
+k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, int killkinen) {
+  __shared__ double red[16];
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const double y = part[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  const double E = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) {
+    st->last_energy = E;
+    const int step = st->step;
+    if (step > 0) {
+      st->kill = (killkinen && E > st->prev_energy) ? 1 : 0;
+      st->prev_energy = E;
+      if (step - 1 < cap) histE[step - 1] = E;
+    } else {
+      st->kill = 0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histErr, int cap) {
+  __shared__ double red[16];
+  double m0 = -1.0e300, m1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    m0 = fmax(m0, p0[i]);
+    m1 = fmax(m1, p1[i]);
+  }
+  const double a = block_max<1024>(m0, red);
+  const double b = block_max<1024>(m1, red);
+  if (threadIdx.x == 0) {
+    st->err_max = a;
+    st->err_absmax = b;
+    const int step = st->step;
+    if (step < cap) histErr[step] = a;
+    st->step = step + 1;
+  }
+}
+
+// Plain sum of partials into out[0] (observables).
+__global__ void __launch_bounds__(1024) k_fin_sum(const double* part, int n, double* out) {
+  __shared__ double red[16];
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const double y = part[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  const double E = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) out[0] = E;
+}
+
+// Velocity kill of the last step of a call, applied in memory.
+template <typename R>
+__global__ void k_cond_zero(R* V, long long n, const DevState* st) {
+  if (st->kill == 0) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    V[i] = R(0);
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A* src, B* dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = B(src[i]);
+}
+
+// ---------------------------------------------------------------------------
+
+struct ss_ctx {
+  int theory = 0;
+  const TheoryOps* ops = nullptr;
+  int prec = 0;
+  int var_mode = 0;
+  int32_t pi[kMaxPI] = {0};
+  int ni = 0;
+  double pf[kMaxPF] = {0};
+  int nfl = 0;
+  alignas(16) unsigned char tp[512];
+  Geom g{};
+  hipStream_t stream = nullptr;
+  void* F[2] = {nullptr, nullptr};
+  void* V[2] = {nullptr, nullptr};
+  void* l[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  int ntiles = 0;
+  double* part = nullptr;  // 3 x ntiles
+  DevState* st = nullptr;
+  double* hist = nullptr;  // 2 x hist_cap
+  int hist_cap = 0;
+  int last_nsteps = 0;
+  double* dens = nullptr;
+  double* scal = nullptr;  // scalar results on device
+  DevState* host_st = nullptr;  // pinned
+  int graph_steps = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // indexed by starting parity
+  std::string err;
+};
+
+// Status codes: -1 invalid argument / unsupported request, -2 HIP runtime error.
+static int vfail(ss_ctx* c, int code, const char* fmt, va_list ap) {
+  char buf[512];
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  if (c) c->err = buf; else g_global_err = buf;
+  return code;
+}
+static int fail(ss_ctx* c, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  const int rc = vfail(c, -1, fmt, ap);
+  va_end(ap);
+  return rc;
+}
+static int hipfail(ss_ctx* c, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  const int rc = vfail(c, -2, fmt, ap);
+  va_end(ap);
+  return rc;
+}
+
+#define HIPCHK(c, expr)                                                                     \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return hipfail((c), "%s failed: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+static size_t elem_size(int prec) { return prec ? sizeof(float) : sizeof(double); }
+
+static int resolve_geom(ss_ctx* c, const int32_t* pi, int ni, const double* pf, int nfl, Geom* g) {
+  if (ni < 10 || nfl < 6) return fail(c, "packed parameters too short: need p_i[0..9], p_f[0..5] (got %d, %d)", ni, nfl);
+  if (ni > kMaxPI || nfl > kMaxPF) return fail(c, "packed parameters too long (%d ints, %d doubles)", ni, nfl);
+  g->X = pi[0];
+  g->Y = pi[1];
+  g->halo = pi[2];
+  g->nf = pi[4];
+  if (g->X < 1 || g->Y < 1) return fail(c, "xlen and ylen must be positive (got %d, %d)", g->X, g->Y);
+  if (g->halo < 2) return fail(c, "halo must be >= 2 for the 5-point stencils (got %d)", g->halo);
+  if (pi[3] != 2) return fail(c, "number_coordinates must be 2 (got %d)", pi[3]);
+  g->N = (long long)g->X * g->Y;
+  if ((long long)pi[5] != g->N) return fail(c, "dim_grid p_i[5]=%d does not equal xlen*ylen=%lld", pi[5], g->N);
+  g->lsx = pf[2];
+  g->lsy = pf[3];
+  g->gv = pf[4];
+  g->dt = pf[5];
+  if (!(g->lsx > 0.0) || !(g->lsy > 0.0)) return fail(c, "lattice spacings must be positive");
+  g->inv_lsx = 1.0 / g->lsx;
+  g->inv_lsy = 1.0 / g->lsy;
+  g->inv_lsx2 = 1.0 / (g->lsx * g->lsx);
+  g->inv_lsy2 = 1.0 / (g->lsy * g->lsy);
+  g->inv_2gv = 1.0 / (2.0 * g->gv);
+  g->inv_2lsx2 = 1.0 / (2.0 * g->lsx * g->lsx);
+  g->inv_2lsy2 = 1.0 / (2.0 * g->lsy * g->lsy);
+  g->killkinen = pi[7];
+  g->unit_mag = pi[9];
+  return 0;
+}
+
+static void free_graphs(ss_ctx* c) {
+  for (int i = 0; i < 2; ++i) {
+    if (c->graph[i]) hipGraphExecDestroy(c->graph[i]);
+    c->graph[i] = nullptr;
+  }
+}
+
+static void free_fields(ss_ctx* c) {
+  for (int i = 0; i < 2; ++i) {
+    if (c->F[i]) hipFree(c->F[i]);
+    if (c->V[i]) hipFree(c->V[i]);
+    c->F[i] = c->V[i] = nullptr;
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (c->l[i]) hipFree(c->l[i]);
+    c->l[i] = nullptr;
+  }
+}
+
+static int alloc_fields(ss_ctx* c) {
+  const size_t bytes = (size_t)c->g.nf * (size_t)c->g.N * elem_size(c->prec);
+  for (int i = 0; i < 2; ++i) {
+    HIPCHK(c, hipMalloc(&c->F[i], bytes));
+    HIPCHK(c, hipMalloc(&c->V[i], bytes));
+    HIPCHK(c, hipMemsetAsync(c->F[i], 0, bytes, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->V[i], 0, bytes, c->stream));
+  }
+  for (int i = 0; i < 3; ++i) {
+    HIPCHK(c, hipMalloc(&c->l[i], bytes));
+    HIPCHK(c, hipMemsetAsync(c->l[i], 0, bytes, c->stream));
+  }
+  c->cur = 0;
+  return 0;
+}
+
+static int alloc_partials(ss_ctx* c) {
+  const int nt = c->ops->ntiles(c->prec, c->g);
+  if (nt > c->ntiles || !c->part) {
+    if (c->part) hipFree(c->part);
+    c->part = nullptr;
+    HIPCHK(c, hipMalloc(&c->part, sizeof(double) * 3 * (size_t)nt));
+    c->ntiles = nt;
+  }
+  return 0;
+}
+
+static int ensure_hist(ss_ctx* c, int n) {
+  if (n <= c->hist_cap) return 0;
+  int cap = c->hist_cap ? c->hist_cap : 1024;
+  while (cap < n) cap *= 2;
+  if (c->hist) hipFree(c->hist);
+  c->hist = nullptr;
+  HIPCHK(c, hipMalloc(&c->hist, sizeof(double) * 2 * (size_t)cap));
+  c->hist_cap = cap;
+  free_graphs(c);  // graphs captured the old history pointer
+  return 0;
+}
+
+static PassBufs bufs_for(ss_ctx* c, int parity) {
+  PassBufs b{};
+  b.F = c->F[parity];
+  b.V = c->V[parity];
+  b.l1 = c->l[0];
+  b.l2 = c->l[1];
+  b.l3 = c->l[2];
+  b.Fout = c->F[parity ^ 1];
+  b.Vout = c->V[parity ^ 1];
+  b.st = c->st;
+  return b;
+}
+
+// Launches of one step from buffer parity ``parity``.  With ``ev`` non-null an
+// event is recorded before the first and after every launch.
+static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
+  PassBufs b = bufs_for(c, parity);
+  double* p0 = c->part;
+  double* p1 = c->part + c->ntiles;
+  double* histE = c->hist;
+  double* histErr = c->hist + c->hist_cap;
+  int e = 0;
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = c->l[0];
+  b.part0 = p0;
+  HIPCHK(c, c->ops->pass(c->prec, 1, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, c->ntiles, c->st, histE,
+                     c->hist_cap, c->g.killkinen);
+  HIPCHK(c, hipGetLastError());
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = c->l[1];
+  HIPCHK(c, c->ops->pass(c->prec, 2, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = c->l[2];
+  HIPCHK(c, c->ops->pass(c->prec, 3, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = nullptr;
+  b.part0 = p0;
+  b.part1 = p1;
+  HIPCHK(c, c->ops->pass(c->prec, 4, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1, c->ntiles,
+                     c->st, histErr, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  return 0;
+}
+
+// After the last step of a call: E(F_final), the last kill decision, and the
+// kill applied to V in memory.
+static int launch_finish(ss_ctx* c) {
+  HIPCHK(c, c->ops->density(c->prec, -1, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+                     c->hist, c->hist_cap, c->g.killkinen);
+  HIPCHK(c, hipGetLastError());
+  const long long n = (long long)c->g.nf * c->g.N;
+  if (c->prec)
+    hipLaunchKernelGGL(k_cond_zero<float>, dim3(1024), dim3(256), 0, c->stream, (float*)c->V[c->cur], n, c->st);
+  else
+    hipLaunchKernelGGL(k_cond_zero<double>, dim3(1024), dim3(256), 0, c->stream, (double*)c->V[c->cur], n, c->st);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static int build_graph(ss_ctx* c, int parity) {
+  hipGraph_t gr = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < c->graph_steps && rc == 0; ++s) rc = launch_step(c, parity ^ (s & 1), nullptr);
+  hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  if (rc) {
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return hipfail(c, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->graph[parity], gr, nullptr, nullptr, 0);
+  hipGraphDestroy(gr);
+  if (e != hipSuccess) return hipfail(c, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" {
+
+int ss_version(void) { return 100; }
+
+const char* ss_global_error(void) { return g_global_err.c_str(); }
+
+int ss_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* ss_last_error(ss_ctx* c) { return c ? c->err.c_str() : g_global_err.c_str(); }
+
+int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f, int n_f, int precision,
+                  int variation_mode, ss_ctx** out) {
+  if (!out) return fail(nullptr, "ss_ctx_create: out is NULL");
+  *out = nullptr;
+  const TheoryOps* ops = theory_ops(theory_id);
+  if (!ops) return fail(nullptr, "unknown or unbuilt theory id %d", theory_id);
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return fail(nullptr, "bad precision %d", precision);
+  if (variation_mode != SS_VAR_REFERENCE && variation_mode != SS_VAR_EXACT_ADJOINT)
+    return fail(nullptr, "bad variation mode %d", variation_mode);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return hipfail(nullptr, "no HIP device available");
+  ss_ctx* c = new ss_ctx();
+  c->theory = theory_id;
+  c->ops = ops;
+  c->prec = precision;
+  c->var_mode = variation_mode;
+  auto bail = [&](int rc) {
+    g_global_err = c->err;
+    ss_ctx_destroy(c);
+    return rc;
+  };
+  if (resolve_geom(c, p_i, n_i, p_f, n_f, &c->g)) return bail(-1);
+  if (c->g.nf != ops->nf) {
+    fail(c, "theory %s is built for %d fields (number_total_fields=%d)", ops->name, ops->nf, c->g.nf);
+    return bail(-1);
+  }
+  char msg[256] = {0};
+  if (ops->make_tp(p_i, n_i, p_f, n_f, c->tp, msg, sizeof(msg))) {
+    fail(c, "%s", msg);
+    return bail(-1);
+  }
+  memcpy(c->pi, p_i, sizeof(int32_t) * n_i);
+  c->ni = n_i;
+  memcpy(c->pf, p_f, sizeof(double) * n_f);
+  c->nfl = n_f;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    hipfail(c, "hipStreamCreate failed");
+    return bail(-2);
+  }
+  int rc = alloc_fields(c);
+  if (rc == 0) rc = alloc_partials(c);
+  if (rc) return bail(rc);
+  if (hipMalloc(&c->st, sizeof(DevState)) != hipSuccess || hipMalloc(&c->scal, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->host_st, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+    hipfail(c, "allocation of state failed");
+    return bail(-2);
+  }
+  if (hipMemsetAsync(c->st, 0, sizeof(DevState), c->stream) != hipSuccess) {
+    hipfail(c, "hipMemsetAsync of state failed");
+    return bail(-2);
+  }
+  rc = ensure_hist(c, 1024);
+  if (rc) return bail(rc);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    hipfail(c, "initial synchronize failed");
+    return bail(-2);
+  }
+  *out = c;
+  return 0;
+}
+
+int ss_ctx_destroy(ss_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_graphs(c);
+  free_fields(c);
+  if (c->part) hipFree(c->part);
+  if (c->st) hipFree(c->st);
+  if (c->hist) hipFree(c->hist);
+  if (c->dens) hipFree(c->dens);
+  if (c->scal) hipFree(c->scal);
+  if (c->host_st) hipHostFree(c->host_st);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int ss_set_params(ss_ctx* c, const int32_t* p_i, int n_i, const double* p_f, int n_f) {
+  if (!c) return fail(nullptr, "null context");
+  Geom g{};
+  if (resolve_geom(c, p_i, n_i, p_f, n_f, &g)) return -1;
+  if (g.X != c->g.X || g.Y != c->g.Y || g.nf != c->g.nf)
+    return fail(c, "set_params cannot change xlen, ylen or number_total_fields");
+  char msg[256] = {0};
+  alignas(16) unsigned char tp[512];
+  if (c->ops->make_tp(p_i, n_i, p_f, n_f, tp, msg, sizeof(msg))) return fail(c, "%s", msg);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  memcpy(c->tp, tp, sizeof(tp));
+  c->g = g;
+  memcpy(c->pi, p_i, sizeof(int32_t) * n_i);
+  c->ni = n_i;
+  memcpy(c->pf, p_f, sizeof(double) * n_f);
+  c->nfl = n_f;
+  free_graphs(c);
+  return 0;
+}
+
+int ss_set_variation_mode(ss_ctx* c, int mode) {
+  if (!c) return fail(nullptr, "null context");
+  if (mode != SS_VAR_REFERENCE && mode != SS_VAR_EXACT_ADJOINT) return fail(c, "bad variation mode %d", mode);
+  if (mode == SS_VAR_EXACT_ADJOINT) return fail(c, "exact-adjoint variation is not available for %s", c->ops->name);
+  c->var_mode = mode;
+  free_graphs(c);
+  return 0;
+}
+
+int ss_set_precision(ss_ctx* c, int prec) {
+  if (!c) return fail(nullptr, "null context");
+  if (prec != SS_PREC_FP64 && prec != SS_PREC_FP32) return fail(c, "bad precision %d", prec);
+  if (prec == c->prec) return 0;
+  const long long n = (long long)c->g.nf * c->g.N;
+  const size_t nb = (size_t)n * elem_size(prec);
+  void* nF = nullptr;
+  void* nV = nullptr;
+  HIPCHK(c, hipMalloc(&nF, nb));
+  HIPCHK(c, hipMalloc(&nV, nb));
+  if (prec == SS_PREC_FP64) {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, c->stream, (const float*)c->F[c->cur], (double*)nF, n);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, c->stream, (const float*)c->V[c->cur], (double*)nV, n);
+  } else {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, c->stream, (const double*)c->F[c->cur], (float*)nF, n);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, c->stream, (const double*)c->V[c->cur], (float*)nV, n);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_graphs(c);
+  free_fields(c);
+  c->prec = prec;
+  if (alloc_fields(c)) return -1;
+  HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->V[0], nV, nb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipFree(nF);
+  hipFree(nV);
+  c->cur = 0;
+  return alloc_partials(c);
+}
+
+static int host_copy(ss_ctx* c, int buf, double* host, size_t n, bool up) {
+  if (!c) return fail(nullptr, "null context");
+  const size_t want = (size_t)c->g.nf * (size_t)c->g.N;
+  if (n != want) return fail(c, "buffer size %zu does not equal nf*xlen*ylen = %zu", n, want);
+  if (!host) return fail(c, "null host pointer");
+  void* dev;
+  if (buf == SS_BUF_FIELD) dev = c->F[c->cur];
+  else if (buf == SS_BUF_VELOCITY) dev = c->V[c->cur];
+  else return fail(c, "unknown buffer %d", buf);
+  if (c->prec == SS_PREC_FP64) {
+    if (up) HIPCHK(c, hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    if (up) {
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+      HIPCHK(c, hipMemcpyAsync(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+      HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < n; ++i) host[i] = (double)tmp[i];
+    }
+  }
+  return 0;
+}
+
+int ss_upload(ss_ctx* c, int buf, const double* host, size_t n) { return host_copy(c, buf, (double*)host, n, true); }
+int ss_download(ss_ctx* c, int buf, double* host, size_t n) { return host_copy(c, buf, host, n, false); }
+
+int ss_use_graphs(ss_ctx* c, int steps) {
+  if (!c) return fail(nullptr, "null context");
+  if (steps < 0) return fail(c, "steps_per_graph must be >= 0");
+  free_graphs(c);
+  c->graph_steps = steps ? ((steps + 1) & ~1) : 0;
+  return 0;
+}
+
+static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, double* acc_ms) {
+  if (!c) return fail(nullptr, "null context");
+  if (nsteps < 0) return fail(c, "nsteps must be >= 0");
+  if (c->var_mode != SS_VAR_REFERENCE) return fail(c, "the fused flow runs the reference variation only");
+  if (ensure_hist(c, nsteps + 1)) return -1;
+  hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, c->stream, c->st, prev_energy);
+  HIPCHK(c, hipGetLastError());
+  int done = 0;
+  if (!ev && c->graph_steps > 0 && nsteps >= c->graph_steps) {
+    while (nsteps - done >= c->graph_steps) {
+      if (!c->graph[c->cur] && build_graph(c, c->cur)) return -1;
+      HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
+      done += c->graph_steps;  // even: parity unchanged
+    }
+  }
+  for (; done < nsteps; ++done) {
+    if (launch_step(c, c->cur, ev)) return -1;
+    if (ev) {
+      HIPCHK(c, hipEventSynchronize(ev[6]));
+      for (int k = 0; k < 6; ++k) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        acc_ms[k] += ms;
+      }
+    }
+    c->cur ^= 1;
+  }
+  if (launch_finish(c)) return -1;
+  c->last_nsteps = nsteps;
+  return 0;
+}
+
+int ss_step(ss_ctx* c, double prev_energy, int nsteps, double* e_new, double* err_max, double* err_abs) {
+  if (do_steps(c, prev_energy, nsteps, nullptr, nullptr)) return -1;
+  HIPCHK(c, hipMemcpyAsync(c->host_st, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (e_new) *e_new = c->host_st->last_energy;
+  if (err_max) *err_max = c->host_st->err_max;
+  if (err_abs) *err_abs = c->host_st->err_absmax;
+  return 0;
+}
+
+int ss_time_passes(ss_ctx* c, double prev_energy, int nsteps, double* avg_ms) {
+  if (!c) return fail(nullptr, "null context");
+  if (!avg_ms || nsteps < 1) return fail(c, "ss_time_passes needs nsteps >= 1 and an output array");
+  hipEvent_t ev[7];
+  for (int k = 0; k < 7; ++k) HIPCHK(c, hipEventCreate(&ev[k]));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int rc = do_steps(c, prev_energy, nsteps, ev, acc);
+  if (rc == 0) {
+    hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 6; ++k) avg_ms[k] = acc[k] / nsteps;
+  }
+  for (int k = 0; k < 7; ++k) hipEventDestroy(ev[k]);
+  return rc;
+}
+
+int ss_step_history(ss_ctx* c, double* energies, double* errs, int n) {
+  if (!c) return fail(nullptr, "null context");
+  if (n < 0 || n > c->last_nsteps) return fail(c, "history has %d entries (asked %d)", c->last_nsteps, n);
+  if (n == 0) return 0;
+  if (energies) HIPCHK(c, hipMemcpyAsync(energies, c->hist, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (errs) HIPCHK(c, hipMemcpyAsync(errs, c->hist + c->hist_cap, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int reduce_kind(ss_ctx* c, int kind, double* out) {
+  if (!c) return fail(nullptr, "null context");
+  if (!out) return fail(c, "null output");
+  if (kind >= c->ops->ncharges) return fail(c, "theory %s has %d charges (asked %d)", c->ops->name, c->ops->ncharges, kind);
+  HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->scal);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ss_energy(ss_ctx* c, double* e) { return reduce_kind(c, -1, e); }
+
+int ss_charge(ss_ctx* c, int kind, double* q) {
+  if (c && kind < 0) return fail(c, "charge kind must be >= 0");
+  return reduce_kind(c, kind, q);
+}
+
+int ss_num_charges(ss_ctx* c) { return c ? c->ops->ncharges : 0; }
+
+int ss_density(ss_ctx* c, int kind, double* host_out, size_t n) {
+  if (!c) return fail(nullptr, "null context");
+  if (n != (size_t)c->g.N) return fail(c, "density needs xlen*ylen = %lld values (got %zu)", c->g.N, n);
+  if (kind >= c->ops->ncharges || kind < -1) return fail(c, "bad density kind %d", kind);
+  if (!c->dens) HIPCHK(c, hipMalloc(&c->dens, sizeof(double) * (size_t)c->g.N));
+  HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], c->dens, c->part, c->tp, c->g, c->stream));
+  HIPCHK(c, hipMemcpyAsync(host_out, c->dens, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int variation_into(ss_ctx* c, const void* field, void* out, int mode, hipStream_t st) {
+  if (mode != SS_VAR_REFERENCE) return fail(c, "exact-adjoint variation is not available for %s", c->ops->name);
+  PassBufs b{};
+  b.F = field;
+  b.lout = out;
+  b.st = c->st;
+  HIPCHK(c, c->ops->pass(c->prec, 0, b, c->tp, c->g, st));
+  return 0;
+}
+
+int ss_variation(ss_ctx* c, int mode, double* host_eg, size_t n) {
+  if (!c) return fail(nullptr, "null context");
+  const size_t want = (size_t)c->g.nf * (size_t)c->g.N;
+  if (n != want) return fail(c, "variation needs nf*xlen*ylen = %zu values (got %zu)", want, n);
+  void* scratch = c->l[2];
+  if (variation_into(c, c->F[c->cur], scratch, mode, c->stream)) return -1;
+  if (c->prec == SS_PREC_FP64) {
+    HIPCHK(c, hipMemcpyAsync(host_eg, scratch, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), scratch, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i) host_eg[i] = tmp[i];
+  }
+  return 0;
+}
+
+int ss_variation_dev(ss_ctx* c, const void* field_dev, void* grad_dev, int mode, void* stream) {
+  if (!c) return fail(nullptr, "null context");
+  if (!field_dev || !grad_dev) return fail(c, "null device pointer");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  return variation_into(c, field_dev, grad_dev, mode, st);
+}
+
+int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* stream) {
+  if (!c) return fail(nullptr, "null context");
+  if (!field_dev || !energy_dev) return fail(c, "null device pointer");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, c->part, c->tp, c->g, st));
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->ntiles, energy_dev);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int ss_synchronize(ss_ctx* c) {
+  if (!c) return fail(nullptr, "null context");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ss_memory_bytes(ss_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail(c, "null argument");
+  const size_t f = (size_t)c->g.nf * (size_t)c->g.N * elem_size(c->prec);
+  *bytes = 7 * f + sizeof(double) * 3 * (size_t)c->ntiles + sizeof(double) * 2 * (size_t)c->hist_cap +
+           (c->dens ? sizeof(double) * (size_t)c->g.N : 0);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,13 @@
+"""Ginzburg-Landau observables (reference: ginzburg_landau_superconductor/observables.py:14-100).
+
+``compute_vortex_number`` keeps the reference's ``which`` argument, which the
+reference's density kernel ignores (kernels.py:145-178)."""
+from .kernels import CHARGE_KINDS
+
+
+def compute_energy(sim) -> float:
+    return sim.compute_energy()
+
+
+def compute_vortex_number(sim, which: int = 2) -> float:
+    return sim.compute_charge(CHARGE_KINDS["vortex_number"])

@@ -1,0 +1,79 @@
+"""Shared builders for the tests: parameters, seeded fields, oracle runs."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import core as ocore
+from oracle.theories import chiral_magnet as o_cm
+from oracle.theories import ginzburg_landau as o_gl
+
+ORACLES = {
+    "Chiral magnet": o_cm,
+    "Ginzburg-Landau superconductor": o_gl,
+}
+
+
+def chiral_params(theory, n=64, m=None, xsize=10.0, **kw):
+    m = m or n
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=xsize * (m - 1) / (n - 1),
+                J=40e-12, K=0.8e+6, D=4e-3, M=580e+3, B=0e-3, dmi_term="Heusler", demag=True,
+                unit_magnetization=True)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def gl_params(theory, n=48, m=None, lsx=0.2, **kw):
+    m = m or n
+    base = dict(xlen=n, ylen=m, xsize=lsx * (n - 1), ysize=lsx * (m - 1), q=1.0, Lambda=1.0, u1=1.0,
+                vortex_number=1.0, time_step=0.5 * lsx)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def packed(theory, params):
+    p_i, p_f = theory.params.pack_device_params(params.resolved())
+    return np.ascontiguousarray(p_i, dtype=np.int32), np.ascontiguousarray(p_f, dtype=np.float64)
+
+
+def grid_of(p_i, p_f):
+    X, Y = int(p_i[0]), int(p_i[1])
+    g = np.empty((2, X, Y))
+    g[0] = p_f[2] * np.arange(X)[:, None]
+    g[1] = p_f[3] * np.arange(Y)[None, :]
+    return g
+
+
+def chiral_field(theory, p_i, p_f, seed=0, kind="anti", sigma=1e-3, radius_scale=1.0, rough=False):
+    """Anti-skyrmion at the centre plus seeded Gaussian noise, unit norm; psi seeded small."""
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    cfg = {"mode": "initial", "ansatz": kind, "noise": sigma, "seed": seed}
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config=cfg)
+    rng = np.random.default_rng(seed + 1000)
+    if rough:
+        F[:3] += rng.normal(0.0, 0.3, size=F[:3].shape)
+        s = np.sqrt((F[:3] ** 2).sum(0))
+        F[:3] /= s
+    F[3] = 0.01 * rng.normal(size=F[3].shape)
+    return F
+
+
+def gl_field(theory, p_i, p_f, seed=0, amp=1e-3, velocity=False):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "vortex", "noise": amp, "seed": seed})
+    rng = np.random.default_rng(seed + 7)
+    F[0] += 0.05 * rng.normal(size=F[0].shape)
+    F[1] += 0.05 * rng.normal(size=F[1].shape)
+    if velocity:
+        V = 0.01 * rng.normal(size=F.shape)
+        return F, V
+    return F
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)

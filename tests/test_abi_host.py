@@ -1,0 +1,99 @@
+"""CPU checks of the boundary: the C-ABI library loads and exports every symbol
+include/soliton.h declares; host-side parameter packing and the registry
+behave like the reference's (theories/registry.py, core/params.py)."""
+from __future__ import annotations
+
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import soliton_solver_amd as ssa
+from soliton_solver_amd import _lib
+from soliton_solver_amd.theories import registry
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "soliton.h").read_text()
+    return sorted(set(re.findall(r"\b(ss_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (ss_\w+)", out))
+    assert set(names) <= exported
+    assert set(_lib.SIGNATURES) == set(names)
+
+
+def test_library_reports_version_and_no_crash_without_gpu():
+    L = _lib.lib()
+    assert L.ss_version() >= 100
+    assert _lib.device_count() >= 0
+
+
+def test_registry_lookup_and_aliases():
+    assert "Chiral magnet" in ssa.list_theories()
+    a = ssa.load_theory("magnetic skyrmion")
+    b = ssa.load_theory("  CHIRAL MAGNET ")
+    assert a is b
+    assert ssa.load_theory("Abelian Higgs").THEORY_SPEC.name == "Ginzburg-Landau superconductor"
+    with pytest.raises(KeyError):
+        ssa.load_theory("no such theory")
+    with pytest.raises(ValueError):
+        registry.register_theory(registry.TheorySpec(name="Chiral magnet", import_path="x"))
+    assert registry.DISCOVERY_ERRORS == {}
+
+
+def test_core_param_resolution():
+    from soliton_solver_amd.core.params import default_params, pack_device_params
+    rp = default_params(xlen=11, ylen=21, xsize=5.0, ysize=4.0, number_total_fields=3).resolved()
+    assert rp.lsx == 0.5 and rp.lsy == 0.2 and rp.grid_volume == 0.5 * 0.2
+    assert rp.time_step == 0.25  # courant * lsx, lsy plays no part
+    p_i, p_f = pack_device_params(rp)
+    assert p_i.dtype == np.int32 and p_f.dtype == np.float64
+    assert list(p_i) == [11, 21, 2, 2, 3, 231, 693, 1, 1, 0]
+    assert list(p_f) == [5.0, 4.0, 0.5, 0.2, 0.1, 0.25]
+    rp2 = default_params(lsx=0.3, time_step=0.01).resolved()
+    assert rp2.lsx == 0.3 and rp2.time_step == 0.01
+
+
+def test_chiral_params_packing_and_dmi_selector():
+    th = ssa.load_theory("Chiral magnet")
+    p = th.params.default_params(xlen=320, ylen=320, xsize=10.0, ysize=10.0, dmi_term="Heusler", demag=True,
+                                 ansatz="anti", unit_magnetization=True)
+    p_i, p_f = th.params.pack_device_params(p.resolved())
+    assert len(p_i) == 16 and len(p_f) == 15
+    assert list(p_i[10:]) == [3, 0, 0, 1, 0, 1]
+    assert abs(p_f[6] - 2.0) < 1e-12                      # coup_K = K J / D^2
+    assert abs(p_f[8] - 1.25663706127e-6 * 580e3 ** 2 * 40e-12 / 16e-6) < 1e-12
+    assert list(p_f[11:]) == [0.0, 0.0, 1.0, 0.0]         # anti ansatz flag
+    assert th.params.default_params(dmi_term=["bulk", "b20"]).resolved().dmi_dresselhaus
+    with pytest.raises(ValueError):
+        th.params.default_params(dmi_term="weird").resolved()
+    rp = th.params.default_params(D=0.0).resolved()
+    assert (rp.coup_K, rp.coup_h, rp.coup_mu) == (1.0, 1.0, 1.0)
+
+
+def test_gl_params_packing():
+    th = ssa.load_theory("Ginzburg-Landau superconductor")
+    p_i, p_f = th.params.pack_device_params(th.params.default_params(q=2.0, vortex_number=3.0).resolved())
+    assert list(p_i[10:]) == [2, 2]
+    assert list(p_f[6:]) == [2.0, 1.0, 1.0, 3.0, 1.5]     # ainf = N / q
+
+
+def test_simulation_requires_gradient_kernel():
+    import types
+    th = ssa.load_theory("Chiral magnet")
+    fake = types.SimpleNamespace(kernels=types.SimpleNamespace(), initial_config=th.initial_config,
+                                 observables=th.observables, params=th.params)
+    with pytest.raises(AttributeError):
+        ssa.Simulation(th.params.default_params(xlen=16, ylen=16), fake)

@@ -1,0 +1,205 @@
+"""HIP path vs the FP64 oracle on identical seeded inputs (chiral magnet, Ginzburg-Landau).
+
+Tolerances (FP64 form): per-stage energy gradient within 1e-11 of max|EG|;
+energies and charges within 1e-12 relative for a given field; after N steps
+energy within 1e-10 relative and max |Delta F| stated (asserted <= 1e-9).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import soliton_solver_amd as ssa
+from oracle import core as ocore
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(theory, params, F, V=None, precision="fp64"):
+    sim = ssa.Simulation(params, theory, precision=precision)
+    sim.initialize({"mode": "array", "field": F, "velocity": V})
+    return sim
+
+
+@pytest.mark.parametrize("shape", [(48, 40), (100, 150), (37, 70)])
+@pytest.mark.parametrize("rough", [False, True])
+def test_chiral_variation_energy_charges(shape, rough):
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, shape[0], shape[1])
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=3, rough=rough)
+    sim = _sim(th, p, F)
+    o = H.ORACLES["Chiral magnet"]
+    eg_gpu = sim.energy_gradient()
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F.copy(), D1, D2, p_i, p_f)
+    scale = np.abs(eg_ref).max()
+    print(f"[chiral {shape} rough={rough}] max|EG|={scale:.3e} maxdiff={np.abs(eg_gpu - eg_ref).max():.3e}")
+    assert np.abs(eg_gpu - eg_ref).max() <= 1e-11 * scale
+    E_ref = ocore.reference_sum(o.energy_density(F, D1, p_i, p_f).ravel())
+    assert H.rel(sim.compute_energy(), E_ref) < 1e-12
+    obs = sim.observables()
+    Q_ref = ocore.reference_sum(o.skyrmion_density(F, D1, p_i, p_f).ravel())
+    M_ref = ocore.reference_sum(o.magnetic_charge_density(F, D1, p_i, p_f).ravel())
+    assert abs(obs["skyrmion_number"] - Q_ref) < 1e-11
+    assert abs(obs["magnetic_charge"] - M_ref) < 1e-11 * max(1.0, np.abs(o.magnetic_charge_density(F, D1, p_i, p_f)).sum())
+    dens = sim.compute_energy_density()
+    assert np.allclose(dens, o.energy_density(F, D1, p_i, p_f), rtol=1e-12, atol=1e-14)
+    sim.close()
+
+
+@pytest.mark.parametrize("shape", [(48, 40), (66, 130)])
+def test_gl_variation_energy_charge(shape):
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, shape[0], shape[1])
+    p_i, p_f = H.packed(th, p)
+    F = H.gl_field(th, p_i, p_f, seed=5)
+    sim = _sim(th, p, F)
+    o = H.ORACLES["Ginzburg-Landau superconductor"]
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F, D1, D2, p_i, p_f)
+    eg_gpu = sim.energy_gradient()
+    scale = np.abs(eg_ref).max()
+    assert np.abs(eg_gpu - eg_ref).max() <= 1e-11 * scale
+    E_ref = ocore.reference_sum(o.energy_density(F, D1, p_i, p_f).ravel())
+    assert H.rel(sim.compute_energy(), E_ref) < 1e-12
+    N_ref = ocore.reference_sum(o.vortex_density(F, D1, p_i, p_f).ravel())
+    assert abs(sim.observables()["vortex_number"] - N_ref) < 1e-11
+    sim.close()
+
+
+def _trajectory(name, th, p, F, V, nsteps, chunks):
+    p_i, p_f = H.packed(th, p)
+    o = H.ORACLES[name]
+    Fo, Vo = F.copy(), V.copy()
+    E0 = ocore.theory_energy(o, Fo, p_i, p_f)
+    Es, errs = ocore.relax(o, Fo, Vo, p_i, p_f, nsteps, prev_energy=E0)
+    sim = _sim(th, p, F, V)
+    E0g = sim.compute_energy()
+    assert H.rel(E0g, E0) < 1e-12
+    E = E0g
+    gE, gerr = [], []
+    for n in chunks:
+        E, err, _ = sim.steps(E, n, history=True)
+        gE.extend(sim.last_history[0])
+        gerr.extend(sim.last_history[1])
+    Fg, Vg = sim.get_field(), sim.get_velocity()
+    sim.close()
+    return np.array(Es), np.array(errs), Fo, Vo, np.array(gE), np.array(gerr), Fg, Vg
+
+
+@pytest.mark.parametrize("chunks", [[1] * 12, [5, 7], [12]])
+def test_chiral_steps_match_oracle(chunks):
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 40, 52)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=11)
+    V = np.zeros_like(F)
+    n = sum(chunks)
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Chiral magnet", th, p, F, V, n, chunks)
+    dF = np.abs(Fg - Fo).max()
+    print(f"[chiral steps {chunks}] E={gE[-1]:.15e} relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={dF:.2e}")
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.allclose(gerr, errs, rtol=1e-9, atol=1e-9 * np.abs(errs).max())
+    assert dF <= 1e-9
+    assert np.abs(Vg - Vo).max() <= 1e-9 * max(1.0, np.abs(Vo).max())
+
+
+def test_gl_steps_match_oracle_with_velocity():
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, 48, 56)
+    p_i, p_f = H.packed(th, p)
+    F, V = H.gl_field(th, p_i, p_f, seed=2, velocity=True)
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Ginzburg-Landau superconductor", th, p, F, V, 30, [10, 20])
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9
+    assert np.abs(Vg - Vo).max() <= 1e-9
+
+
+def test_kill_semantics_forced_from_host():
+    """prev_energy = -inf forces a kill every step; +inf never kills."""
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, 32, 32)
+    p_i, p_f = H.packed(th, p)
+    F, V = H.gl_field(th, p_i, p_f, seed=9, velocity=True)
+    o = H.ORACLES["Ginzburg-Landau superconductor"]
+    for prev in (-np.inf, np.inf):
+        Fo, Vo = F.copy(), V.copy()
+        Eo, erro = ocore.arrested_newton_flow_step(o, Fo, Vo, p_i, p_f, prev)
+        sim = _sim(th, p, F, V)
+        Eg, errg = sim.step(prev)
+        Vg = sim.get_velocity()
+        sim.close()
+        assert H.rel(Eg, Eo) < 1e-12
+        if prev == -np.inf:
+            assert np.all(Vg == 0.0) and np.all(Vo == 0.0)
+        else:
+            assert np.abs(Vg - Vo).max() < 1e-12 and np.abs(Vo).max() > 0
+
+
+def test_killkinen_off_never_kills():
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, 32, 32, killkinen=False)
+    p_i, p_f = H.packed(th, p)
+    F, V = H.gl_field(th, p_i, p_f, seed=9, velocity=True)
+    sim = _sim(th, p, F, V)
+    sim.step(-np.inf)
+    assert np.abs(sim.get_velocity()).max() > 0
+    sim.close()
+
+
+def test_graph_replay_matches_eager():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 64, 64)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=21)
+    a = _sim(th, p, F)
+    b = _sim(th, p, F)
+    b.use_graphs(4)
+    Ea = Eb = a.compute_energy()
+    for _ in range(4):
+        Ea, erra, _ = a.steps(Ea, 10)
+        a.synchronize()
+        Eb, errb, _ = b.steps(Eb, 10)
+        b.synchronize()
+        assert Ea == Eb and erra == errb
+    assert np.array_equal(a.get_field(), b.get_field())
+    assert np.array_equal(a.get_velocity(), b.get_velocity())
+    a.close()
+    b.close()
+
+
+def test_fp32_form_tracks_fp64():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 64, 64)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=4)
+    a = _sim(th, p, F)
+    b = _sim(th, p, F, precision="fp32")
+    E0 = a.compute_energy()
+    assert H.rel(b.compute_energy(), E0) < 1e-6
+    Ea, _, _ = a.steps(E0, 50)
+    Eb, _, _ = b.steps(E0, 50)
+    print(f"[fp32 vs fp64] relE={H.rel(Eb, Ea):.2e} max|dm|={np.abs(a.get_field() - b.get_field()).max():.2e}")
+    assert H.rel(Eb, Ea) < 1e-5
+    # polish: switch to FP64 in place and keep relaxing
+    b.set_precision("fp64")
+    Eb2, _, _ = b.steps(Eb, 10)
+    assert np.isfinite(Eb2)
+    a.close()
+    b.close()
+
+
+def test_invalid_arguments_raise():
+    th = ssa.load_theory("Chiral magnet")
+    with pytest.raises(ValueError):
+        ssa.Simulation(H.chiral_params(th, 32, 32, number_total_fields=5), th)
+    sim = ssa.Simulation(H.chiral_params(th, 32, 32), th)
+    with pytest.raises(ValueError):
+        sim.set_field(np.zeros(10))
+    with pytest.raises(ValueError):
+        sim.compute_charge(7)
+    sim.close()

@@ -1,0 +1,272 @@
+"""Pins the oracle against what mathematics fixes (the reference ships no
+tests, fixtures or golden vectors for this path).
+
+Each check is chosen so that a plausible transcription mistake -- a wrong
+stencil coefficient or sign, a transposed derivative index, a dropped term in
+the variation, a wrong RK4 weight -- makes it fail.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import core as ocore
+from oracle.theories import chiral_magnet as o_cm
+from oracle.theories import ginzburg_landau as o_gl
+
+import soliton_solver_amd as ssa
+from tests import helpers as H
+
+
+def _pp(X=24, Y=20, lsx=0.3, lsy=0.3, halo=2):
+    p_i = np.array([X, Y, halo, 2, 1, X * Y, X * Y, 1, 1, 0], dtype=np.int32)
+    p_f = np.array([lsx * (X - 1), lsy * (Y - 1), lsx, lsy, lsx * lsy, 0.5 * lsx])
+    return p_i, p_f
+
+
+def _poly_field(p_i, p_f, coeffs):
+    X, Y = int(p_i[0]), int(p_i[1])
+    x = p_f[2] * np.arange(X)[:, None] + 0.37
+    y = p_f[3] * np.arange(Y)[None, :] - 0.21
+    f = np.zeros((X, Y))
+    for (a, b), c in coeffs.items():
+        f = f + c * x ** a * y ** b
+    return f[None], x, y
+
+
+@pytest.mark.parametrize("lsx,lsy", [(0.3, 0.3), (0.25, 0.4)])
+def test_first_derivative_exact_to_degree_4_and_zero_band(lsx, lsy):
+    p_i, p_f = _pp(lsx=lsx, lsy=lsy)
+    c = {(4, 0): 0.3, (3, 1): -0.2, (0, 4): 0.1, (2, 2): 0.05, (1, 0): 1.5, (0, 1): -0.7}
+    F, x, y = _poly_field(p_i, p_f, c)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    fx = 4 * 0.3 * x ** 3 + 3 * -0.2 * x ** 2 * y + 2 * 0.05 * x * y ** 2 + 1.5
+    fy = -0.2 * x ** 3 + 4 * 0.1 * y ** 3 + 2 * 0.05 * x ** 2 * y - 0.7
+    h = 2
+    X, Y = F.shape[1:]
+    assert np.allclose(D1[0, 0, h:X - h, :], fx[h:X - h, :] * np.ones((1, Y)), rtol=1e-9, atol=1e-9)
+    assert np.allclose(D1[0, 1, :, h:Y - h], (fy * np.ones((X, 1)))[:, h:Y - h], rtol=1e-9, atol=1e-9)
+    assert np.all(D1[0, 0, :h] == 0) and np.all(D1[0, 0, X - h:] == 0)
+    assert np.all(D1[0, 1, :, :h] == 0) and np.all(D1[0, 1, :, Y - h:] == 0)
+
+
+@pytest.mark.parametrize("lsx,lsy", [(0.3, 0.3), (0.25, 0.4)])
+def test_second_derivatives_exact_and_mixed_formula(lsx, lsy):
+    """xx, yy exact to degree 5; the mixed entry equals
+    f_xy + (lsx/lsy - 1) f_xx/2 + (lsy/lsx - 1) f_yy/2 (exact for polynomials
+    of degree <= 5 along the diagonal) and is 0 unless halo < x, y (strict)."""
+    p_i, p_f = _pp(lsx=lsx, lsy=lsy)
+    F, x, y = _poly_field(p_i, p_f, {(5, 0): 0.01, (0, 5): -0.02, (2, 2): 0.3, (3, 1): 0.1, (1, 1): 2.0})
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    fxx = 20 * 0.01 * x ** 3 + 2 * 0.3 * y ** 2 + 6 * 0.1 * x * y
+    fyy = 20 * -0.02 * y ** 3 + 2 * 0.3 * x ** 2
+    fxy = 4 * 0.3 * x * y + 3 * 0.1 * x ** 2 + 2.0
+    X, Y = F.shape[1:]
+    h = 2
+    fxx, fyy, fxy = (np.broadcast_to(a, (X, Y)) for a in (fxx, fyy, fxy))
+    assert np.allclose(D2[0, 0, h:X - h], fxx[h:X - h], rtol=1e-8, atol=1e-8)
+    assert np.allclose(D2[0, 1, :, h:Y - h], fyy[:, h:Y - h], rtol=1e-8, atol=1e-8)
+    mixed = fxy + (lsx / lsy - 1) * fxx / 2 + (lsy / lsx - 1) * fyy / 2
+    assert np.allclose(D2[0, 2, h + 1:X - h, h + 1:Y - h], mixed[h + 1:X - h, h + 1:Y - h], rtol=1e-8, atol=1e-8)
+    assert np.all(D2[0, 2, h, :] == 0) and np.all(D2[0, 2, :, h] == 0)   # strict lower bound
+    assert np.all(D2[0, 2, X - h:, :] == 0) and np.all(D2[0, 2, :, Y - h:] == 0)
+
+
+def test_reference_sum_matches_fsum_and_tree_order():
+    rng = np.random.default_rng(0)
+    for n in (1, 1023, 1024, 1025, 5000):
+        v = rng.normal(size=n) * 10.0 ** rng.integers(-3, 3, size=n)
+        assert abs(ocore.reference_sum(v) - math.fsum(v)) <= 1e-13 * np.abs(v).sum()
+    # tree order: pairs (i, i+512) are added first inside a block
+    v = np.zeros(1024)
+    v[0], v[512] = 1e16, 1.0
+    v[1] = -1e16
+    assert ocore.reference_sum(v) == (1e16 + 1.0) + (-1e16)
+
+
+def test_reference_max_is_signed():
+    assert ocore.reference_max(np.array([-5.0, -3.0, -4.0])) == -3.0
+    assert ocore.reference_max(np.array([])) == -1.0e300
+
+
+class _Linear:
+    """Test theory with EG = w^2 phi pointwise (no stencil)."""
+    HAS_CONSTRAINT = False
+    W2 = 3.0
+
+    @classmethod
+    def energy_density(cls, F, D1, p_i, p_f):
+        return 0.5 * cls.W2 * (F * F).sum(0) * p_f[4]
+
+    @classmethod
+    def variation(cls, F, D1, D2, p_i, p_f):
+        return cls.W2 * F
+
+
+def test_rk4_step_is_taylor_polynomial_of_exponential():
+    """One step of phi'' = -w^2 phi equals sum_{k<=4} (dt A)^k/k! applied to (phi, v)."""
+    p_i, p_f = _pp(X=8, Y=8)
+    p_i[7] = 0  # no kill
+    dt = p_f[5]
+    rng = np.random.default_rng(1)
+    F = rng.normal(size=(1, 8, 8))
+    V = rng.normal(size=(1, 8, 8))
+    F0, V0 = F.copy(), V.copy()
+    ocore.arrested_newton_flow_step(_Linear, F, V, p_i, p_f, np.inf)
+    A = np.array([[0.0, 1.0], [-_Linear.W2, 0.0]])
+    M = np.eye(2)
+    term = np.eye(2)
+    for k in range(1, 5):
+        term = term @ (dt * A) / k
+        M = M + term
+    assert np.allclose(F, M[0, 0] * F0 + M[0, 1] * V0, rtol=1e-14, atol=1e-14)
+    assert np.allclose(V, M[1, 0] * F0 + M[1, 1] * V0, rtol=1e-14, atol=1e-14)
+
+
+def test_kill_and_no_kill():
+    p_i, p_f = _pp(X=8, Y=8)
+    rng = np.random.default_rng(2)
+    F = rng.normal(size=(1, 8, 8))
+    V = rng.normal(size=(1, 8, 8))
+    ocore.arrested_newton_flow_step(_Linear, F.copy(), V, p_i, p_f, -np.inf)
+    assert np.all(V == 0)
+    V = rng.normal(size=(1, 8, 8))
+    ocore.arrested_newton_flow_step(_Linear, F.copy(), V, p_i, p_f, np.inf)
+    assert np.abs(V).max() > 0
+    p_i[7] = 0
+    ocore.arrested_newton_flow_step(_Linear, F.copy(), V, p_i, p_f, -np.inf)
+    assert np.abs(V).max() > 0
+
+
+# --- variation against the energy: directional derivatives ------------------
+
+def _bump(p_i, p_f, cx=0.5, cy=0.5, w=0.18):
+    X, Y = int(p_i[0]), int(p_i[1])
+    x = np.linspace(0, 1, X)[:, None]
+    y = np.linspace(0, 1, Y)[None, :]
+    r2 = ((x - cx) ** 2 + (y - cy) ** 2) / w ** 2
+    return np.where(r2 < 1, np.exp(-1.0 / np.maximum(1 - r2, 1e-300)), 0.0)
+
+
+def _dir_mismatch(o, F, Psi, p_i, p_f, eg_scale=1.0, comps=None):
+    eps = 1e-5
+    Ep = ocore.theory_energy(o, F + eps * Psi, p_i, p_f)
+    Em = ocore.theory_energy(o, F - eps * Psi, p_i, p_f)
+    dE = (Ep - Em) / (2 * eps)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    EG = o.variation(F.copy(), D1, D2, p_i, p_f)
+    sel = comps if comps is not None else range(F.shape[0])
+    G = sum((EG[a] * Psi[a]).sum() for a in sel) * p_f[4] * eg_scale
+    return abs(G - dE) / abs(dE)
+
+
+def test_chiral_variation_is_functional_derivative_of_energy():
+    """sum gv EG.psi = dE_h(phi + e psi)/de + O(h^4) for smooth psi tangent to m
+    (psi component: EG_psi = -coup_mu dE/dpsi, the reference's saddle form)."""
+    th = ssa.load_theory("Chiral magnet")
+    errs = []
+    for n in (61, 121):
+        p = H.chiral_params(th, n, n, xsize=4.0, demag=True)
+        p_i, p_f = H.packed(th, p)
+        g = H.grid_of(p_i, p_f)
+        F = np.zeros((4, n, n))
+        F[0] = 0.6 * np.sin(1.3 * g[0] + 0.4 * g[1])
+        F[1] = 0.5 * np.cos(0.7 * g[1] - 0.9 * g[0])
+        F[2] = 1.0
+        F[:3] /= np.sqrt((F[:3] ** 2).sum(0))
+        b = _bump(p_i, p_f, w=0.35)
+        F[3] = 0.3 * b
+        w = np.stack([b * np.cos(3.0 * np.linspace(0, 1, n))[:, None] * (1 + 0 * b), b * 0.5, b * 0.2])
+        w = w - (w * F[:3]).sum(0) * F[:3]  # tangent to m
+        Psi = np.zeros_like(F)
+        Psi[:3] = w
+        e_m = _dir_mismatch(o_cm, F, Psi, p_i, p_f, comps=range(3))
+        Psi2 = np.zeros_like(F)
+        Psi2[3] = b * (1 + 0.5 * np.sin(np.linspace(0, 3, n))[None, :])
+        e_psi = _dir_mismatch(o_cm, F, Psi2, p_i, p_f, eg_scale=-1.0 / p_f[8], comps=[3])
+        errs.append((e_m, e_psi))
+    print("chiral directional mismatch (h, h/2):", errs)
+    for k in range(2):
+        assert errs[1][k] < 1e-3
+        assert errs[0][k] / errs[1][k] > 8.0   # ~16x for a 4th-order scheme
+
+
+def test_gl_variation_is_functional_derivative_of_energy():
+    th = ssa.load_theory("Abelian Higgs")
+    errs = []
+    for n in (61, 121):
+        lsx = 8.0 / (n - 1)
+        p = H.gl_params(th, n, n, lsx=lsx, Lambda=1.3, q=0.8, u1=1.0, vortex_number=1.0)
+        p_i, p_f = H.packed(th, p)
+        F = np.zeros((4, n, n))
+        V = np.zeros_like(F)
+        th.initial_config.initialize(Field=F, Velocity=V, grid=H.grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "vortex", "scale": 1.0})
+        b = _bump(p_i, p_f, w=0.3)
+        F[0] += 0.2 * b
+        F[1] -= 0.1 * b
+        Psi = np.stack([b, 0.5 * b, -0.3 * b, 0.7 * b])
+        errs.append(_dir_mismatch(o_gl, F, Psi, p_i, p_f))
+    print("GL directional mismatch (h, h/2):", errs)
+    assert errs[1] < 5e-4
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_gl_energy_reads_vortex_number_not_u1():
+    """Quirk kept from the reference: energy uses p_f[9], variation p_f[8]."""
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, 24, 24, u1=1.0, vortex_number=2.0)
+    p_i, p_f = H.packed(th, p)
+    F = H.gl_field(th, p_i, p_f)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    e2 = o_gl.energy_density(F, D1, p_i, p_f)
+    p_f1 = p_f.copy()
+    p_f1[8] = 7.0  # u1 must not matter for the energy
+    assert np.array_equal(e2, o_gl.energy_density(F, D1, p_i, p_f1))
+
+
+def test_skyrmion_number_of_ansatz():
+    """Q of the reference ansatz: Bloch and Neel -1, anti +1 under the
+    reference's density m.(d_x m x d_y m)/(4 pi)."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 161, 161, xsize=10.0)
+    p_i, p_f = H.packed(th, p)
+    for kind, sign in (("bloch", -1), ("neel", -1), ("anti", 1)):
+        F = np.zeros((4, 161, 161))
+        th.initial_config.initialize(Field=F, Velocity=np.zeros_like(F), grid=H.grid_of(p_i, p_f), p_i_h=p_i,
+                                     p_f_h=p_f, config={"mode": "initial", "ansatz": kind})
+        D1 = ocore.first_derivatives(F, p_i, p_f)
+        Q = ocore.reference_sum(o_cm.skyrmion_density(F, D1, p_i, p_f))
+        assert abs(Q - sign) < 2e-3, (kind, Q)
+
+
+def test_unit_norm_and_projection_invariants_after_step():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 32, 36)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=1, rough=True)
+    V = np.zeros_like(F)
+    ocore.relax(o_cm, F, V, p_i, p_f, 3)
+    assert np.allclose((F[:3] ** 2).sum(0), 1.0, atol=1e-14)
+    assert np.abs((V[:3] * F[:3]).sum(0)).max() < 1e-14
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    EG = o_cm.variation(F, D1, D2, p_i, p_f)
+    assert np.abs((EG[:3] * F[:3]).sum(0)).max() < 1e-10 * np.abs(EG).max()
+    assert np.all(EG[:, ocore.band_mask(p_i)] == 0)
+
+
+def test_ground_state_is_stationary():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 24, 24)
+    p_i, p_f = H.packed(th, p)
+    F = np.zeros((4, 24, 24))
+    F[2] = 1.0
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    # the reference's stencil order leaves ~1e-16 round-off on a constant field
+    assert np.abs(o_cm.variation(F, D1, D2, p_i, p_f)).max() < 1e-14
+    assert abs(ocore.theory_energy(o_cm, F, p_i, p_f)) < 1e-14
