@@ -19,6 +19,7 @@
 namespace ss {
 const TheoryOps* ops_chiral_magnet();
 const TheoryOps* ops_ginzburg_landau();
+const TheoryOps* ops_baby_skyrme();
 }  // namespace ss
 
 using namespace ss;
@@ -29,6 +30,7 @@ static const TheoryOps* theory_ops(int id) {
   switch (id) {
     case SS_THEORY_CHIRAL_MAGNET: return ops_chiral_magnet();
     case SS_THEORY_GINZBURG_LANDAU: return ops_ginzburg_landau();
+    case SS_THEORY_BABY_SKYRME: return ops_baby_skyrme();
     default: return nullptr;
   }
 }
