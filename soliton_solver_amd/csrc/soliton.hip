@@ -20,6 +20,7 @@ namespace ss {
 const TheoryOps* ops_chiral_magnet();
 const TheoryOps* ops_ginzburg_landau();
 const TheoryOps* ops_baby_skyrme();
+const TheoryOps* ops_anisotropic_sc();
 }  // namespace ss
 
 using namespace ss;
@@ -31,6 +32,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_CHIRAL_MAGNET: return ops_chiral_magnet();
     case SS_THEORY_GINZBURG_LANDAU: return ops_ginzburg_landau();
     case SS_THEORY_BABY_SKYRME: return ops_baby_skyrme();
+    case SS_THEORY_ANISOTROPIC_SC: return ops_anisotropic_sc();
     default: return nullptr;
   }
 }

@@ -4,12 +4,16 @@ from __future__ import annotations
 import numpy as np
 
 from oracle import core as ocore
+from oracle.theories import anisotropic_superconductor as o_as
+from oracle.theories import baby_skyrme as o_bs
 from oracle.theories import chiral_magnet as o_cm
 from oracle.theories import ginzburg_landau as o_gl
 
 ORACLES = {
     "Chiral magnet": o_cm,
     "Ginzburg-Landau superconductor": o_gl,
+    "Baby Skyrme model": o_bs,
+    "Anisotropic superconductor": o_as,
 }
 
 
@@ -77,3 +81,44 @@ def gl_field(theory, p_i, p_f, seed=0, amp=1e-3, velocity=False):
 
 def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
+
+
+def bs_params(theory, n=48, m=None, xsize=20.0, **kw):
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), mpi=1.0, kappa=1.0, unit_magnetization=True,
+                time_step=0.0957 * h)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def bs_field(theory, p_i, p_f, seed=0, k=1, amp=1e-3, kind="neel"):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "initial", "ansatz": kind, "skyrmion_number": k,
+                                             "noise": amp, "seed": seed})
+    return F
+
+
+def as_params(theory, n=48, m=None, xsize=25.0, seed=0, **kw):
+    """Config-5 style: alpha1 = -1, alpha2 = -0.5, beta1..beta4 ~ U[0.2, 1] from the seed,
+    gamma1 = gamma2 = 1, gamma12 = 0.3."""
+    m = m or n
+    h = xsize / (n - 1)
+    rng = np.random.default_rng(seed)
+    b = rng.uniform(0.2, 1.0, size=4)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), alpha1=-1.0, alpha2=-0.5, beta1=b[0], beta2=b[1],
+                beta3=b[2], beta4=b[3], gamma1=1.0, gamma2=1.0, gamma12=0.3, Q=1.0, kappa=0.673274)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def as_field(theory, p_i, p_f, seed=0, amp=1e-2, mode="initial"):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": mode, "noise": amp, "seed": seed})
+    return F

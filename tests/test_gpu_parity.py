@@ -203,3 +203,70 @@ def test_invalid_arguments_raise():
     with pytest.raises(ValueError):
         sim.compute_charge(7)
     sim.close()
+
+
+@pytest.mark.parametrize("pot", ["standard", "holomorphic", "easy plane", "dihedral2", "aloof", "dihedral3",
+                                 "broken", "double vacua"])
+def test_baby_skyrme_variation_energy_charge(pot):
+    th = ssa.load_theory("Baby Skyrme model")
+    p = H.bs_params(th, 70, 90, kappa=1.3, mpi=0.7, potential=pot, N=3)
+    p_i, p_f = H.packed(th, p)
+    F = H.bs_field(th, p_i, p_f, seed=4, k=2, amp=1e-2)
+    sim = _sim(th, p, F)
+    o = H.ORACLES["Baby Skyrme model"]
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F.copy(), D1, D2, p_i, p_f)
+    eg_gpu = sim.energy_gradient()
+    scale = np.abs(eg_ref).max()
+    print(f"[bs {pot}] max|EG|={scale:.3e} maxdiff={np.abs(eg_gpu - eg_ref).max():.3e}")
+    assert np.abs(eg_gpu - eg_ref).max() <= 1e-11 * scale
+    E_ref = ocore.reference_sum(o.energy_density(F, D1, p_i, p_f).ravel())
+    assert H.rel(sim.compute_energy(), E_ref) < 1e-12
+    Q_ref = ocore.reference_sum(o.skyrmion_density(F, D1, p_i, p_f).ravel())
+    assert abs(sim.observables()["skyrmion_number"] - Q_ref) < 1e-11
+    sim.close()
+
+
+def test_baby_skyrme_steps_match_oracle():
+    th = ssa.load_theory("Baby Skyrme model")
+    p = H.bs_params(th, 64, 64, kappa=1.0, mpi=0.5)
+    p_i, p_f = H.packed(th, p)
+    F = H.bs_field(th, p_i, p_f, seed=8, k=2)
+    V = np.zeros_like(F)
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Baby Skyrme model", th, p, F, V, 25, [25])
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_anisotropic_variation_energy_charge(seed):
+    th = ssa.load_theory("Anisotropic superconductor")
+    p = H.as_params(th, 72, 60, seed=seed)
+    p_i, p_f = H.packed(th, p)
+    F = H.as_field(th, p_i, p_f, seed=seed, amp=5e-2)
+    sim = _sim(th, p, F)
+    o = H.ORACLES["Anisotropic superconductor"]
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F, D1, D2, p_i, p_f)
+    eg_gpu = sim.energy_gradient()
+    scale = np.abs(eg_ref).max()
+    print(f"[as seed={seed}] max|EG|={scale:.3e} maxdiff={np.abs(eg_gpu - eg_ref).max():.3e}")
+    assert np.abs(eg_gpu - eg_ref).max() <= 1e-11 * scale
+    E_ref = ocore.reference_sum(o.energy_density(F, D1, p_i, p_f).ravel())
+    assert abs(sim.compute_energy() - E_ref) <= 1e-12 * np.abs(o.energy_density(F, D1, p_i, p_f)).sum()
+    N_ref = ocore.reference_sum(o.vortex_density(F, D1, p_i, p_f).ravel())
+    assert abs(sim.observables()["vortex_number"] - N_ref) < 1e-11
+    sim.close()
+
+
+def test_anisotropic_steps_match_oracle():
+    th = ssa.load_theory("Anisotropic superconductor")
+    p = H.as_params(th, 48, 52, seed=2)
+    p_i, p_f = H.packed(th, p)
+    F = H.as_field(th, p_i, p_f, seed=2)
+    V = np.zeros_like(F)
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Anisotropic superconductor", th, p, F, V, 20, [20])
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9

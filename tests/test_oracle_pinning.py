@@ -270,3 +270,103 @@ def test_ground_state_is_stationary():
     # the reference's stencil order leaves ~1e-16 round-off on a constant field
     assert np.abs(o_cm.variation(F, D1, D2, p_i, p_f)).max() < 1e-14
     assert abs(ocore.theory_energy(o_cm, F, p_i, p_f)) < 1e-14
+
+
+_BS_POTENTIALS = ["standard", "holomorphic", "easy plane", "dihedral2", "aloof", "dihedral3", "broken",
+                  "double vacua"]
+
+
+@pytest.mark.parametrize("pot", _BS_POTENTIALS)
+def test_baby_skyrme_variation_is_functional_derivative(pot):
+    """Skyrme-term Euler-Lagrange expression and every potential derivative agree
+    with dE_h along smooth tangent directions, converging at 4th order."""
+    from oracle.theories import baby_skyrme as o_bs
+    th = ssa.load_theory("Baby Skyrme model")
+    errs = []
+    for n in (61, 121):
+        p = H.bs_params(th, n, n, xsize=4.0, kappa=0.9, mpi=0.8, potential=pot, N=2)
+        p_i, p_f = H.packed(th, p)
+        g = H.grid_of(p_i, p_f)
+        F = np.zeros((3, n, n))
+        F[0] = 0.6 * np.sin(1.3 * g[0] + 0.4 * g[1])
+        F[1] = 0.5 * np.cos(0.7 * g[1] - 0.9 * g[0])
+        F[2] = 0.4 + 0.2 * np.sin(g[0])
+        F /= np.sqrt((F ** 2).sum(0))
+        b = _bump(p_i, p_f, w=0.35)
+        w = np.stack([b * np.cos(1.5 * g[1]), b * 0.5, b * (0.2 + 0.1 * g[0])])
+        w = w - (w * F).sum(0) * F
+        errs.append(_dir_mismatch(o_bs, F, w, p_i, p_f))
+    print(pot, "baby Skyrme directional mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def _as_smooth(n, lsx):
+    x = lsx * np.arange(n)[:, None]
+    y = lsx * np.arange(n)[None, :]
+    F = np.zeros((6, n, n))
+    F[0] = 0.3 * np.sin(0.7 * x + 0.2 * y)
+    F[1] = 0.2 * np.cos(0.5 * y - 0.3 * x)
+    F[2] = 0.6 + 0.2 * np.sin(x)
+    F[3] = 0.3 * np.cos(y)
+    F[4] = 0.5 * np.sin(0.4 * x + 0.3 * y)
+    F[5] = 0.4 + 0.1 * np.cos(x * y / 5)
+    return F, x
+
+
+@pytest.mark.parametrize("part", ["gauge", "kinetic"])
+def test_anisotropic_variation_matches_energy_without_potential(part):
+    """Gauge components: EG = dE_h/dA at 4th order.  Order-parameter components with
+    the potential switched off: EG = dE_h/dpsi at 4th order."""
+    from oracle.theories import anisotropic_superconductor as o_as
+    th = ssa.load_theory("Anisotropic superconductor")
+    errs = []
+    for n in (61, 121):
+        lsx = 6.0 / (n - 1)
+        kw = dict(xsize=6.0)
+        if part == "kinetic":
+            kw.update(alpha1=0.0, alpha2=0.0, beta1=0.0, beta2=0.0, beta3=0.0, beta4=0.0)
+        p = H.as_params(th, n, n, **kw)
+        p_i, p_f = H.packed(th, p)
+        F, x = _as_smooth(n, lsx)
+        b = _bump(p_i, p_f, w=0.35)
+        Psi = np.zeros_like(F)
+        comps = [0, 1] if part == "gauge" else [2, 3, 4, 5]
+        for c in comps:
+            Psi[c] = b * (1 + 0.3 * c * np.sin(x + c))
+        errs.append(_dir_mismatch(o_as, F, Psi, p_i, p_f))
+    print(part, "anisotropic directional mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_anisotropic_potential_term_is_twice_its_derivative():
+    """Reference quirk kept: with the anisotropy tensor switched off (no kinetic
+    term for psi) the order-parameter variation is exactly 2 dV/dpsi."""
+    from oracle.theories import anisotropic_superconductor as o_as
+    th = ssa.load_theory("Anisotropic superconductor")
+    n = 41
+    p = H.as_params(th, n, n, xsize=6.0, gamma1=0.0, gamma2=0.0, gamma12=0.0, seed=3)
+    p_i, p_f = H.packed(th, p)
+    F, x = _as_smooth(n, 6.0 / (n - 1))
+    b = _bump(p_i, p_f, w=0.35)
+    Psi = np.zeros_like(F)
+    Psi[2:] = b
+    eps = 1e-5
+    dE = (ocore.theory_energy(o_as, F + eps * Psi, p_i, p_f) - ocore.theory_energy(o_as, F - eps * Psi, p_i, p_f)) / (2 * eps)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    EG = o_as.variation(F, D1, D2, p_i, p_f)
+    G = (EG * Psi).sum() * p_f[4]
+    assert abs(G - 2.0 * dE) < 1e-8 * abs(dE)
+
+
+def test_anisotropic_vacuum_has_zero_energy():
+    """The subtracted constant makes the uniform vacuum psi1 = u1, psi2 = u2 e^{i theta} energy-free."""
+    from oracle.theories import anisotropic_superconductor as o_as
+    th = ssa.load_theory("Anisotropic superconductor")
+    p = H.as_params(th, 24, 24, seed=1)
+    p_i, p_f = H.packed(th, p)
+    F = H.as_field(th, p_i, p_f, amp=0.0, mode="ground")
+    E = ocore.theory_energy(o_as, F, p_i, p_f)
+    assert abs(E) < 1e-12 * 24 * 24
