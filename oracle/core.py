@@ -35,7 +35,7 @@ def first_derivatives(F, p_i, p_f):
     X, Y, h = _dims(p_i)
     lsx, lsy = float(p_f[2]), float(p_f[3])
     nf = F.shape[0]
-    D1 = np.zeros((nf, 2, X, Y))
+    D1 = F.new_zeros((nf, 2, X, Y)) if hasattr(F, "new_zeros") else np.zeros((nf, 2, X, Y))
     xs = slice(h, X - h)
     ys = slice(h, Y - h)
     if X - h > h:

@@ -58,7 +58,7 @@ def energy_density(F, D1, p_i, p_f):
     F_pot = F_pot + b1 * s1 * s1 + b2 * s2 * s2 + b3 * s1 * s2
     F_pot = F_pot + 2.0 * b4 * (p1r * p1r * p2r * p2r + p1i * p1i * p2i * p2i - p1i * p1i * p2r * p2r
                                 - p1r * p1r * p2i * p2i + 4.0 * p1r * p1i * p2r * p2i)
-    F_kin = np.zeros_like(p1r)
+    F_kin = 0.0 * p1r
     for i in range(2):
         Ai = A[i]
         for j in range(2):

@@ -47,7 +47,7 @@ def energy_density(F, D1, p_i, p_f):
     coup_K, coup_h, coup_mu = float(p_f[6]), float(p_f[7]), float(p_f[8])
     nc, nmag, demag = int(p_i[3]), int(p_i[10]), int(p_i[15])
     D00, D01, D10, D11 = dmi_matrix(p_i)
-    e = np.zeros(F.shape[1:])
+    e = 0.0 * F[0]
     for a in range(nmag):
         for i in range(nc):
             d = D1[a, i]

@@ -30,7 +30,7 @@ def energy_density(F, D1, p_i, p_f):
     dA2dx = D1[1, 0]
     psi_sq = psi1 * psi1 + psi2 * psi2
     A_sq = A1 * A1 + A2 * A2
-    e = np.zeros(F.shape[1:])
+    e = 0.0 * F[0]
     e = e + 0.5 * (dpsi1dx * dpsi1dx + dpsi1dy * dpsi1dy + dpsi2dx * dpsi2dx + dpsi2dy * dpsi2dy)
     e = e + q * A1 * (psi1 * dpsi2dx - psi2 * dpsi1dx)
     e = e + q * A2 * (psi1 * dpsi2dy - psi2 * dpsi1dy)

@@ -24,6 +24,7 @@
 #pragma once
 
 #include "common.h"
+#include "dual.h"
 #include "stencil.h"
 
 namespace ss {
@@ -197,6 +198,166 @@ density_pass(const void* Fv, double* dens, double* part, int kind, typename TH::
   if (threadIdx.x == 0) part[t] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Exact adjoint of the discrete energy E_h = sum_s gv e(f_s, D1 f_s).
+//
+// dE_h/df_a(t) = gv de/df_a(t) + sum_i sum_k w_k P_ai(t - k e_i), with
+// P_ai(s) = gv de/d(D1_i f_a)(s) / h_i where the site's D1_i is the stencil
+// (0 in the band, where D1 is the constant 0) and w = (1/12, -2/3, 0, 2/3,
+// -1/12) the first-derivative weights at offsets -2..2.  Pass A evaluates the
+// local partials with one-direction dual numbers through the theory's own
+// energy(); pass B applies the transposed stencil.
+
+template <class TH>
+__global__ void __launch_bounds__(kThreads)
+adjoint_a(const double* Fv, double* G, double* Px, double* Py, typename TH::TP tp, Geom g, Launch L) {
+  using S = TileShape<double, TH::NF>;
+  constexpr int NF = TH::NF;
+  __shared__ Tile<double, NF> tile;
+  const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
+  if (t < 0) return;
+  const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
+  const int x0 = tx * S::TX, y0 = ty * TY;
+  load_tile<double, NF>(tile, g, x0, y0, [&](int a, long long off) -> double { return Fv[off]; });
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < S::kRowsPerWave; ++k) {
+    const int lx = w * S::kRowsPerWave + k, ly = lane;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= g.X || y >= g.Y) continue;
+    const SiteInfo s = site_info(g, x, y);
+    double f[NF], d1[NF][2];
+    tile_d1<double, double, NF>(tile, lx, ly, g, s, f, d1);
+    const long long off = (long long)x * g.Y + y;
+    for (int dir = 0; dir < 3 * NF; ++dir) {
+      Dual<1> fd[NF], dd[NF][2];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        fd[a] = Dual<1>(f[a]);
+        dd[a][0] = Dual<1>(d1[a][0]);
+        dd[a][1] = Dual<1>(d1[a][1]);
+        if (dir == a) fd[a].d[0] = 1.0;
+        if (dir == NF + a) dd[a][0].d[0] = 1.0;
+        if (dir == 2 * NF + a) dd[a][1].d[0] = 1.0;
+      }
+      const double de = TH::template energy<Dual<1>>(tp, g, s, fd, dd).d[0] * g.gv;
+      const int a = dir % NF;
+      const long long o = off + (long long)a * g.N;
+      if (dir < NF) G[o] = de;
+      else if (dir < 2 * NF) Px[o] = s.inx ? de * g.inv_lsx : 0.0;
+      else Py[o] = s.iny ? de * g.inv_lsy : 0.0;
+    }
+  }
+}
+
+// Pass B: G += transposed stencil of (Px, Py).  With ``flow`` the theory's
+// flow convention is applied against the stage field Tv (tangent projection,
+// saddle-field sign) and l = -dt EG is written to lout.
+template <class TH, typename R>
+__global__ void __launch_bounds__(kThreads)
+adjoint_b(double* G, const double* Px, const double* Py, const void* Tv, void* lout, int flow,
+          typename TH::TP tp, Geom g) {
+  constexpr int NF = TH::NF;
+  const long long n = g.N;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / g.Y), y = (int)(i - (long long)x * g.Y);
+    double eg[NF];
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const double* px = Px + (long long)a * n;
+      const double* py = Py + (long long)a * n;
+      double acc = 0.0;
+      if (x + 2 < g.X) acc += SS_C12 * px[i + 2 * (long long)g.Y];
+      if (x + 1 < g.X) acc -= SS_C23 * px[i + g.Y];
+      if (x - 1 >= 0) acc += SS_C23 * px[i - g.Y];
+      if (x - 2 >= 0) acc -= SS_C12 * px[i - 2 * (long long)g.Y];
+      if (y + 2 < g.Y) acc += SS_C12 * py[i + 2];
+      if (y + 1 < g.Y) acc -= SS_C23 * py[i + 1];
+      if (y - 1 >= 0) acc += SS_C23 * py[i - 1];
+      if (y - 2 >= 0) acc -= SS_C12 * py[i - 2];
+      eg[a] = G[i + (long long)a * n] + acc;
+    }
+    if (flow) {
+      // flow units: EG = (1/gv) dE_h/dphi, the scale of the reference's variation
+      const double igv = 1.0 / g.gv;
+      const R* T = (const R*)Tv;
+      double f[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        f[a] = (double)T[i + (long long)a * n];
+        eg[a] *= igv;
+      }
+      TH::template flow_fix<double>(tp, f, eg);
+      R* l = (R*)lout;
+#pragma unroll
+      for (int a = 0; a < NF; ++a) l[i + (long long)a * n] = R(-g.dt * eg[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < NF; ++a) G[i + (long long)a * n] = eg[a];
+  }
+}
+
+// Unfused-flow helpers (used by the exact-adjoint flow).
+// Stage field Temp = F + ca dt (V + cb lprev), V read as 0 when killed.
+template <typename R>
+__global__ void stage_field(R* Temp, const R* F, const R* V, const R* lp, double ca, double cb, int use_kill,
+                            const DevState* st, Geom g) {
+  const long long n = (long long)g.nf * g.N;
+  const bool kill = use_kill && st->kill;
+  const R dt = R(g.dt), a = R(ca), b = R(cb);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    if (ca == 0.0) {
+      Temp[i] = F[i];
+    } else {
+      const R v = kill ? R(0) : V[i];
+      Temp[i] = F[i] + a * (dt * (v + (cb == 0.0 ? R(0) : b * lp[i])));
+    }
+  }
+}
+
+// RK4 combine from stored l1..l3 and the stage-4 gradient EG4 (FP64).
+template <class TH, typename R>
+__global__ void __launch_bounds__(kThreads)
+combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, const double* EG4, R* Fo, R* Vo,
+             double* part0, double* part1, const DevState* st, Geom g) {
+  constexpr int NF = TH::NF;
+  __shared__ double red[kWaves];
+  const bool kill = st->kill != 0;
+  const R dt = R(g.dt), half = R(0.5), sixth = R(1.0 / 6.0);
+  double m0 = -1.0e300, m1 = 0.0;
+  const long long n = g.N;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) {
+    R Fn[NF], Vn[NF];
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const long long o = i + (long long)a * n;
+      const R Vc = kill ? R(0) : V[o];
+      const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
+      const double e = EG4[o];
+      const R a4 = R(-g.dt * e);
+      const R k1 = dt * Vc, k2 = dt * (Vc + half * a1), k3 = dt * (Vc + half * a2), k4 = dt * (Vc + a3);
+      Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
+      Fn[a] = F[o] + sixth * (k1 + R(2) * k2 + R(2) * k3 + k4);
+      m0 = fmax(m0, e);
+      m1 = fmax(m1, fabs(e));
+    }
+    if (TH::kUnitConstraint && g.unit_mag) {
+      TH::template normalize<R>(Fn);
+      TH::template project<R>(Vn, Fn);
+    }
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      Fo[i + (long long)a * n] = Fn[a];
+      Vo[i + (long long)a * n] = Vn[a];
+    }
+  }
+  const double b0 = block_max<kThreads>(m0, red);
+  if (threadIdx.x == 0) part0[blockIdx.x] = b0;
+  const double b1 = block_max<kThreads>(m1, red);
+  if (threadIdx.x == 0) part1[blockIdx.x] = b1;
+}
+
 // Host-side launchers for one theory.  ``prec`` 0 = FP64 storage, 1 = FP32.
 template <class TH>
 struct TheoryLaunchers {
@@ -239,6 +400,42 @@ struct TheoryLaunchers {
                 : density_r<double>(kind, F, dens, part, *(const TP*)tp, g, st);
   }
 
+  // Exact adjoint of the energy of Fv (FP64 storage) into G, scratch Px, Py
+  // (nf*N doubles each).  With ``flow`` (storage precision ``prec``) also the
+  // flow convention against Tv and l = -dt EG into lout.
+  static hipError_t adjoint(int prec, const void* Fv, double* G, double* Px, double* Py, const void* Tv, void* lout,
+                            int flow, const void* tpv, const Geom& g, hipStream_t st) {
+    const TP& tp = *(const TP*)tpv;
+    const Launch L = make_launch<double, TH::NF>(g);
+    hipLaunchKernelGGL((adjoint_a<TH>), dim3(L.grid), dim3(kThreads), 0, st, (const double*)Fv, G, Px, Py, tp, g, L);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nb = (int)((g.N + kThreads - 1) / kThreads);
+    const dim3 grid(nb < 4096 ? nb : 4096);
+    if (prec)
+      hipLaunchKernelGGL((adjoint_b<TH, float>), grid, dim3(kThreads), 0, st, G, (const double*)Px,
+                         (const double*)Py, Tv, lout, flow, tp, g);
+    else
+      hipLaunchKernelGGL((adjoint_b<TH, double>), grid, dim3(kThreads), 0, st, G, (const double*)Px,
+                         (const double*)Py, Tv, lout, flow, tp, g);
+    return hipGetLastError();
+  }
+
+  static hipError_t combine(int prec, const PassBufs& b, const double* EG4, const Geom& g, hipStream_t st,
+                            int* nblocks) {
+    const int nb = (int)((g.N + kThreads - 1) / kThreads);
+    *nblocks = nb;
+    if (prec)
+      hipLaunchKernelGGL((combine_pass<TH, float>), dim3(nb), dim3(kThreads), 0, st, (const float*)b.F,
+                         (const float*)b.V, (const float*)b.l1, (const float*)b.l2, (const float*)b.l3, EG4,
+                         (float*)b.Fout, (float*)b.Vout, b.part0, b.part1, b.st, g);
+    else
+      hipLaunchKernelGGL((combine_pass<TH, double>), dim3(nb), dim3(kThreads), 0, st, (const double*)b.F,
+                         (const double*)b.V, (const double*)b.l1, (const double*)b.l2, (const double*)b.l3, EG4,
+                         (double*)b.Fout, (double*)b.Vout, b.part0, b.part1, b.st, g);
+    return hipGetLastError();
+  }
+
   static int make_tp(const int32_t* pi, int ni, const double* pf, int nfl, void* tp, char* msg, int cap) {
     static_assert(sizeof(TP) <= 512, "theory parameter block too large");
     return TH::make(pi, ni, pf, nfl, (TP*)tp, msg, cap);
@@ -254,6 +451,9 @@ struct TheoryOps {
   int (*ntiles)(int, const Geom&);
   hipError_t (*pass)(int, int, const PassBufs&, const void*, const Geom&, hipStream_t);
   hipError_t (*density)(int, int, const void*, double*, double*, const void*, const Geom&, hipStream_t);
+  hipError_t (*adjoint)(int, const void*, double*, double*, double*, const void*, void*, int, const void*,
+                        const Geom&, hipStream_t);
+  hipError_t (*combine)(int, const PassBufs&, const double*, const Geom&, hipStream_t, int*);
 };
 
 #define SS_DEFINE_THEORY_OPS(TH, fn)                                                        \
@@ -262,7 +462,9 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::make_tp,                    \
                                       &ss::TheoryLaunchers<TH>::ntiles,                     \
                                       &ss::TheoryLaunchers<TH>::pass,                       \
-                                      &ss::TheoryLaunchers<TH>::density};                   \
+                                      &ss::TheoryLaunchers<TH>::density,                    \
+                                      &ss::TheoryLaunchers<TH>::adjoint,                    \
+                                      &ss::TheoryLaunchers<TH>::combine};                   \
     return &ops;                                                                            \
   }
 

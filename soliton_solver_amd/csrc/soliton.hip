@@ -149,6 +149,14 @@ struct ss_ctx {
   double* dens = nullptr;
   double* scal = nullptr;  // scalar results on device
   DevState* host_st = nullptr;  // pinned
+  // exact-adjoint scratch (allocated on first use): stage field, gradient,
+  // the two transposed-stencil inputs, and per-block partials of the combine
+  void* temp = nullptr;
+  double* G = nullptr;
+  double* Px = nullptr;
+  double* Py = nullptr;
+  double* xpart = nullptr;
+  int xpart_n = 0;
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // indexed by starting parity
   std::string err;
@@ -284,9 +292,15 @@ static PassBufs bufs_for(ss_ctx* c, int parity) {
   return b;
 }
 
+static int launch_step_exact(ss_ctx* c, int parity);
+
 // Launches of one step from buffer parity ``parity``.  With ``ev`` non-null an
 // event is recorded before the first and after every launch.
 static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
+  if (c->var_mode == SS_VAR_EXACT_ADJOINT) {
+    if (ev) return fail(c, "pass timing is available for the fused reference flow only");
+    return launch_step_exact(c, parity);
+  }
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
   double* p1 = c->part + c->ntiles;
@@ -317,6 +331,58 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
                      c->st, histErr, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  return 0;
+}
+
+static int ensure_exact(ss_ctx* c) {
+  if (c->prec != SS_PREC_FP64) return fail(c, "the exact-adjoint variation needs FP64 storage");
+  const size_t nd = (size_t)c->g.nf * (size_t)c->g.N;
+  if (!c->G) {
+    HIPCHK(c, hipMalloc(&c->temp, nd * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->G, nd * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->Px, nd * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->Py, nd * sizeof(double)));
+  }
+  const int nb = (int)((c->g.N + kThreads - 1) / kThreads);
+  if (!c->xpart || c->xpart_n < nb) {
+    if (c->xpart) hipFree(c->xpart);
+    c->xpart = nullptr;
+    HIPCHK(c, hipMalloc(&c->xpart, 2 * sizeof(double) * (size_t)nb));
+    c->xpart_n = nb;
+  }
+  return 0;
+}
+
+// One step of the unfused exact-adjoint flow: the same RK4 stages, kill and
+// error as the fused passes, with the variation replaced by
+// (1/gv) dE_h/dphi (theory flow convention applied).
+static int launch_step_exact(ss_ctx* c, int parity) {
+  if (ensure_exact(c)) return -1;
+  PassBufs b = bufs_for(c, parity);
+  HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+                     c->hist, c->hist_cap, c->g.killkinen);
+  HIPCHK(c, hipGetLastError());
+  const double ca[4] = {0.0, 0.5, 0.5, 1.0};
+  const double cb[4] = {0.0, 0.0, 0.5, 0.5};
+  const void* lprev[4] = {nullptr, nullptr, c->l[0], c->l[1]};
+  void* lout[4] = {c->l[0], c->l[1], c->l[2], c->temp};
+  const long long n = (long long)c->g.nf * c->g.N;
+  const int nb = (int)((n + 255) / 256);
+  for (int st = 0; st < 4; ++st) {
+    hipLaunchKernelGGL((stage_field<double>), dim3(nb < 4096 ? nb : 4096), dim3(256), 0, c->stream,
+                       (double*)c->temp, (const double*)b.F, (const double*)b.V, (const double*)lprev[st], ca[st],
+                       cb[st], st > 0 ? 1 : 0, (const DevState*)c->st, c->g);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, c->ops->adjoint(c->prec, c->temp, c->G, c->Px, c->Py, c->temp, lout[st], 1, c->tp, c->g, c->stream));
+  }
+  b.part0 = c->xpart;
+  b.part1 = c->xpart + c->xpart_n;
+  int nblk = 0;
+  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->g, c->stream, &nblk));
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
+                     nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
@@ -376,6 +442,8 @@ int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f,
   if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return fail(nullptr, "bad precision %d", precision);
   if (variation_mode != SS_VAR_REFERENCE && variation_mode != SS_VAR_EXACT_ADJOINT)
     return fail(nullptr, "bad variation mode %d", variation_mode);
+  if (variation_mode == SS_VAR_EXACT_ADJOINT && precision != SS_PREC_FP64)
+    return fail(nullptr, "the exact-adjoint variation needs FP64 storage");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return hipfail(nullptr, "no HIP device available");
   ss_ctx* c = new ss_ctx();
@@ -439,6 +507,11 @@ int ss_ctx_destroy(ss_ctx* c) {
   if (c->dens) hipFree(c->dens);
   if (c->scal) hipFree(c->scal);
   if (c->host_st) hipHostFree(c->host_st);
+  if (c->temp) hipFree(c->temp);
+  if (c->G) hipFree(c->G);
+  if (c->Px) hipFree(c->Px);
+  if (c->Py) hipFree(c->Py);
+  if (c->xpart) hipFree(c->xpart);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -467,7 +540,8 @@ int ss_set_params(ss_ctx* c, const int32_t* p_i, int n_i, const double* p_f, int
 int ss_set_variation_mode(ss_ctx* c, int mode) {
   if (!c) return fail(nullptr, "null context");
   if (mode != SS_VAR_REFERENCE && mode != SS_VAR_EXACT_ADJOINT) return fail(c, "bad variation mode %d", mode);
-  if (mode == SS_VAR_EXACT_ADJOINT) return fail(c, "exact-adjoint variation is not available for %s", c->ops->name);
+  if (mode == SS_VAR_EXACT_ADJOINT && c->prec != SS_PREC_FP64)
+    return fail(c, "the exact-adjoint variation needs FP64 storage");
   c->var_mode = mode;
   free_graphs(c);
   return 0;
@@ -477,6 +551,8 @@ int ss_set_precision(ss_ctx* c, int prec) {
   if (!c) return fail(nullptr, "null context");
   if (prec != SS_PREC_FP64 && prec != SS_PREC_FP32) return fail(c, "bad precision %d", prec);
   if (prec == c->prec) return 0;
+  if (prec == SS_PREC_FP32 && c->var_mode == SS_VAR_EXACT_ADJOINT)
+    return fail(c, "the exact-adjoint variation needs FP64 storage");
   const long long n = (long long)c->g.nf * c->g.N;
   const size_t nb = (size_t)n * elem_size(prec);
   void* nF = nullptr;
@@ -547,7 +623,6 @@ int ss_use_graphs(ss_ctx* c, int steps) {
 static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, double* acc_ms) {
   if (!c) return fail(nullptr, "null context");
   if (nsteps < 0) return fail(c, "nsteps must be >= 0");
-  if (c->var_mode != SS_VAR_REFERENCE) return fail(c, "the fused flow runs the reference variation only");
   if (ensure_hist(c, nsteps + 1)) return -1;
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, c->stream, c->st, prev_energy);
   HIPCHK(c, hipGetLastError());
@@ -644,7 +719,12 @@ int ss_density(ss_ctx* c, int kind, double* host_out, size_t n) {
 }
 
 static int variation_into(ss_ctx* c, const void* field, void* out, int mode, hipStream_t st) {
-  if (mode != SS_VAR_REFERENCE) return fail(c, "exact-adjoint variation is not available for %s", c->ops->name);
+  if (mode == SS_VAR_EXACT_ADJOINT) {
+    if (ensure_exact(c)) return -1;
+    HIPCHK(c, c->ops->adjoint(c->prec, field, (double*)out, c->Px, c->Py, nullptr, nullptr, 0, c->tp, c->g, st));
+    return 0;
+  }
+  if (mode != SS_VAR_REFERENCE) return fail(c, "bad variation mode %d", mode);
   PassBufs b{};
   b.F = field;
   b.lout = out;

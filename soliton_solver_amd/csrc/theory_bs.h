@@ -165,6 +165,11 @@ struct BabySkyrme {
   }
 
   template <typename T>
+  __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
+    project(eg, f);
+  }
+
+  template <typename T>
   __device__ __forceinline__ static void normalize(T f[NF]) {
     const T n = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
     if (n != T(0)) {

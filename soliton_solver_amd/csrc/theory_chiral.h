@@ -140,6 +140,15 @@ struct ChiralMagnet {
     return -(d1[0][0] + d1[1][1]);
   }
 
+  // Exact-adjoint flow convention: psi is a saddle field, flowed along
+  // -coup_mu dE/dpsi as the reference's variation does (kernels.py:503); the
+  // magnetisation gradient is projected onto the tangent space.
+  template <typename T>
+  __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
+    if (tp.demag) eg[3] = -T(tp.coupmu) * eg[3];
+    project(eg, f);
+  }
+
   // Unit-norm constraint (kernels.py:47-83): sites with |m| == 0 are left as is.
   template <typename T>
   __device__ __forceinline__ static void normalize(T f[NF]) {
