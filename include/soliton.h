@@ -133,6 +133,47 @@ int ss_memory_bytes(ss_ctx* ctx, size_t* bytes);
 
 const char* ss_last_error(ss_ctx* ctx);
 
+/* ---------------------------------------------------------------------------
+ * Batched form: many independent relaxations of one theory (each with its own
+ * lattice shape, spacing, time step and couplings) advanced together, one
+ * kernel launch per RK4 pass over the tiles of every active instance.  The
+ * reference drives one Simulation per lattice (core/simulation.py:43-409);
+ * ss_batch replaces a Python loop over such Simulations.  Each row of p_i_all
+ * / p_f_all is one instance's pack_device_params output (all rows the same
+ * length). */
+typedef struct ss_batch ss_batch;
+
+int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, const double* p_f_all, int n_f,
+                    int precision, ss_batch** out);
+int ss_batch_destroy(ss_batch* b);
+const char* ss_batch_last_error(ss_batch* b);
+int ss_batch_num_tiles(ss_batch* b);
+
+/* Field / Velocity of one instance (FP64 host, reference layout, n = nf*xlen*ylen). */
+int ss_batch_upload(ss_batch* b, int inst, int buf, const double* host, size_t n);
+int ss_batch_download(ss_batch* b, int inst, int buf, double* host, size_t n);
+
+/* Start a relaxation: energy of every instance's field becomes its first
+ * prev_energy (the reference's viewer loop seeds its E the same way,
+ * chiral_magnet/render_gl.py:663-666); all instances active. */
+int ss_batch_begin(ss_batch* b);
+
+/* Up to nsteps arrested-Newton steps of every active instance.  An instance
+ * retires (stops updating) after the first step whose max |stage-4 energy
+ * gradient| is below tol.  The host reads the activity flags every
+ * check_every steps and returns early when none is active. */
+int ss_batch_step(ss_batch* b, int nsteps, double tol, int check_every, int* n_active);
+
+/* Capture steps_per_graph steps into one hipGraph replayed by ss_batch_step
+ * (0 disables). */
+int ss_batch_use_graphs(ss_batch* b, int steps_per_graph);
+
+/* Per-instance energy of the current field, last max |EG|, steps taken and
+ * the step at which it converged (-1 if not yet); arrays of ninst, any may be
+ * NULL. */
+int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps, int* converged_step);
+int ss_batch_memory_bytes(ss_batch* b, size_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

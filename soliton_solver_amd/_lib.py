@@ -60,6 +60,17 @@ SIGNATURES = {
     "ss_time_passes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _f64p]),
     "ss_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "ss_last_error": (C.c_char_p, [C.c_void_p]),
+    "ss_batch_create": (C.c_int, [C.c_int, C.c_int, _i32p, C.c_int, _f64p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "ss_batch_destroy": (C.c_int, [C.c_void_p]),
+    "ss_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "ss_batch_num_tiles": (C.c_int, [C.c_void_p]),
+    "ss_batch_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
+    "ss_batch_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
+    "ss_batch_begin": (C.c_int, [C.c_void_p]),
+    "ss_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
+    "ss_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_batch_state": (C.c_int, [C.c_void_p, _f64p, _f64p, _i32p, _i32p]),
+    "ss_batch_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
 }
 
 _lib = None
@@ -90,13 +101,16 @@ def lib():
     return L
 
 
-def error_text(ctx=None) -> str:
+def error_text(ctx=None, batch=None) -> str:
     L = lib()
-    msg = L.ss_last_error(ctx) if ctx else L.ss_global_error()
+    if batch:
+        msg = L.ss_batch_last_error(batch)
+    else:
+        msg = L.ss_last_error(ctx) if ctx else L.ss_global_error()
     return (msg or b"").decode(errors="replace")
 
 
-def check(rc: int, ctx=None) -> None:
+def check(rc: int, ctx=None, batch=None) -> None:
     """Map a non-zero status to the exception the reference raises for it.
 
     -1 (bad argument or unsupported request) -> ValueError,
@@ -104,7 +118,7 @@ def check(rc: int, ctx=None) -> None:
     """
     if rc == 0:
         return
-    msg = error_text(ctx)
+    msg = error_text(ctx, batch)
     if rc == -1:
         raise ValueError(msg)
     raise RuntimeError(msg)

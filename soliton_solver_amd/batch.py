@@ -1,0 +1,120 @@
+"""Many independent relaxations of one theory in one set of launches.
+
+``BatchSimulation(theory, [params_0, params_1, ...])`` is the batched
+counterpart of a list of reference ``Simulation`` objects (one per lattice,
+core/simulation.py:43-409): every instance keeps its own lattice shape,
+spacing, time step, couplings and arrest state, but each RK4 pass is a single
+kernel launch over the tiles of all active instances, the K-step loop is a
+replayed hipGraph, and an instance retires on the device as soon as its max
+|stage-4 energy gradient| drops below the tolerance.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+_PREC = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}
+
+
+class BatchSimulation:
+    def __init__(self, theory, params_list, precision: str = "fp32"):
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        if not params_list:
+            raise ValueError("params_list is empty")
+        self.theory = theory
+        self.precision = precision
+        self.params = list(params_list)
+        self.rps = [p.resolved() for p in self.params]
+        packed = [theory.params.pack_device_params(rp) for rp in self.rps]
+        n_i = {len(pi) for pi, _ in packed}
+        n_f = {len(pf) for _, pf in packed}
+        if len(n_i) != 1 or len(n_f) != 1:
+            raise ValueError("all instances must pack to the same p_i / p_f lengths")
+        self.p_i = np.ascontiguousarray(np.stack([pi for pi, _ in packed]), dtype=np.int32)
+        self.p_f = np.ascontiguousarray(np.stack([pf for _, pf in packed]), dtype=np.float64)
+        self.ninst = len(self.params)
+        b = C.c_void_p()
+        _lib.check(_lib.lib().ss_batch_create(int(theory.kernels.THEORY_ID), self.ninst, self.p_i.ravel(),
+                                              self.p_i.shape[1], self.p_f.ravel(), self.p_f.shape[1],
+                                              _PREC[precision], C.byref(b)))
+        self._b = b
+
+    def close(self):
+        if self._b:
+            _lib.lib().ss_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check(rc, batch=self._b)
+
+    def field_shape(self, i):
+        rp = self.rps[i]
+        return (int(rp.number_total_fields), int(rp.xlen), int(rp.ylen))
+
+    def grid(self, i):
+        rp = self.rps[i]
+        g = np.empty((2, int(rp.xlen), int(rp.ylen)))
+        g[0] = rp.lsx * np.arange(int(rp.xlen))[:, None]
+        g[1] = rp.lsy * np.arange(int(rp.ylen))[None, :]
+        return g
+
+    def set_field(self, i, F, V=None):
+        F = np.ascontiguousarray(np.asarray(F, dtype=np.float64).ravel())
+        self._check(_lib.lib().ss_batch_upload(self._b, int(i), _lib.BUF_FIELD, F, F.size))
+        V = np.zeros_like(F) if V is None else np.ascontiguousarray(np.asarray(V, dtype=np.float64).ravel())
+        self._check(_lib.lib().ss_batch_upload(self._b, int(i), _lib.BUF_VELOCITY, V, V.size))
+
+    def get_field(self, i):
+        out = np.empty(int(np.prod(self.field_shape(i))))
+        self._check(_lib.lib().ss_batch_download(self._b, int(i), _lib.BUF_FIELD, out, out.size))
+        return out.reshape(self.field_shape(i))
+
+    def initialize(self, configs):
+        """One theory ``initial_config`` dict per instance (host-side generators)."""
+        for i, cfg in enumerate(configs):
+            shape = self.field_shape(i)
+            F = np.zeros(shape)
+            V = np.zeros(shape)
+            self.theory.initial_config.initialize(Field=F, Velocity=V, grid=self.grid(i), p_i_h=self.p_i[i],
+                                                  p_f_h=self.p_f[i], config=cfg)
+            self.set_field(i, F, V)
+        self.begin()
+
+    def begin(self):
+        self._check(_lib.lib().ss_batch_begin(self._b))
+
+    def use_graphs(self, steps_per_graph: int):
+        self._check(_lib.lib().ss_batch_use_graphs(self._b, int(steps_per_graph)))
+
+    def run(self, nsteps: int, tol: float, check_every: int = 100) -> int:
+        """Advance up to ``nsteps``; returns how many instances are still active."""
+        n = C.c_int()
+        self._check(_lib.lib().ss_batch_step(self._b, int(nsteps), float(tol), int(check_every), C.byref(n)))
+        return n.value
+
+    def state(self):
+        E = np.empty(self.ninst)
+        err = np.empty(self.ninst)
+        steps = np.empty(self.ninst, dtype=np.int32)
+        conv = np.empty(self.ninst, dtype=np.int32)
+        self._check(_lib.lib().ss_batch_state(self._b, E, err, steps, conv))
+        return {"energy": E, "err_absmax": err, "steps": steps, "converged_step": conv}
+
+    @property
+    def num_tiles(self) -> int:
+        return int(_lib.lib().ss_batch_num_tiles(self._b))
+
+    def memory_bytes(self) -> int:
+        n = C.c_size_t()
+        self._check(_lib.lib().ss_batch_memory_bytes(self._b, C.byref(n)))
+        return int(n.value)

@@ -23,6 +23,8 @@
 // as zero, exactly as the reference's zeroed V would be read.
 #pragma once
 
+#include <string.h>
+
 #include "common.h"
 #include "dual.h"
 #include "stencil.h"
@@ -59,25 +61,21 @@ inline Launch make_launch(const Geom& g) {
   return L;
 }
 
+// One tile of one pass: stage field into LDS, stencils + variation from LDS,
+// outputs for the sites of the tile, and the block's partial (energy sum for
+// pass 1, signed max / max |EG| for pass 4) written to part0[t] / part1[t].
+// Shared by the single-lattice kernel and the batched kernel.
 template <class TH, typename R, int PASS>
-__global__ void __launch_bounds__(kThreads)
-stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
+__device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, const PassBufs& b,
+                                           const typename TH::TP& tp, const Geom& g, int x0, int y0, bool kill,
+                                           int t) {
   using S = TileShape<R, TH::NF>;
   constexpr int NF = TH::NF;
-  __shared__ Tile<R, NF> tile;
-  __shared__ double red[kWaves];
-
-  const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
-  if (t < 0) return;  // whole block exits together, before any barrier
-  const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
-  const int x0 = tx * S::TX, y0 = ty * TY;
-
   const R* __restrict__ F = (const R*)b.F;
   const R* __restrict__ V = (const R*)b.V;
   const R* __restrict__ l1 = (const R*)b.l1;
   const R* __restrict__ l2 = (const R*)b.l2;
   const R* __restrict__ l3 = (const R*)b.l3;
-  const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
   const R dt = R(g.dt);
   const R half = R(0.5);
 
@@ -160,6 +158,19 @@ stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
     const double m1 = block_max<kThreads>(acc1, red);
     if (threadIdx.x == 0) b.part1[t] = m1;
   }
+}
+
+template <class TH, typename R, int PASS>
+__global__ void __launch_bounds__(kThreads)
+stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
+  using S = TileShape<R, TH::NF>;
+  __shared__ Tile<R, TH::NF> tile;
+  __shared__ double red[kWaves];
+  const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
+  if (t < 0) return;  // whole block exits together, before any barrier
+  const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
+  const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
+  stage_tile<TH, R, PASS>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
 }
 
 // Energy (kind < 0) or charge density ``kind`` of F, times gv: block partial
@@ -358,6 +369,173 @@ combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, cons
   if (threadIdx.x == 0) part1[blockIdx.x] = b1;
 }
 
+// ---------------------------------------------------------------------------
+// Batched form: many independent lattices (own shape, spacing, time step and
+// couplings) relaxed together.  Every pass is ONE launch over the tiles of all
+// active instances; a global tile index t maps to (instance, local tile)
+// through ``tile_inst``.  Per-instance arrest state, buffer parity and
+// activity live on the device, so a captured K-step graph replays with static
+// arguments and a converged instance simply stops (its blocks exit at once).
+
+template <class TH>
+struct BatchInst {
+  Geom g;
+  typename TH::TP tp;
+  long long off;      // element offset of the instance inside every field buffer
+  int tile_begin;     // first global tile index of the instance
+  int ntiles;         // tiles of the instance
+  int ntiles_y;       // tiles along y
+  int pad;
+};
+
+struct BatchState {
+  double prev_energy, last_energy, err_max, err_absmax;
+  int kill, step, active, cur;
+  int converged_step, pad[3];
+};
+
+struct BatchBufs {
+  void* F[2];
+  void* V[2];
+  void* l[3];
+  double* part0;
+  double* part1;
+  const int* tile_inst;
+  const void* insts;   // BatchInst<TH>[ninst]
+  BatchState* st;
+  int ntiles, per_xcd;
+};
+
+template <class TH, typename R, int PASS>
+__global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
+  using S = TileShape<R, TH::NF>;
+  __shared__ Tile<R, TH::NF> tile;
+  __shared__ double red[kWaves];
+  const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
+  if (t < 0) return;
+  const int i = B.tile_inst[t];
+  const BatchState& st = B.st[i];
+  if (!st.active) return;  // uniform per block
+  const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
+  const int lt = t - I.tile_begin;
+  const int tx = lt / I.ntiles_y, ty = lt - tx * I.ntiles_y;
+  const int cur = st.cur;
+  const size_t es = sizeof(R);
+  PassBufs b{};
+  b.F = (const char*)B.F[cur] + I.off * es;
+  b.V = (const char*)B.V[cur] + I.off * es;
+  b.l1 = (const char*)B.l[0] + I.off * es;
+  b.l2 = (const char*)B.l[1] + I.off * es;
+  b.l3 = (const char*)B.l[2] + I.off * es;
+  b.lout = PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + I.off * es : nullptr;
+  b.Fout = (char*)B.F[cur ^ 1] + I.off * es;
+  b.Vout = (char*)B.V[cur ^ 1] + I.off * es;
+  b.part0 = B.part0;
+  b.part1 = B.part1;
+  const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
+  stage_tile<TH, R, PASS>(tile, red, b, I.tp, I.g, tx * S::TX, ty * TY, kill, t);
+}
+
+// Energy partials of every instance (active or not) from its current field.
+template <class TH, typename R>
+__global__ void __launch_bounds__(kThreads) batch_energy_pass(BatchBufs B) {
+  using S = TileShape<R, TH::NF>;
+  constexpr int NF = TH::NF;
+  __shared__ Tile<R, NF> tile;
+  __shared__ double red[kWaves];
+  const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
+  if (t < 0) return;
+  const int i = B.tile_inst[t];
+  const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
+  const Geom& g = I.g;
+  const int lt = t - I.tile_begin;
+  const int tx = lt / I.ntiles_y, ty = lt - tx * I.ntiles_y;
+  const int x0 = tx * S::TX, y0 = ty * TY;
+  const R* __restrict__ F = (const R*)B.F[B.st[i].cur] + I.off;
+  load_tile<R, NF>(tile, g, x0, y0, [&](int a, long long off) -> R { return F[off]; });
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int k = 0; k < S::kRowsPerWave; ++k) {
+    const int lx = w * S::kRowsPerWave + k, x = x0 + lx, y = y0 + lane;
+    if (x < g.X && y < g.Y) {
+      const SiteInfo s = site_info(g, x, y);
+      double fd[NF], d1d[NF][2];
+      tile_d1<double, R, NF>(tile, lx, lane, g, s, fd, d1d);
+      acc += TH::template energy<double>(I.tp, g, s, fd, d1d) * g.gv;
+    }
+  }
+  const double sum = block_sum<kThreads>(acc, red);
+  if (threadIdx.x == 0) B.part0[t] = sum;
+}
+
+// Per-instance finalize of the energy (one block per instance).  ``report``
+// 0: arrest decision of the previous step, exactly as the single-lattice
+// k_fin_energy (active instances only); 1: only record the energy (every
+// instance), leaving the arrest state untouched; 2: start of a relaxation
+// (prev_energy = E, step 0, active, no kill).
+template <class TH>
+__global__ void __launch_bounds__(256) batch_fin_energy(BatchBufs B, int report) {
+  __shared__ double red[4];
+  const int i = blockIdx.x;
+  BatchState& st = B.st[i];
+  if (report == 0 && !st.active) return;
+  const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
+  double s = 0.0, c = 0.0;
+  for (int k = threadIdx.x; k < I.ntiles; k += 256) {
+    const double y = B.part0[I.tile_begin + k] - c;
+    const double tt = s + y;
+    c = (tt - s) - y;
+    s = tt;
+  }
+  const double E = block_sum<256>(s, red);
+  if (threadIdx.x == 0) {
+    st.last_energy = E;
+    if (report == 2) {
+      st.prev_energy = E;
+      st.kill = 0;
+      st.step = 0;
+      st.active = 1;
+      st.converged_step = -1;
+    } else if (report == 0) {
+      if (st.step > 0) {
+        st.kill = (I.g.killkinen && E > st.prev_energy) ? 1 : 0;
+        st.prev_energy = E;
+      } else {
+        st.kill = 0;
+      }
+    }
+  }
+}
+
+// Per-instance finalize of the stage-4 error; flips the buffer parity and
+// retires instances whose max |EG| fell below ``tol``.
+template <class TH>
+__global__ void __launch_bounds__(256) batch_fin_err(BatchBufs B, double tol) {
+  __shared__ double red[4];
+  const int i = blockIdx.x;
+  BatchState& st = B.st[i];
+  if (!st.active) return;
+  const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
+  double m0 = -1.0e300, m1 = 0.0;
+  for (int k = threadIdx.x; k < I.ntiles; k += 256) {
+    m0 = fmax(m0, B.part0[I.tile_begin + k]);
+    m1 = fmax(m1, B.part1[I.tile_begin + k]);
+  }
+  const double a = block_max<256>(m0, red);
+  const double b = block_max<256>(m1, red);
+  if (threadIdx.x == 0) {
+    st.err_max = a;
+    st.err_absmax = b;
+    st.step += 1;
+    st.cur ^= 1;
+    if (b < tol) {
+      st.active = 0;
+      st.converged_step = st.step;
+    }
+  }
+}
+
 // Host-side launchers for one theory.  ``prec`` 0 = FP64 storage, 1 = FP32.
 template <class TH>
 struct TheoryLaunchers {
@@ -440,6 +618,51 @@ struct TheoryLaunchers {
     static_assert(sizeof(TP) <= 512, "theory parameter block too large");
     return TH::make(pi, ni, pf, nfl, (TP*)tp, msg, cap);
   }
+
+  // -- batched form ----------------------------------------------------------
+  static int batch_inst_size() { return (int)sizeof(BatchInst<TH>); }
+  static int tile_rows(int prec) { return prec ? TileShape<float, TH::NF>::TX : TileShape<double, TH::NF>::TX; }
+
+  static void batch_fill(void* dst, const Geom& g, const void* tp, long long off, int tile_begin, int ntiles,
+                         int ntiles_y) {
+    BatchInst<TH> I;
+    memset(&I, 0, sizeof(I));
+    I.g = g;
+    I.tp = *(const TP*)tp;
+    I.off = off;
+    I.tile_begin = tile_begin;
+    I.ntiles = ntiles;
+    I.ntiles_y = ntiles_y;
+    memcpy(dst, &I, sizeof(I));
+  }
+
+  template <typename R>
+  static hipError_t batch_pass_r(int pass, const BatchBufs& B, hipStream_t st) {
+    const dim3 grid(8 * B.per_xcd), block(kThreads);
+    switch (pass) {
+      case 1: hipLaunchKernelGGL((batch_stage_pass<TH, R, 1>), grid, block, 0, st, B); break;
+      case 2: hipLaunchKernelGGL((batch_stage_pass<TH, R, 2>), grid, block, 0, st, B); break;
+      case 3: hipLaunchKernelGGL((batch_stage_pass<TH, R, 3>), grid, block, 0, st, B); break;
+      case 4: hipLaunchKernelGGL((batch_stage_pass<TH, R, 4>), grid, block, 0, st, B); break;
+      case 5: hipLaunchKernelGGL((batch_energy_pass<TH, R>), grid, block, 0, st, B); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+
+  // pass 1..4: fused RK4 passes; 5: energy partials of every instance.
+  static hipError_t batch_pass(int prec, int pass, const BatchBufs& B, hipStream_t st) {
+    return prec ? batch_pass_r<float>(pass, B, st) : batch_pass_r<double>(pass, B, st);
+  }
+
+  // which 0: energy finalize with ``mode`` (0 arrest, 1 report, 2 begin); 1: error finalize with ``tol``.
+  static hipError_t batch_fin(int which, int ninst, const BatchBufs& B, int mode, double tol, hipStream_t st) {
+    if (which == 0)
+      hipLaunchKernelGGL((batch_fin_energy<TH>), dim3(ninst), dim3(256), 0, st, B, mode);
+    else
+      hipLaunchKernelGGL((batch_fin_err<TH>), dim3(ninst), dim3(256), 0, st, B, tol);
+    return hipGetLastError();
+  }
 };
 
 // Table of entry points one theory exposes to the context code.
@@ -454,6 +677,11 @@ struct TheoryOps {
   hipError_t (*adjoint)(int, const void*, double*, double*, double*, const void*, void*, int, const void*,
                         const Geom&, hipStream_t);
   hipError_t (*combine)(int, const PassBufs&, const double*, const Geom&, hipStream_t, int*);
+  int (*batch_inst_size)();
+  int (*tile_rows)(int);
+  void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
+  hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
+  hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
 };
 
 #define SS_DEFINE_THEORY_OPS(TH, fn)                                                        \
@@ -464,7 +692,12 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::pass,                       \
                                       &ss::TheoryLaunchers<TH>::density,                    \
                                       &ss::TheoryLaunchers<TH>::adjoint,                    \
-                                      &ss::TheoryLaunchers<TH>::combine};                   \
+                                      &ss::TheoryLaunchers<TH>::combine,                    \
+                                      &ss::TheoryLaunchers<TH>::batch_inst_size,            \
+                                      &ss::TheoryLaunchers<TH>::tile_rows,                  \
+                                      &ss::TheoryLaunchers<TH>::batch_fill,                 \
+                                      &ss::TheoryLaunchers<TH>::batch_pass,                 \
+                                      &ss::TheoryLaunchers<TH>::batch_fin};                 \
     return &ops;                                                                            \
   }
 

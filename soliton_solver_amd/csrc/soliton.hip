@@ -783,3 +783,328 @@ int ss_memory_bytes(ss_ctx* c, size_t* bytes) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched form: ss_batch holds many independent lattices of one theory.
+
+struct ss_batch {
+  int theory = 0;
+  const TheoryOps* ops = nullptr;
+  int prec = 0;
+  int ninst = 0;
+  int ni = 0, nfl = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Geom> geoms;
+  std::vector<long long> offs;
+  std::vector<int> tile_begin, ntiles_i;
+  long long total = 0;  // elements per field buffer
+  int ntiles = 0;
+  void* F[2] = {nullptr, nullptr};
+  void* V[2] = {nullptr, nullptr};
+  void* l[3] = {nullptr, nullptr, nullptr};
+  double* part = nullptr;
+  int* tile_inst = nullptr;
+  void* insts = nullptr;
+  BatchState* st = nullptr;
+  BatchState* host_st = nullptr;
+  int graph_steps = 0;
+  double graph_tol = -1.0;
+  hipGraphExec_t graph = nullptr;
+  std::string err;
+};
+
+static int bfail(ss_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf; else g_global_err = buf;
+  return code;
+}
+
+#define BHIP(b, expr)                                                                              \
+  do {                                                                                             \
+    hipError_t _e = (expr);                                                                        \
+    if (_e != hipSuccess) return bfail((b), -2, "%s failed: %s", #expr, hipGetErrorString(_e));   \
+  } while (0)
+
+static BatchBufs batch_bufs(ss_batch* b) {
+  BatchBufs B{};
+  for (int k = 0; k < 2; ++k) {
+    B.F[k] = b->F[k];
+    B.V[k] = b->V[k];
+  }
+  for (int k = 0; k < 3; ++k) B.l[k] = b->l[k];
+  B.part0 = b->part;
+  B.part1 = b->part + b->ntiles;
+  B.tile_inst = b->tile_inst;
+  B.insts = b->insts;
+  B.st = b->st;
+  B.ntiles = b->ntiles;
+  B.per_xcd = (b->ntiles + 7) / 8;
+  return B;
+}
+
+static int batch_launch_step(ss_batch* b, double tol) {
+  const BatchBufs B = batch_bufs(b);
+  BHIP(b, b->ops->batch_pass(b->prec, 1, B, b->stream));
+  BHIP(b, b->ops->batch_fin(0, b->ninst, B, 0, 0.0, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 2, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 3, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 4, B, b->stream));
+  BHIP(b, b->ops->batch_fin(1, b->ninst, B, 0, tol, b->stream));
+  return 0;
+}
+
+static int batch_read_state(ss_batch* b) {
+  BHIP(b, hipMemcpyAsync(b->host_st, b->st, sizeof(BatchState) * b->ninst, hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+extern "C" {
+
+int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, const double* p_f_all, int n_f,
+                    int precision, ss_batch** out) {
+  if (!out) return bfail(nullptr, -1, "ss_batch_create: out is NULL");
+  *out = nullptr;
+  const TheoryOps* ops = theory_ops(theory_id);
+  if (!ops) return bfail(nullptr, -1, "unknown or unbuilt theory id %d", theory_id);
+  if (ninst < 1) return bfail(nullptr, -1, "ninst must be >= 1");
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return bfail(nullptr, -1, "bad precision %d", precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, -2, "no HIP device available");
+  ss_batch* b = new ss_batch();
+  b->theory = theory_id;
+  b->ops = ops;
+  b->prec = precision;
+  b->ninst = ninst;
+  b->ni = n_i;
+  b->nfl = n_f;
+  auto bail = [&](int rc) {
+    g_global_err = b->err;
+    ss_batch_destroy(b);
+    return rc;
+  };
+  const int isz = ops->batch_inst_size();
+  std::vector<unsigned char> host_insts((size_t)isz * ninst);
+  std::vector<int> tinst;
+  const int tx = ops->tile_rows(precision);
+  for (int i = 0; i < ninst; ++i) {
+    ss_ctx tmp;
+    Geom g{};
+    const int32_t* pi = p_i_all + (size_t)i * n_i;
+    const double* pf = p_f_all + (size_t)i * n_f;
+    if (resolve_geom(&tmp, pi, n_i, pf, n_f, &g)) {
+      bfail(b, -1, "instance %d: %s", i, tmp.err.c_str());
+      return bail(-1);
+    }
+    if (g.nf != ops->nf) {
+      bfail(b, -1, "instance %d: theory %s is built for %d fields", i, ops->name, ops->nf);
+      return bail(-1);
+    }
+    alignas(16) unsigned char tp[512];
+    char msg[256] = {0};
+    if (ops->make_tp(pi, n_i, pf, n_f, tp, msg, sizeof(msg))) {
+      bfail(b, -1, "instance %d: %s", i, msg);
+      return bail(-1);
+    }
+    const int nty = (g.Y + TY - 1) / TY;
+    const int nt = ((g.X + tx - 1) / tx) * nty;
+    b->geoms.push_back(g);
+    b->offs.push_back(b->total);
+    b->tile_begin.push_back(b->ntiles);
+    b->ntiles_i.push_back(nt);
+    ops->batch_fill(host_insts.data() + (size_t)isz * i, g, tp, b->total, b->ntiles, nt, nty);
+    for (int k = 0; k < nt; ++k) tinst.push_back(i);
+    b->total += (long long)g.nf * g.N;
+    b->ntiles += nt;
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    bfail(b, -2, "hipStreamCreate failed");
+    return bail(-2);
+  }
+  const size_t bytes = (size_t)b->total * elem_size(precision);
+  bool ok = true;
+  for (int k = 0; k < 2 && ok; ++k)
+    ok = hipMalloc(&b->F[k], bytes) == hipSuccess && hipMalloc(&b->V[k], bytes) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&b->l[k], bytes) == hipSuccess;
+  ok = ok && hipMalloc(&b->part, sizeof(double) * 2 * (size_t)b->ntiles) == hipSuccess;
+  ok = ok && hipMalloc(&b->tile_inst, sizeof(int) * (size_t)b->ntiles) == hipSuccess;
+  ok = ok && hipMalloc(&b->insts, host_insts.size()) == hipSuccess;
+  ok = ok && hipMalloc(&b->st, sizeof(BatchState) * ninst) == hipSuccess;
+  ok = ok && hipHostMalloc(&b->host_st, sizeof(BatchState) * ninst, hipHostMallocDefault) == hipSuccess;
+  if (!ok) {
+    bfail(b, -2, "device allocation of the batch failed (%zu bytes per field buffer)", bytes);
+    return bail(-2);
+  }
+  for (int k = 0; k < 2 && ok; ++k)
+    ok = hipMemsetAsync(b->F[k], 0, bytes, b->stream) == hipSuccess &&
+         hipMemsetAsync(b->V[k], 0, bytes, b->stream) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k) ok = hipMemsetAsync(b->l[k], 0, bytes, b->stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(b->st, 0, sizeof(BatchState) * ninst, b->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(b->tile_inst, tinst.data(), sizeof(int) * tinst.size(), hipMemcpyHostToDevice,
+                            b->stream) == hipSuccess;
+  ok = ok && hipMemcpyAsync(b->insts, host_insts.data(), host_insts.size(), hipMemcpyHostToDevice, b->stream) ==
+                 hipSuccess;
+  ok = ok && hipStreamSynchronize(b->stream) == hipSuccess;
+  if (!ok) {
+    bfail(b, -2, "initialisation of the batch failed");
+    return bail(-2);
+  }
+  *out = b;
+  return 0;
+}
+
+int ss_batch_destroy(ss_batch* b) {
+  if (!b) return 0;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (b->F[k]) hipFree(b->F[k]);
+    if (b->V[k]) hipFree(b->V[k]);
+  }
+  for (int k = 0; k < 3; ++k)
+    if (b->l[k]) hipFree(b->l[k]);
+  if (b->part) hipFree(b->part);
+  if (b->tile_inst) hipFree(b->tile_inst);
+  if (b->insts) hipFree(b->insts);
+  if (b->st) hipFree(b->st);
+  if (b->host_st) hipHostFree(b->host_st);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+const char* ss_batch_last_error(ss_batch* b) { return b ? b->err.c_str() : g_global_err.c_str(); }
+
+int ss_batch_num_tiles(ss_batch* b) { return b ? b->ntiles : 0; }
+
+static int batch_copy(ss_batch* b, int inst, int buf, double* host, size_t n, bool up) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (inst < 0 || inst >= b->ninst) return bfail(b, -1, "instance %d out of range [0, %d)", inst, b->ninst);
+  const Geom& g = b->geoms[inst];
+  const size_t want = (size_t)g.nf * (size_t)g.N;
+  if (n != want) return bfail(b, -1, "instance %d needs %zu values (got %zu)", inst, want, n);
+  if (!host) return bfail(b, -1, "null host pointer");
+  if (batch_read_state(b)) return -2;
+  const int cur = b->host_st[inst].cur;
+  void* base;
+  if (buf == SS_BUF_FIELD) base = b->F[cur];
+  else if (buf == SS_BUF_VELOCITY) base = b->V[cur];
+  else return bfail(b, -1, "unknown buffer %d", buf);
+  const size_t es = elem_size(b->prec);
+  char* dev = (char*)base + (size_t)b->offs[inst] * es;
+  if (b->prec == SS_PREC_FP64) {
+    if (up) BHIP(b, hipMemcpyAsync(dev, host, n * es, hipMemcpyHostToDevice, b->stream));
+    else BHIP(b, hipMemcpyAsync(host, dev, n * es, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+  } else {
+    std::vector<float> tmp(n);
+    if (up) {
+      for (size_t k = 0; k < n; ++k) tmp[k] = (float)host[k];
+      BHIP(b, hipMemcpyAsync(dev, tmp.data(), n * es, hipMemcpyHostToDevice, b->stream));
+      BHIP(b, hipStreamSynchronize(b->stream));
+    } else {
+      BHIP(b, hipMemcpyAsync(tmp.data(), dev, n * es, hipMemcpyDeviceToHost, b->stream));
+      BHIP(b, hipStreamSynchronize(b->stream));
+      for (size_t k = 0; k < n; ++k) host[k] = tmp[k];
+    }
+  }
+  return 0;
+}
+
+int ss_batch_upload(ss_batch* b, int inst, int buf, const double* host, size_t n) {
+  return batch_copy(b, inst, buf, (double*)host, n, true);
+}
+int ss_batch_download(ss_batch* b, int inst, int buf, double* host, size_t n) {
+  return batch_copy(b, inst, buf, host, n, false);
+}
+
+int ss_batch_begin(ss_batch* b) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  const BatchBufs B = batch_bufs(b);
+  BHIP(b, b->ops->batch_pass(b->prec, 5, B, b->stream));
+  BHIP(b, b->ops->batch_fin(0, b->ninst, B, 2, 0.0, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int ss_batch_use_graphs(ss_batch* b, int steps) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (steps < 0) return bfail(b, -1, "steps_per_graph must be >= 0");
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = steps;
+  return 0;
+}
+
+int ss_batch_step(ss_batch* b, int nsteps, double tol, int check_every, int* n_active) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (nsteps < 0) return bfail(b, -1, "nsteps must be >= 0");
+  if (check_every < 1) check_every = nsteps > 0 ? nsteps : 1;
+  if (b->graph_steps > 0 && (!b->graph || b->graph_tol != tol)) {
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    hipGraph_t gr = nullptr;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < b->graph_steps && rc == 0; ++s) rc = batch_launch_step(b, tol);
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+      if (gr) hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return bfail(b, -2, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (e != hipSuccess) return bfail(b, -2, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    b->graph_tol = tol;
+  }
+  int done = 0, since = 0, active = b->ninst;
+  while (done < nsteps) {
+    int did;
+    if (b->graph && nsteps - done >= b->graph_steps) {
+      BHIP(b, hipGraphLaunch(b->graph, b->stream));
+      did = b->graph_steps;
+    } else {
+      if (batch_launch_step(b, tol)) return -2;
+      did = 1;
+    }
+    done += did;
+    since += did;
+    if (since >= check_every || done >= nsteps) {
+      since = 0;
+      if (batch_read_state(b)) return -2;
+      active = 0;
+      for (int i = 0; i < b->ninst; ++i) active += b->host_st[i].active ? 1 : 0;
+      if (active == 0) break;
+    }
+  }
+  if (n_active) *n_active = active;
+  return 0;
+}
+
+int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps, int* converged_step) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  const BatchBufs B = batch_bufs(b);
+  BHIP(b, b->ops->batch_pass(b->prec, 5, B, b->stream));
+  BHIP(b, b->ops->batch_fin(0, b->ninst, B, 1, 0.0, b->stream));
+  if (batch_read_state(b)) return -2;
+  for (int i = 0; i < b->ninst; ++i) {
+    if (energies) energies[i] = b->host_st[i].last_energy;
+    if (err_absmax) err_absmax[i] = b->host_st[i].err_absmax;
+    if (steps) steps[i] = b->host_st[i].step;
+    if (converged_step) converged_step[i] = b->host_st[i].converged_step;
+  }
+  return 0;
+}
+
+int ss_batch_memory_bytes(ss_batch* b, size_t* bytes) {
+  if (!b || !bytes) return bfail(b, -1, "null argument");
+  *bytes = 7 * (size_t)b->total * elem_size(b->prec) + sizeof(double) * 2 * (size_t)b->ntiles;
+  return 0;
+}
+
+}  // extern "C"
