@@ -682,6 +682,7 @@ struct TheoryOps {
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
+  int l2_normalize;  // the flow L2-normalises stage fields (unfused normalised path)
 };
 
 #define SS_DEFINE_THEORY_OPS(TH, fn)                                                        \
@@ -697,7 +698,8 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::tile_rows,                  \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
-                                      &ss::TheoryLaunchers<TH>::batch_fin};                 \
+                                      &ss::TheoryLaunchers<TH>::batch_fin,                  \
+                                      TH::kL2Normalize ? 1 : 0};                            \
     return &ops;                                                                            \
   }
 

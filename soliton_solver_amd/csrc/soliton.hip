@@ -21,6 +21,7 @@ const TheoryOps* ops_chiral_magnet();
 const TheoryOps* ops_ginzburg_landau();
 const TheoryOps* ops_baby_skyrme();
 const TheoryOps* ops_anisotropic_sc();
+const TheoryOps* ops_bec();
 }  // namespace ss
 
 using namespace ss;
@@ -33,6 +34,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_GINZBURG_LANDAU: return ops_ginzburg_landau();
     case SS_THEORY_BABY_SKYRME: return ops_baby_skyrme();
     case SS_THEORY_ANISOTROPIC_SC: return ops_anisotropic_sc();
+    case SS_THEORY_BEC: return ops_bec();
     default: return nullptr;
   }
 }
@@ -120,6 +122,37 @@ template <typename A, typename B>
 __global__ void k_convert(const A* src, B* dst, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     dst[i] = B(src[i]);
+}
+
+// L2 normalisation of a field (core/integrator.py:295-321 with the theory's
+// compute_norm / do_norm_kernel): sum |f|^2 gv over all components, then
+// f /= sqrt(sum).  Partial sums per block, one-block finalize.
+__global__ void __launch_bounds__(256) k_l2_partial(const double* G, long long n, double gv, double* part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    s += G[i] * G[i] * gv;
+  const double b = block_sum<256>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+__global__ void __launch_bounds__(1024) k_fin_sqrt(const double* part, int n, double* out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += part[i];
+  const double t = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) out[0] = sqrt(t);
+}
+
+__global__ void k_div_scalar(double* G, long long n, const double* s) {
+  const double d = s[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    G[i] = G[i] / d;
+}
+
+__global__ void k_scale_into(double* dst, const double* src, long long n, double a) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = a * src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -293,13 +326,14 @@ static PassBufs bufs_for(ss_ctx* c, int parity) {
 }
 
 static int launch_step_exact(ss_ctx* c, int parity);
+static int launch_step_l2(ss_ctx* c, int parity);
 
 // Launches of one step from buffer parity ``parity``.  With ``ev`` non-null an
 // event is recorded before the first and after every launch.
 static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
-  if (c->var_mode == SS_VAR_EXACT_ADJOINT) {
+  if (c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) {
     if (ev) return fail(c, "pass timing is available for the fused reference flow only");
-    return launch_step_exact(c, parity);
+    return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
@@ -335,7 +369,9 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
 }
 
 static int ensure_exact(ss_ctx* c) {
-  if (c->prec != SS_PREC_FP64) return fail(c, "the exact-adjoint variation needs FP64 storage");
+  if (c->prec != SS_PREC_FP64)
+    return fail(c, c->ops->l2_normalize ? "the L2-normalised flow of %s needs FP64 storage"
+                                        : "the exact-adjoint variation needs FP64 storage", c->ops->name);
   const size_t nd = (size_t)c->g.nf * (size_t)c->g.N;
   if (!c->G) {
     HIPCHK(c, hipMalloc(&c->temp, nd * sizeof(double)));
@@ -384,6 +420,61 @@ static int launch_step_exact(ss_ctx* c, int parity) {
                      nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+static int l2_normalize_buf(ss_ctx* c, double* G) {
+  const long long n = (long long)c->g.nf * c->g.N;
+  int nb = (int)((n + 255) / 256);
+  if (nb > 1024) nb = 1024;
+  if (nb > c->xpart_n) nb = c->xpart_n;
+  hipLaunchKernelGGL(k_l2_partial, dim3(nb), dim3(256), 0, c->stream, (const double*)G, n, c->g.gv, c->xpart);
+  hipLaunchKernelGGL(k_fin_sqrt, dim3(1), dim3(1024), 0, c->stream, (const double*)c->xpart, nb, c->scal + 1);
+  hipLaunchKernelGGL(k_div_scalar, dim3(nb), dim3(256), 0, c->stream, G, n, (const double*)(c->scal + 1));
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// One step of the unfused L2-normalised reference flow (Bose-Einstein
+// condensate): every stage field and the combined field are normalised as
+// core/integrator.py:295-337 does.  FP64 storage only.
+static int launch_step_l2(ss_ctx* c, int parity) {
+  if (ensure_exact(c)) return -1;
+  PassBufs b = bufs_for(c, parity);
+  HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+                     c->hist, c->hist_cap, c->g.killkinen);
+  HIPCHK(c, hipGetLastError());
+  const double ca[4] = {0.0, 0.5, 0.5, 1.0};
+  const double cb[4] = {0.0, 0.0, 0.5, 0.5};
+  const void* lprev[4] = {nullptr, nullptr, c->l[0], c->l[1]};
+  const long long n = (long long)c->g.nf * c->g.N;
+  const int nb = (int)((n + 255) / 256);
+  const dim3 grid(nb < 4096 ? nb : 4096);
+  for (int st = 0; st < 4; ++st) {
+    hipLaunchKernelGGL((stage_field<double>), grid, dim3(256), 0, c->stream, (double*)c->temp, (const double*)b.F,
+                       (const double*)b.V, (const double*)lprev[st], ca[st], cb[st], st > 0 ? 1 : 0,
+                       (const DevState*)c->st, c->g);
+    HIPCHK(c, hipGetLastError());
+    if (l2_normalize_buf(c, (double*)c->temp)) return -1;
+    PassBufs v{};
+    v.F = c->temp;
+    v.lout = c->G;
+    v.st = c->st;
+    HIPCHK(c, c->ops->pass(c->prec, 0, v, c->tp, c->g, c->stream));
+    if (st < 3) {
+      hipLaunchKernelGGL(k_scale_into, grid, dim3(256), 0, c->stream, (double*)c->l[st], (const double*)c->G, n,
+                         -c->g.dt);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  b.part0 = c->xpart;
+  b.part1 = c->xpart + c->xpart_n;
+  int nblk = 0;
+  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->g, c->stream, &nblk));
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
+                     nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
+  return l2_normalize_buf(c, (double*)b.Fout);
 }
 
 // After the last step of a call: E(F_final), the last kill decision, and the
@@ -872,6 +963,8 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
   const TheoryOps* ops = theory_ops(theory_id);
   if (!ops) return bfail(nullptr, -1, "unknown or unbuilt theory id %d", theory_id);
   if (ninst < 1) return bfail(nullptr, -1, "ninst must be >= 1");
+  if (ops->l2_normalize)
+    return bfail(nullptr, -1, "theory %s L2-normalises its stage fields; the batched form does not", ops->name);
   if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return bfail(nullptr, -1, "bad precision %d", precision);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, -2, "no HIP device available");

@@ -6,6 +6,7 @@ import numpy as np
 from oracle import core as ocore
 from oracle.theories import anisotropic_superconductor as o_as
 from oracle.theories import baby_skyrme as o_bs
+from oracle.theories import bose_einstein_condensate as o_bec
 from oracle.theories import chiral_magnet as o_cm
 from oracle.theories import ginzburg_landau as o_gl
 
@@ -14,6 +15,7 @@ ORACLES = {
     "Ginzburg-Landau superconductor": o_gl,
     "Baby Skyrme model": o_bs,
     "Anisotropic superconductor": o_as,
+    "Bose-Einstein condensate": o_bec,
 }
 
 
@@ -121,4 +123,21 @@ def as_field(theory, p_i, p_f, seed=0, amp=1e-2, mode="initial"):
     V = np.zeros_like(F)
     theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
                                      config={"mode": mode, "noise": amp, "seed": seed})
+    return F
+
+
+def bec_params(theory, n=48, m=None, xsize=12.0, **kw):
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), beta=50.0, omega_rot=0.5, vortex_number=2.0)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def bec_field(theory, p_i, p_f, seed=0, amp=1e-3):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "ground", "noise": amp, "seed": seed})
     return F

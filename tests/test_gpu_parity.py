@@ -270,3 +270,23 @@ def test_anisotropic_steps_match_oracle():
     Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Anisotropic superconductor", th, p, F, V, 20, [20])
     assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
     assert np.abs(Fg - Fo).max() <= 1e-9
+
+
+def test_bec_variation_energy_and_normalised_steps():
+    th = ssa.load_theory("Bose-Einstein condensate")
+    p = H.bec_params(th, 56, 48)
+    p_i, p_f = H.packed(th, p)
+    F = H.bec_field(th, p_i, p_f, seed=3)
+    o = H.ORACLES["Bose-Einstein condensate"]
+    sim = _sim(th, p, F)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F, D1, D2, p_i, p_f)
+    assert np.abs(sim.energy_gradient() - eg_ref).max() <= 1e-11 * np.abs(eg_ref).max()
+    assert H.rel(sim.compute_energy(), ocore.reference_sum(o.energy_density(F, D1, p_i, p_f))) < 1e-12
+    sim.close()
+    V = np.zeros_like(F)
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Bose-Einstein condensate", th, p, F, V, 20, [8, 12])
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9
+    assert abs((Fg ** 2).sum() * p_f[4] - 1.0) < 1e-12

@@ -370,3 +370,32 @@ def test_anisotropic_vacuum_has_zero_energy():
     F = H.as_field(th, p_i, p_f, amp=0.0, mode="ground")
     E = ocore.theory_energy(o_as, F, p_i, p_f)
     assert abs(E) < 1e-12 * 24 * 24
+
+
+def test_bec_variation_is_functional_derivative():
+    from oracle.theories import bose_einstein_condensate as o_bec
+    th = ssa.load_theory("BEC")
+    errs = []
+    for n in (61, 121):
+        p = H.bec_params(th, n, n, xsize=6.0, beta=3.0, omega_rot=0.4)
+        p_i, p_f = H.packed(th, p)
+        g = H.grid_of(p_i, p_f)
+        F = np.stack([0.5 * np.exp(-((g[0] - 3) ** 2 + (g[1] - 3) ** 2) / 4) * np.cos(0.3 * g[1]),
+                      0.4 * np.exp(-((g[0] - 3) ** 2 + (g[1] - 3) ** 2) / 5) * np.sin(0.2 * g[0])])
+        b = _bump(p_i, p_f, w=0.35)
+        Psi = np.stack([b, 0.6 * b])
+        errs.append(_dir_mismatch(o_bec, F, Psi, p_i, p_f))
+    print("BEC directional mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_bec_step_keeps_unit_norm():
+    from oracle.theories import bose_einstein_condensate as o_bec
+    th = ssa.load_theory("BEC")
+    p = H.bec_params(th, 40, 40)
+    p_i, p_f = H.packed(th, p)
+    F = H.bec_field(th, p_i, p_f)
+    V = np.zeros_like(F)
+    ocore.relax(o_bec, F, V, p_i, p_f, 3)
+    assert abs(((F ** 2).sum(0) * p_f[4]).sum() - 1.0) < 1e-13
