@@ -110,7 +110,9 @@ int ss_variation(ss_ctx* ctx, int mode, double* host_eg, size_t n);
 
 /* Device-pointer variation for the autograd boundary: field_dev and grad_dev
  * hold nf*xlen*ylen values of the context's precision in reference layout;
- * runs on ``stream`` (hipStream_t, NULL = the context's stream). */
+ * runs on ``stream`` (a hipStream_t; NULL is the null stream, as for any HIP
+ * call).  These calls share the context's scratch and partial buffers: they
+ * must not overlap with each other or with ss_step on the same context. */
 int ss_variation_dev(ss_ctx* ctx, const void* field_dev, void* grad_dev, int mode, void* stream);
 
 /* Device-pointer energy: writes the FP64 total into energy_dev[0]. */

@@ -79,26 +79,109 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, c
   const R dt = R(g.dt);
   const R half = R(0.5);
 
-  load_tile<R, NF>(tile, g, x0, y0, [&](int a, long long off) -> R {
-    if (PASS <= 1) return F[off];
-    const R v = kill ? R(0) : V[off];
-    if (PASS == 2) return F[off] + half * (dt * v);
-    if (PASS == 3) return F[off] + half * (dt * (v + half * l1[off]));
-    return F[off] + dt * (v + half * l2[off]);
-  });
+  constexpr int RW = S::kRowsPerWave;
+  constexpr int RY = S::RY, TXr = S::TX;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int y = y0 + lane;
+
+  // Phase A: the interior sites this thread computes (rows w*RW..w*RW+RW-1 of
+  // column ``lane``), one coalesced row of 64 values per wave and component.
+  // Pass 4 keeps V and l2 of its own sites in registers for the combine.
+  R Vk[PASS == 4 ? RW : 1][NF], L2k[PASS == 4 ? RW : 1][NF];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int lx = w * RW + k, x = x0 + lx;
+    const bool in = (x < g.X) && (y < g.Y);
+    const long long off = (long long)x * g.Y + y;
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const long long o = off + (long long)a * g.N;
+      R val = R(0);
+      if (PASS == 4) {
+        Vk[k][a] = R(0);
+        L2k[k][a] = R(0);
+      }
+      if (in) {
+        if (PASS <= 1) {
+          val = F[o];
+        } else {
+          const R fo = F[o];
+          const R v = kill ? R(0) : V[o];
+          if (PASS == 2) {
+            val = fo + half * (dt * v);
+          } else if (PASS == 3) {
+            val = fo + half * (dt * (v + half * l1[o]));
+          } else {
+            const R q = l2[o];
+            val = fo + dt * (v + half * q);
+            Vk[k][a] = v;
+            L2k[k][a] = q;
+          }
+        }
+      }
+      tile.v[a][lx + 2][lane + 2] = val;
+    }
+  }
+  // Phase B: the 2-site halo ring (2 rows above and below, 2 columns either side).
+  constexpr int kRing = 4 * RY + 4 * TXr;
+  for (int idx = threadIdx.x; idx < kRing; idx += kThreads) {
+    int r, c;
+    if (idx < 2 * RY) {
+      r = idx / RY;
+      c = idx - r * RY;
+    } else if (idx < 4 * RY) {
+      const int j = idx - 2 * RY;
+      r = TXr + 2 + j / RY;
+      c = j - (j / RY) * RY;
+    } else {
+      const int j = idx - 4 * RY;
+      r = 2 + (j >> 2);
+      const int q = j & 3;
+      c = q < 2 ? q : RY - 4 + q;
+    }
+    const int gx = x0 - 2 + r, gy = y0 - 2 + c;
+    const bool in = (gx >= 0) && (gx < g.X) && (gy >= 0) && (gy < g.Y);
+    const long long off = (long long)gx * g.Y + gy;
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const long long o = off + (long long)a * g.N;
+      R val = R(0);
+      if (in) {
+        if (PASS <= 1) {
+          val = F[o];
+        } else {
+          const R v = kill ? R(0) : V[o];
+          if (PASS == 2) val = F[o] + half * (dt * v);
+          else if (PASS == 3) val = F[o] + half * (dt * (v + half * l1[o]));
+          else val = F[o] + dt * (v + half * l2[o]);
+        }
+      }
+      tile.v[a][r][c] = val;
+    }
+  }
   __syncthreads();
 
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // Phase C: march down the thread's rows with an x-window in registers.
+  const int cy = lane + 2;
+  R win[NF][5];
+#pragma unroll
+  for (int a = 0; a < NF; ++a) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) win[a][j] = tile.v[a][w * RW + j][cy];
+  }
   double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
   double acc1 = 0.0;
 #pragma unroll
-  for (int k = 0; k < S::kRowsPerWave; ++k) {
-    const int lx = w * S::kRowsPerWave + k, ly = lane;
-    const int x = x0 + lx, y = y0 + ly;
+  for (int k = 0; k < RW; ++k) {
+    const int lx = w * RW + k, cx = lx + 2;
+    const int x = x0 + lx;
+#pragma unroll
+    for (int a = 0; a < NF; ++a) win[a][4] = tile.v[a][cx + 2][cy];
     if (x < g.X && y < g.Y) {
       const SiteInfo s = site_info(g, x, y);
       R f[NF], d1[NF][2], d2[NF][3], eg[NF];
-      tile_derivs<R, R, NF>(tile, lx, ly, g, s, f, d1, d2);
+#pragma unroll
+      for (int a = 0; a < NF; ++a) window_derivs<R, R, NF>(tile, a, win[a], cx, cy, g, s, f[a], d1[a], d2[a]);
       TH::template variation<R>(tp, g, s, f, d1, d2, eg);
       const long long off = (long long)x * g.Y + y;
       if (PASS == 0) {
@@ -111,26 +194,28 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, c
         for (int a = 0; a < NF; ++a) out[off + a * g.N] = -dt * eg[a];
       }
       if (PASS == 1) {
+        // FP64 energy density from the same window values (no LDS re-read).
         double fd[NF], d1d[NF][2];
-        tile_d1<double, R, NF>(tile, lx, ly, g, s, fd, d1d);
+#pragma unroll
+        for (int a = 0; a < NF; ++a) window_d1<double, R, NF>(tile, a, win[a], cx, cy, g, s, fd[a], d1d[a]);
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
       if (PASS == 4) {
+        // RK4 combine.  With Temp4 = F + dt (V + l2/2) at the centre (f, from
+        // LDS), F' = F + (k1 + 2 k2 + 2 k3 + k4)/6 = Temp4 + dt (l1 - 2 l2 + l3)/6,
+        // so F is not re-read; V and l2 are the registers kept in phase A.
         R Fn[NF], Vn[NF];
         const R sixth = R(1.0 / 6.0);
+        const R dt6 = dt * sixth;
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
           const long long o = off + a * g.N;
-          const R Fc = F[o];
-          const R Vc = kill ? R(0) : V[o];
-          const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
+          const R Vc = Vk[k][a];
+          const R a2 = L2k[k][a];
+          const R a1 = l1[o], a3 = l3[o];
           const R a4 = -dt * eg[a];
-          const R k1 = dt * Vc;
-          const R k2 = dt * (Vc + half * a1);
-          const R k3 = dt * (Vc + half * a2);
-          const R k4 = dt * (Vc + a3);
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          Fn[a] = Fc + sixth * (k1 + R(2) * k2 + R(2) * k3 + k4);
+          Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
           const double e = (double)eg[a];
           acc0 = fmax(acc0, e);
           acc1 = fmax(acc1, fabs(e));
@@ -147,6 +232,11 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, c
           Vo[off + a * g.N] = Vn[a];
         }
       }
+    }
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) win[a][j] = win[a][j + 1];
     }
   }
   if (PASS == 1) {

@@ -845,14 +845,14 @@ int ss_variation(ss_ctx* c, int mode, double* host_eg, size_t n) {
 int ss_variation_dev(ss_ctx* c, const void* field_dev, void* grad_dev, int mode, void* stream) {
   if (!c) return fail(nullptr, "null context");
   if (!field_dev || !grad_dev) return fail(c, "null device pointer");
-  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
   return variation_into(c, field_dev, grad_dev, mode, st);
 }
 
 int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* stream) {
   if (!c) return fail(nullptr, "null context");
   if (!field_dev || !energy_dev) return fail(c, "null device pointer");
-  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
   HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, c->part, c->tp, c->g, st));
   hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->ntiles, energy_dev);
   HIPCHK(c, hipGetLastError());

@@ -94,6 +94,47 @@ __device__ __forceinline__ void tile_derivs(const Tile<R, NF>& t, int lx, int ly
   }
 }
 
+// Derivatives of one component from its x-window (x-2..x+2 at the site's
+// column, held in registers by a thread marching along x) plus the y and
+// diagonal neighbours read from LDS at tile row cx, column cy.
+template <typename T, typename R, int NF>
+__device__ __forceinline__ void window_derivs(const Tile<R, NF>& t, int a, const R win[5], int cx, int cy,
+                                              const Geom& g, const SiteInfo& s, T& f, T d1[2], T d2[3]) {
+  const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
+  const T c = T(win[2]);
+  const T xm2 = T(win[0]), xm1 = T(win[1]), xp1 = T(win[3]), xp2 = T(win[4]);
+  const T ym2 = T(t.v[a][cx][cy - 2]), ym1 = T(t.v[a][cx][cy - 1]);
+  const T yp1 = T(t.v[a][cx][cy + 1]), yp2 = T(t.v[a][cx][cy + 2]);
+  const T sxx = c43 * ((xm1 - c) + (xp1 - c)) - c12 * ((xm2 - c) + (xp2 - c));
+  const T syy = c43 * ((ym1 - c) + (yp1 - c)) - c12 * ((ym2 - c) + (yp2 - c));
+  f = c;
+  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
+  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
+  d2[0] = s.inx ? sxx * T(g.inv_lsx2) : T(0);
+  d2[1] = s.iny ? syy * T(g.inv_lsy2) : T(0);
+  if (s.inxy) {
+    const T dm2 = T(t.v[a][cx - 2][cy - 2]), dm1 = T(t.v[a][cx - 1][cy - 1]);
+    const T dp1 = T(t.v[a][cx + 1][cy + 1]), dp2 = T(t.v[a][cx + 2][cy + 2]);
+    const T sdd = c43 * ((dm1 - c) + (dp1 - c)) - c12 * ((dm2 - c) + (dp2 - c));
+    d2[2] = sdd * T(g.inv_2gv) - sxx * T(g.inv_2lsx2) - syy * T(g.inv_2lsy2);
+  } else {
+    d2[2] = T(0);
+  }
+}
+
+// First derivatives of one component from the same window, in type T.
+template <typename T, typename R, int NF>
+__device__ __forceinline__ void window_d1(const Tile<R, NF>& t, int a, const R win[5], int cx, int cy,
+                                          const Geom& g, const SiteInfo& s, T& f, T d1[2]) {
+  const T c12 = T(SS_C12), c23 = T(SS_C23);
+  const T xm2 = T(win[0]), xm1 = T(win[1]), xp1 = T(win[3]), xp2 = T(win[4]);
+  const T ym2 = T(t.v[a][cx][cy - 2]), ym1 = T(t.v[a][cx][cy - 1]);
+  const T yp1 = T(t.v[a][cx][cy + 1]), yp2 = T(t.v[a][cx][cy + 2]);
+  f = T(win[2]);
+  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
+  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
+}
+
 // First derivatives only (energies and charges).
 template <typename T, typename R, int NF>
 __device__ __forceinline__ void tile_d1(const Tile<R, NF>& t, int lx, int ly, const Geom& g,

@@ -104,13 +104,29 @@ def bs_field(theory, p_i, p_f, seed=0, k=1, amp=1e-3, kind="neel"):
     return F
 
 
+def as_betas(seed, alpha1=-1.0, alpha2=-0.5):
+    """beta1..beta4 ~ U[0.2, 1] from the seed, redrawn until the quartic form
+    beta1 s1^2 + beta2 s2^2 + (beta3 - 2 beta4) s1 s2 is bounded below (with
+    beta4 > 0 the phase locks at theta = pi/2) and both vacuum values are real
+    and non-zero; otherwise the energy is unbounded below and no relaxation
+    converges."""
+    rng = np.random.default_rng(seed)
+    while True:
+        b1, b2, b3, b4 = rng.uniform(0.2, 1.0, size=4)
+        b3t = b3 - 2.0 * b4
+        den = b3t * b3t - 4.0 * b1 * b2
+        n1 = 2.0 * b2 * alpha1 - b3t * alpha2
+        n2 = 2.0 * b1 * alpha2 - b3t * alpha1
+        if b3t > -2.0 * np.sqrt(b1 * b2) + 0.1 and den != 0 and n1 / den > 0 and n2 / den > 0:
+            return b1, b2, b3, b4
+
+
 def as_params(theory, n=48, m=None, xsize=25.0, seed=0, **kw):
     """Config-5 style: alpha1 = -1, alpha2 = -0.5, beta1..beta4 ~ U[0.2, 1] from the seed,
     gamma1 = gamma2 = 1, gamma12 = 0.3."""
     m = m or n
     h = xsize / (n - 1)
-    rng = np.random.default_rng(seed)
-    b = rng.uniform(0.2, 1.0, size=4)
+    b = as_betas(seed)
     base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), alpha1=-1.0, alpha2=-0.5, beta1=b[0], beta2=b[1],
                 beta3=b[2], beta4=b[3], gamma1=1.0, gamma2=1.0, gamma12=0.3, Q=1.0, kappa=0.673274)
     base.update(kw)
