@@ -288,9 +288,15 @@ density_pass(const void* Fv, double* dens, double* part, int kind, typename TH::
     if (x < g.X && y < g.Y) {
       const SiteInfo s = site_info(g, x, y);
       double fd[NF], d1d[NF][2];
-      tile_d1<double, R, NF>(tile, lx, ly, g, s, fd, d1d);
-      const double v = (kind < 0 ? TH::template energy<double>(tp, g, s, fd, d1d)
-                                 : TH::template charge<double>(kind, tp, g, s, fd, d1d)) * g.gv;
+      double v;
+      if (kind < 0) {
+        tile_d1<double, R, NF>(tile, lx, ly, g, s, fd, d1d);
+        v = TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+      } else {
+        double d2d[NF][3];
+        tile_derivs<double, R, NF>(tile, lx, ly, g, s, fd, d1d, d2d);
+        v = TH::template charge<double>(kind, tp, g, s, fd, d1d, d2d) * g.gv;
+      }
       if (dens) dens[(long long)x * g.Y + y] = v;
       acc += v;
     }

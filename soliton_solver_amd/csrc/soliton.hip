@@ -22,6 +22,7 @@ const TheoryOps* ops_ginzburg_landau();
 const TheoryOps* ops_baby_skyrme();
 const TheoryOps* ops_anisotropic_sc();
 const TheoryOps* ops_bec();
+const TheoryOps* ops_mcsh();
 }  // namespace ss
 
 using namespace ss;
@@ -35,6 +36,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_BABY_SKYRME: return ops_baby_skyrme();
     case SS_THEORY_ANISOTROPIC_SC: return ops_anisotropic_sc();
     case SS_THEORY_BEC: return ops_bec();
+    case SS_THEORY_MAXWELL_CHERN_SIMONS_HIGGS: return ops_mcsh();
     default: return nullptr;
   }
 }

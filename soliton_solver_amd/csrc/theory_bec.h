@@ -81,7 +81,7 @@ struct BoseEinsteinCondensate {
   // |psi|^2 (compute_norm_kernel, kernels.py:196-225, without gv).
   template <typename T>
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
-                                             const T f[NF], const T d1[NF][2]) {
+                                             const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     return f[0] * f[0] + f[1] * f[1];
   }
 

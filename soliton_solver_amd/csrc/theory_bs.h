@@ -157,7 +157,7 @@ struct BabySkyrme {
   // Skyrmion number density m.(d_x m x d_y m)/(4 pi) (kernels.py:236-270).
   template <typename T>
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
-                                             const T f[NF], const T d1[NF][2]) {
+                                             const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     const T cx0 = d1[1][0] * d1[2][1] - d1[2][0] * d1[1][1];
     const T cx1 = d1[2][0] * d1[0][1] - d1[0][0] * d1[2][1];
     const T cx2 = d1[0][0] * d1[1][1] - d1[1][0] * d1[0][1];

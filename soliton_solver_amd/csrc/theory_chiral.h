@@ -130,7 +130,7 @@ struct ChiralMagnet {
   // 0: m.(d_x m x d_y m)/(4 pi) (kernels.py:253-291); 1: -(d_x m1 + d_y m2) (kernels.py:328-360).
   template <typename T>
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
-                                             const T f[NF], const T d1[NF][2]) {
+                                             const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     if (kind == 0) {
       const T cx0 = d1[1][0] * d1[2][1] - d1[2][0] * d1[1][1];
       const T cx1 = d1[2][0] * d1[0][1] - d1[0][0] * d1[2][1];

@@ -79,7 +79,7 @@ struct GinzburgLandau {
   // Vortex number density (d_x A2 - d_y A1)/(2 pi q) (kernels.py:145-178).
   template <typename T>
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
-                                             const T f[NF], const T d1[NF][2]) {
+                                             const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.q));
   }
 

@@ -8,6 +8,7 @@ from oracle.theories import anisotropic_superconductor as o_as
 from oracle.theories import baby_skyrme as o_bs
 from oracle.theories import bose_einstein_condensate as o_bec
 from oracle.theories import chiral_magnet as o_cm
+from oracle.theories import maxwell_chern_simons_higgs as o_mcsh
 from oracle.theories import ginzburg_landau as o_gl
 
 ORACLES = {
@@ -16,6 +17,7 @@ ORACLES = {
     "Baby Skyrme model": o_bs,
     "Anisotropic superconductor": o_as,
     "Bose-Einstein condensate": o_bec,
+    "Anyon superconductor": o_mcsh,
 }
 
 
@@ -156,4 +158,23 @@ def bec_field(theory, p_i, p_f, seed=0, amp=1e-3):
     V = np.zeros_like(F)
     theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
                                      config={"mode": "ground", "noise": amp, "seed": seed})
+    return F
+
+
+def mcsh_params(theory, n=48, m=None, xsize=12.0, **kw):
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), q=1.0, Lambda=1.5, kappa=0.7, u1=1.0)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def mcsh_field(theory, p_i, p_f, seed=0, amp=1e-2):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "initial", "noise": amp, "seed": seed})
+    g = grid_of(p_i, p_f)
+    F[4] = 0.1 * np.exp(-((g[0] - g[0].mean()) ** 2 + (g[1] - g[1].mean()) ** 2) / 4.0)
     return F

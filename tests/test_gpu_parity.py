@@ -290,3 +290,27 @@ def test_bec_variation_energy_and_normalised_steps():
     assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
     assert np.abs(Fg - Fo).max() <= 1e-9
     assert abs((Fg ** 2).sum() * p_f[4] - 1.0) < 1e-12
+
+
+def test_mcsh_variation_energy_charges_and_steps():
+    th = ssa.load_theory("Anyon superconductor")
+    p = H.mcsh_params(th, 60, 52)
+    p_i, p_f = H.packed(th, p)
+    F = H.mcsh_field(th, p_i, p_f, seed=2)
+    o = H.ORACLES["Anyon superconductor"]
+    sim = _sim(th, p, F)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F, D1, D2, p_i, p_f)
+    assert np.abs(sim.energy_gradient() - eg_ref).max() <= 1e-11 * np.abs(eg_ref).max()
+    assert H.rel(sim.compute_energy(), ocore.reference_sum(o.energy_density(F, D1, p_i, p_f))) < 1e-12
+    obs = sim.observables()
+    assert abs(obs["vortex_number"] - ocore.reference_sum(o.vortex_density(F, D1, p_i, p_f))) < 1e-11
+    el = ocore.reference_sum(o.electric_charge_density(F, D1, D2, p_i, p_f))
+    assert abs(obs["electric_charge"] - el) < 1e-10 * max(1.0, abs(el))
+    nt = ocore.reference_sum(o.noether_charge_density(F, D1, p_i, p_f))
+    assert abs(th.observables.compute_noether_charge(sim) - nt) < 1e-12 * max(1.0, abs(nt))
+    sim.close()
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory("Anyon superconductor", th, p, F, np.zeros_like(F), 15, [15])
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9

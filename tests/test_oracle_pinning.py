@@ -399,3 +399,37 @@ def test_bec_step_keeps_unit_norm():
     V = np.zeros_like(F)
     ocore.relax(o_bec, F, V, p_i, p_f, 3)
     assert abs(((F ** 2).sum(0) * p_f[4]).sum() - 1.0) < 1e-13
+
+
+def test_mcsh_variation_matches_energy_and_a0_quirk():
+    """With kappa = 0: EG = dE_h/dphi at 4th order for A0 = 0; with A0 != 0 the
+    psi equations carry -A0^2 where dE/dpsi has +A0^2 (reference quirk), so
+    EG.Psi - dE = sum gv (-2 q^2 A0^2 psi).Psi, also at 4th order."""
+    from oracle.theories import maxwell_chern_simons_higgs as o_m
+    th = ssa.load_theory("Anyon superconductor")
+    for with_a0 in (False, True):
+        errs = []
+        for n in (61, 121):
+            p = H.mcsh_params(th, n, n, xsize=8.0, kappa=0.0, q=0.9)
+            p_i, p_f = H.packed(th, p)
+            g = H.grid_of(p_i, p_f)
+            F = np.zeros((5, n, n))
+            F[0] = 0.3 * np.sin(0.7 * g[0] + 0.2 * g[1])
+            F[1] = 0.2 * np.cos(0.5 * g[1] - 0.3 * g[0])
+            F[2] = 0.7 + 0.2 * np.sin(0.5 * g[0])
+            F[3] = 0.3 * np.cos(0.4 * g[1])
+            F[4] = (0.25 * np.exp(-((g[0] - 4) ** 2 + (g[1] - 4) ** 2) / 6.0)) if with_a0 else 0.0
+            b = _bump(p_i, p_f, w=0.35)
+            Psi = np.stack([b, 0.5 * b, 0.8 * b, -0.4 * b, 0.6 * b])
+            eps = 1e-5
+            dE = (ocore.theory_energy(o_m, F + eps * Psi, p_i, p_f) - ocore.theory_energy(o_m, F - eps * Psi, p_i, p_f)) / (2 * eps)
+            D1 = ocore.first_derivatives(F, p_i, p_f)
+            D2 = ocore.second_derivatives(F, p_i, p_f)
+            EG = o_m.variation(F, D1, D2, p_i, p_f)
+            G = (EG * Psi).sum() * p_f[4]
+            q = float(p_f[6])
+            quirk = ((-2.0 * q * q * F[4] ** 2) * (F[2] * Psi[2] + F[3] * Psi[3])).sum() * p_f[4]
+            errs.append(abs(G - dE - quirk) / abs(dE))
+        print("MCSH mismatch (A0 on=%s):" % with_a0, errs)
+        assert errs[1] < 1e-3
+        assert errs[0] / errs[1] > 8.0
