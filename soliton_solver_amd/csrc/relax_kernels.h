@@ -221,8 +221,8 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, c
           acc1 = fmax(acc1, fabs(e));
         }
         if (TH::kUnitConstraint && g.unit_mag) {
-          TH::template normalize<R>(Fn);
-          TH::template project<R>(Vn, Fn);
+          TH::template normalize<R>(tp, Fn);
+          TH::template project<R>(tp, Vn, Fn);
         }
         R* Fo = (R*)b.Fout;
         R* Vo = (R*)b.Vout;
@@ -426,7 +426,7 @@ __global__ void stage_field(R* Temp, const R* F, const R* V, const R* lp, double
 template <class TH, typename R>
 __global__ void __launch_bounds__(kThreads)
 combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, const double* EG4, R* Fo, R* Vo,
-             double* part0, double* part1, const DevState* st, Geom g) {
+             double* part0, double* part1, const DevState* st, Geom g, typename TH::TP tp) {
   constexpr int NF = TH::NF;
   __shared__ double red[kWaves];
   const bool kill = st->kill != 0;
@@ -450,8 +450,8 @@ combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, cons
       m1 = fmax(m1, fabs(e));
     }
     if (TH::kUnitConstraint && g.unit_mag) {
-      TH::template normalize<R>(Fn);
-      TH::template project<R>(Vn, Fn);
+      TH::template normalize<R>(tp, Fn);
+      TH::template project<R>(tp, Vn, Fn);
     }
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
@@ -695,18 +695,18 @@ struct TheoryLaunchers {
     return hipGetLastError();
   }
 
-  static hipError_t combine(int prec, const PassBufs& b, const double* EG4, const Geom& g, hipStream_t st,
+  static hipError_t combine(int prec, const PassBufs& b, const double* EG4, const void* tpv, const Geom& g, hipStream_t st,
                             int* nblocks) {
     const int nb = (int)((g.N + kThreads - 1) / kThreads);
     *nblocks = nb;
     if (prec)
       hipLaunchKernelGGL((combine_pass<TH, float>), dim3(nb), dim3(kThreads), 0, st, (const float*)b.F,
                          (const float*)b.V, (const float*)b.l1, (const float*)b.l2, (const float*)b.l3, EG4,
-                         (float*)b.Fout, (float*)b.Vout, b.part0, b.part1, b.st, g);
+                         (float*)b.Fout, (float*)b.Vout, b.part0, b.part1, b.st, g, *(const TP*)tpv);
     else
       hipLaunchKernelGGL((combine_pass<TH, double>), dim3(nb), dim3(kThreads), 0, st, (const double*)b.F,
                          (const double*)b.V, (const double*)b.l1, (const double*)b.l2, (const double*)b.l3, EG4,
-                         (double*)b.Fout, (double*)b.Vout, b.part0, b.part1, b.st, g);
+                         (double*)b.Fout, (double*)b.Vout, b.part0, b.part1, b.st, g, *(const TP*)tpv);
     return hipGetLastError();
   }
 
@@ -772,7 +772,7 @@ struct TheoryOps {
   hipError_t (*density)(int, int, const void*, double*, double*, const void*, const Geom&, hipStream_t);
   hipError_t (*adjoint)(int, const void*, double*, double*, double*, const void*, void*, int, const void*,
                         const Geom&, hipStream_t);
-  hipError_t (*combine)(int, const PassBufs&, const double*, const Geom&, hipStream_t, int*);
+  hipError_t (*combine)(int, const PassBufs&, const double*, const void*, const Geom&, hipStream_t, int*);
   int (*batch_inst_size)();
   int (*tile_rows)(int);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);

@@ -23,6 +23,7 @@ const TheoryOps* ops_baby_skyrme();
 const TheoryOps* ops_anisotropic_sc();
 const TheoryOps* ops_bec();
 const TheoryOps* ops_mcsh();
+const TheoryOps* ops_fs();
 }  // namespace ss
 
 using namespace ss;
@@ -37,6 +38,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_ANISOTROPIC_SC: return ops_anisotropic_sc();
     case SS_THEORY_BEC: return ops_bec();
     case SS_THEORY_MAXWELL_CHERN_SIMONS_HIGGS: return ops_mcsh();
+    case SS_THEORY_FERROMAGNETIC_SC: return ops_fs();
     default: return nullptr;
   }
 }
@@ -417,7 +419,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   b.part0 = c->xpart;
   b.part1 = c->xpart + c->xpart_n;
   int nblk = 0;
-  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->g, c->stream, &nblk));
+  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->tp, c->g, c->stream, &nblk));
   hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
                      nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
   HIPCHK(c, hipGetLastError());
@@ -472,7 +474,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   b.part0 = c->xpart;
   b.part1 = c->xpart + c->xpart_n;
   int nblk = 0;
-  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->g, c->stream, &nblk));
+  HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->tp, c->g, c->stream, &nblk));
   hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
                      nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
   HIPCHK(c, hipGetLastError());

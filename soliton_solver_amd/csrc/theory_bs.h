@@ -112,7 +112,7 @@ struct BabySkyrme {
       eg[2] -= mpi2 * nrm;
     }
     if (tp.doublevacua) eg[2] -= T(2) * mpi2 * m3;
-    project(eg, f);
+    project(tp, eg, f);
   }
 
   // kernels.py:126-199, without the gv factor.
@@ -166,11 +166,11 @@ struct BabySkyrme {
 
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
-    project(eg, f);
+    project(tp, eg, f);
   }
 
   template <typename T>
-  __device__ __forceinline__ static void normalize(T f[NF]) {
+  __device__ __forceinline__ static void normalize(const TP& tp, T f[NF]) {
     const T n = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
     if (n != T(0)) {
       f[0] = f[0] / n;
@@ -180,7 +180,7 @@ struct BabySkyrme {
   }
 
   template <typename T>
-  __device__ __forceinline__ static void project(T v[NF], const T f[NF]) {
+  __device__ __forceinline__ static void project(const TP& tp, T v[NF], const T f[NF]) {
     const T lm = v[0] * f[0] + v[1] * f[1] + v[2] * f[2];
     v[0] -= lm * f[0];
     v[1] -= lm * f[1];

@@ -94,7 +94,7 @@ struct ChiralMagnet {
 #pragma unroll
       for (int a = 0; a < NF; ++a) eg[a] = T(0);
     }
-    project(eg, f);
+    project(tp, eg, f);
   }
 
   // Energy density without the gv factor (kernels.py:124-216).
@@ -146,12 +146,12 @@ struct ChiralMagnet {
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
     if (tp.demag) eg[3] = -T(tp.coupmu) * eg[3];
-    project(eg, f);
+    project(tp, eg, f);
   }
 
   // Unit-norm constraint (kernels.py:47-83): sites with |m| == 0 are left as is.
   template <typename T>
-  __device__ __forceinline__ static void normalize(T f[NF]) {
+  __device__ __forceinline__ static void normalize(const TP& tp, T f[NF]) {
     T s2 = T(0);
 #pragma unroll
     for (int a = 0; a < NMAG; ++a) s2 += f[a] * f[a];
@@ -164,7 +164,7 @@ struct ChiralMagnet {
 
   // v -= (v.m) m (kernels.py:85-119).
   template <typename T>
-  __device__ __forceinline__ static void project(T v[NF], const T f[NF]) {
+  __device__ __forceinline__ static void project(const TP& tp, T v[NF], const T f[NF]) {
     T lm = T(0);
 #pragma unroll
     for (int a = 0; a < NMAG; ++a) lm += v[a] * f[a];

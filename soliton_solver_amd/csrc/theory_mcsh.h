@@ -88,9 +88,9 @@ struct MaxwellChernSimonsHiggs {
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {}
   template <typename T>
-  __device__ __forceinline__ static void normalize(T f[NF]) {}
+  __device__ __forceinline__ static void normalize(const TP& tp, T f[NF]) {}
   template <typename T>
-  __device__ __forceinline__ static void project(T v[NF], const T f[NF]) {}
+  __device__ __forceinline__ static void project(const TP& tp, T v[NF], const T f[NF]) {}
 };
 
 }  // namespace ss

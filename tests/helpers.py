@@ -8,6 +8,7 @@ from oracle.theories import anisotropic_superconductor as o_as
 from oracle.theories import baby_skyrme as o_bs
 from oracle.theories import bose_einstein_condensate as o_bec
 from oracle.theories import chiral_magnet as o_cm
+from oracle.theories import ferromagnetic_superconductor as o_fs
 from oracle.theories import maxwell_chern_simons_higgs as o_mcsh
 from oracle.theories import ginzburg_landau as o_gl
 
@@ -18,6 +19,7 @@ ORACLES = {
     "Anisotropic superconductor": o_as,
     "Bose-Einstein condensate": o_bec,
     "Anyon superconductor": o_mcsh,
+    "Ferromagnetic superconductor": o_fs,
 }
 
 
@@ -177,4 +179,24 @@ def mcsh_field(theory, p_i, p_f, seed=0, amp=1e-2):
                                      config={"mode": "initial", "noise": amp, "seed": seed})
     g = grid_of(p_i, p_f)
     F[4] = 0.1 * np.exp(-((g[0] - g[0].mean()) ** 2 + (g[1] - g[1].mean()) ** 2) / 4.0)
+    return F
+
+
+def fs_params(theory, n=48, m=None, xsize=16.0, **kw):
+    """eta1 = -0.1 (eta2 derived = -0.0125), giving M0 = 1.369, u1 = 2.092."""
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), q=0.8, eta1=-0.1, gamma=1.2,
+                unit_magnetization=True)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def fs_field(theory, p_i, p_f, seed=0, amp=1e-2, ansatz="bloch", vortex_number=1.0):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "initial", "ansatz": ansatz, "vortex_number": vortex_number,
+                                             "noise": amp, "seed": seed})
     return F

@@ -433,3 +433,91 @@ def test_mcsh_variation_matches_energy_and_a0_quirk():
         print("MCSH mismatch (A0 on=%s):" % with_a0, errs)
         assert errs[1] < 1e-3
         assert errs[0] / errs[1] > 8.0
+
+
+def _fs_smooth(th, n, **kw):
+    p = H.fs_params(th, n, n, xsize=8.0, **kw)
+    p_i, p_f = H.packed(th, p)
+    g = H.grid_of(p_i, p_f)
+    M0 = float(p_f[17])
+    F = np.zeros((8, n, n))
+    F[0] = 0.6 * np.sin(0.5 * g[0] + 0.2 * g[1])
+    F[1] = 0.5 * np.cos(0.3 * g[1] - 0.4 * g[0])
+    F[2] = 0.8 + 0.2 * np.sin(0.4 * g[0])
+    F[:3] *= M0 / np.sqrt((F[:3] ** 2).sum(0))
+    F[3] = 0.3 * np.sin(0.7 * g[0] + 0.2 * g[1])
+    F[4] = 0.2 * np.cos(0.5 * g[1] - 0.3 * g[0])
+    F[5] = 0.25 * np.sin(0.3 * g[0] * g[1] / 8.0)
+    F[6] = 1.5 + 0.3 * np.sin(0.5 * g[0])
+    F[7] = 0.4 * np.cos(0.4 * g[1])
+    b = _bump(p_i, p_f, w=0.35)
+    w = np.stack([b * np.cos(0.6 * g[1]), 0.5 * b, b * (0.2 + 0.1 * g[0]), 0.7 * b, -0.4 * b, 0.9 * b, 0.5 * b,
+                  -0.8 * b])
+    w[:3] -= (w[:3] * F[:3]).sum(0) * F[:3] / M0 ** 2
+    return p_i, p_f, F, w
+
+
+@pytest.mark.parametrize("with_eta2", [False, True])
+def test_ferromagnetic_sc_variation_matches_energy_and_eta2_quirks(with_eta2):
+    """With eta2 = 0: sum gv EG.w = dE_h/de at 4th order for w tangent to m (|m| = M0).
+    With eta2 != 0 the reference's variation (kernels.py:591-614) drops the
+    grad|psi|^2 . grad m_a term of d/dm (eta2 |grad m|^2 |psi|^2) and uses
+    psi_b^2 where d/dpsi_b has psi_b, so
+    EG.w - dE = sum gv 2 eta2 [grad|psi|^2 . grad m . w_m + |grad m|^2 (psi_b^2 - psi_b) w_psi_b]."""
+    from oracle.theories import ferromagnetic_superconductor as o_fs
+    th = ssa.load_theory("Ferromagnetic superconductor")
+    errs = []
+    for n in (61, 121):
+        p_i, p_f, F, w = _fs_smooth(th, n, eta2=(0.3 if with_eta2 else 0.0))
+        eps = 1e-5
+        dE = (ocore.theory_energy(o_fs, F + eps * w, p_i, p_f) - ocore.theory_energy(o_fs, F - eps * w, p_i, p_f)) / (2 * eps)
+        D1 = ocore.first_derivatives(F, p_i, p_f)
+        D2 = ocore.second_derivatives(F, p_i, p_f)
+        EG = o_fs.variation(F.copy(), D1, D2, p_i, p_f)
+        G = (EG * w).sum() * p_f[4]
+        e2 = float(p_f[10])
+        psq_d = [2 * (F[6] * D1[6, i] + F[7] * D1[7, i]) for i in (0, 1)]
+        gradm2 = sum(D1[a, i] ** 2 for a in range(3) for i in (0, 1))
+        quirk = sum(2 * e2 * (psq_d[0] * D1[a, 0] + psq_d[1] * D1[a, 1]) * w[a] for a in range(3))
+        quirk = quirk + sum(2 * e2 * gradm2 * (F[b] ** 2 - F[b]) * w[b] for b in (6, 7))
+        errs.append(abs(G - dE - quirk.sum() * p_f[4]) / abs(dE))
+        if with_eta2:
+            assert abs(quirk.sum() * p_f[4]) > 1e-2 * abs(dE)
+    print("ferromagnetic SC mismatch (eta2 on=%s):" % with_eta2, errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_ferromagnetic_sc_vacuum_and_constraint():
+    """The coupled vacuum m = (0, 0, M0), psi = u1 has zero energy (the subtracted
+    constant), and one step keeps |m| = M0 with V tangent to m."""
+    from oracle.theories import ferromagnetic_superconductor as o_fs
+    th = ssa.load_theory("Ferromagnetic superconductor")
+    p = H.fs_params(th, 24, 24)
+    p_i, p_f = H.packed(th, p)
+    F = np.zeros((8, 24, 24))
+    V = np.zeros_like(F)
+    th.initial_config.initialize(Field=F, Velocity=V, grid=H.grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                 config={"mode": "ground"})
+    assert abs(ocore.theory_energy(o_fs, F, p_i, p_f)) < 1e-12 * 24 * 24
+    F = H.fs_field(th, p_i, p_f, seed=2, amp=0.05)
+    V = np.zeros_like(F)
+    ocore.relax(o_fs, F, V, p_i, p_f, 2)
+    M0 = float(p_f[17])
+    assert np.abs(np.sqrt((F[:3] ** 2).sum(0)) - M0).max() < 1e-13
+    assert np.abs((V[:3] * F[:3]).sum(0)).max() < 1e-12
+
+
+def test_ferromagnetic_sc_ansatz_charges():
+    """The skyrmion-vortex ansatz carries Q = -1 for the Bloch texture of winding 1
+    (m3 = M0 cos f, f(0) = pi), and no flux through the x or y faces (A3 = 0)."""
+    from oracle.theories import ferromagnetic_superconductor as o_fs
+    th = ssa.load_theory("Ferromagnetic superconductor")
+    p = H.fs_params(th, 161, 161, xsize=40.0)
+    p_i, p_f = H.packed(th, p)
+    F = H.fs_field(th, p_i, p_f, amp=0.0)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    Q = ocore.reference_sum(o_fs.skyrmion_density(F, D1, p_i, p_f).ravel())
+    assert abs(Q + 1.0) < 2e-3
+    for which in (0, 1):
+        assert ocore.reference_sum(o_fs.vortex_density(F, D1, p_i, p_f, which).ravel()) == 0.0
