@@ -24,6 +24,7 @@ const TheoryOps* ops_anisotropic_sc();
 const TheoryOps* ops_bec();
 const TheoryOps* ops_mcsh();
 const TheoryOps* ops_fs();
+const TheoryOps* ops_st();
 }  // namespace ss
 
 using namespace ss;
@@ -39,6 +40,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_BEC: return ops_bec();
     case SS_THEORY_MAXWELL_CHERN_SIMONS_HIGGS: return ops_mcsh();
     case SS_THEORY_FERROMAGNETIC_SC: return ops_fs();
+    case SS_THEORY_SPIN_TRIPLET: return ops_st();
     default: return nullptr;
   }
 }

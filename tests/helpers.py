@@ -11,6 +11,7 @@ from oracle.theories import chiral_magnet as o_cm
 from oracle.theories import ferromagnetic_superconductor as o_fs
 from oracle.theories import maxwell_chern_simons_higgs as o_mcsh
 from oracle.theories import ginzburg_landau as o_gl
+from oracle.theories import spin_triplet as o_st
 
 ORACLES = {
     "Chiral magnet": o_cm,
@@ -20,6 +21,7 @@ ORACLES = {
     "Bose-Einstein condensate": o_bec,
     "Anyon superconductor": o_mcsh,
     "Ferromagnetic superconductor": o_fs,
+    "Spin triplet superconducting ferromagnet": o_st,
 }
 
 
@@ -199,4 +201,22 @@ def fs_field(theory, p_i, p_f, seed=0, amp=1e-2, ansatz="bloch", vortex_number=1
     theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
                                      config={"mode": "initial", "ansatz": ansatz, "vortex_number": vortex_number,
                                              "noise": amp, "seed": seed})
+    return F
+
+
+def st_params(theory, n=48, m=None, xsize=16.0, **kw):
+    """Defaults give M0 = 1, u1 = u2 = sqrt(5/4)."""
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), q=0.9, gamma=1.1, unit_magnetization=True)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def st_field(theory, p_i, p_f, seed=0, amp=1e-2, ansatz="bloch"):
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "initial", "ansatz": ansatz, "noise": amp, "seed": seed})
     return F

@@ -341,3 +341,28 @@ def test_ferromagnetic_sc_variation_energy_charges_and_steps(ansatz):
     assert np.abs(Fg - Fo).max() <= 1e-9
     M0 = float(p_f[17])
     assert np.abs(np.sqrt((Fg[:3] ** 2).sum(0)) - M0).max() < 1e-12
+
+
+def test_spin_triplet_variation_energy_charges_and_steps():
+    name = "Spin triplet superconducting ferromagnet"
+    th = ssa.load_theory(name)
+    p = H.st_params(th, 54, 46, vortex2_number=2.0, hb2=0.7, hc=-0.4)
+    p_i, p_f = H.packed(th, p)
+    F = H.st_field(th, p_i, p_f, seed=6)
+    o = H.ORACLES[name]
+    sim = _sim(th, p, F)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F.copy(), D1, D2, p_i, p_f)
+    assert np.abs(sim.energy_gradient() - eg_ref).max() <= 1e-11 * np.abs(eg_ref).max()
+    assert H.rel(sim.compute_energy(), ocore.reference_sum(o.energy_density(F, D1, p_i, p_f))) < 1e-12
+    obs = sim.observables()
+    assert abs(obs["skyrmion_number"] - ocore.reference_sum(o.skyrmion_density(F, D1, p_i, p_f))) < 1e-11
+    for which in (2, 0, 1):
+        ref = ocore.reference_sum(o.vortex_density(F, D1, p_i, p_f, which))
+        assert abs(th.observables.compute_vortex_number(sim, which) - ref) < 1e-11
+    sim.close()
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory(name, th, p, F, np.zeros_like(F), 10, [4, 6])
+    print(f"[spin triplet] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={np.abs(Fg - Fo).max():.2e}")
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9

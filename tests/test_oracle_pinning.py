@@ -521,3 +521,52 @@ def test_ferromagnetic_sc_ansatz_charges():
     assert abs(Q + 1.0) < 2e-3
     for which in (0, 1):
         assert ocore.reference_sum(o_fs.vortex_density(F, D1, p_i, p_f, which).ravel()) == 0.0
+
+
+def test_spin_triplet_variation_is_functional_derivative():
+    """sum gv EG.w = dE_h/de at 4th order for w tangent to m (|m| = M0), every
+    field perturbed, both order parameters, the hb2 and hc couplings switched on."""
+    from oracle.theories import spin_triplet as o_st
+    th = ssa.load_theory("Spin triplet superconducting ferromagnet")
+    errs = []
+    for n in (61, 121):
+        p = H.st_params(th, n, n, xsize=8.0, hb2=0.7, hc=-0.4)
+        p_i, p_f = H.packed(th, p)
+        g = H.grid_of(p_i, p_f)
+        M0 = float(p_f[20])
+        F = np.zeros((10, n, n))
+        F[0] = 0.6 * np.sin(0.5 * g[0] + 0.2 * g[1])
+        F[1] = 0.5 * np.cos(0.3 * g[1] - 0.4 * g[0])
+        F[2] = 0.8 + 0.2 * np.sin(0.4 * g[0])
+        F[:3] *= M0 / np.sqrt((F[:3] ** 2).sum(0))
+        F[3] = 0.3 * np.sin(0.7 * g[0] + 0.2 * g[1])
+        F[4] = 0.2 * np.cos(0.5 * g[1] - 0.3 * g[0])
+        F[5] = 0.25 * np.sin(0.3 * g[0] * g[1] / 8.0)
+        F[6] = 1.1 + 0.3 * np.sin(0.5 * g[0])
+        F[7] = 0.4 * np.cos(0.4 * g[1])
+        F[8] = 0.9 + 0.2 * np.cos(0.3 * g[0] - 0.2 * g[1])
+        F[9] = -0.3 * np.sin(0.6 * g[1])
+        b = _bump(p_i, p_f, w=0.35)
+        w = np.stack([b * np.cos(0.6 * g[1]), 0.5 * b, b * (0.2 + 0.1 * g[0]), 0.7 * b, -0.4 * b, 0.9 * b, 0.5 * b,
+                      -0.8 * b, 0.6 * b * np.sin(g[0]), 0.3 * b])
+        w[:3] -= (w[:3] * F[:3]).sum(0) * F[:3] / M0 ** 2
+        errs.append(_dir_mismatch(o_st, F, w, p_i, p_f))
+    print("spin-triplet directional mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_spin_triplet_vacuum_and_ansatz_charge():
+    from oracle.theories import spin_triplet as o_st
+    th = ssa.load_theory("Spin triplet superconducting ferromagnet")
+    p = H.st_params(th, 24, 24)
+    p_i, p_f = H.packed(th, p)
+    F = np.zeros((10, 24, 24))
+    th.initial_config.initialize(Field=F, Velocity=np.zeros_like(F), grid=H.grid_of(p_i, p_f), p_i_h=p_i,
+                                 p_f_h=p_f, config={"mode": "ground"})
+    assert abs(ocore.theory_energy(o_st, F, p_i, p_f)) < 1e-12 * 24 * 24
+    p = H.st_params(th, 161, 161, xsize=40.0, ansatz="neel")
+    p_i, p_f = H.packed(th, p)
+    F = H.st_field(th, p_i, p_f, amp=0.0, ansatz="neel")
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    assert abs(ocore.reference_sum(o_st.skyrmion_density(F, D1, p_i, p_f).ravel()) + 1.0) < 2e-3
