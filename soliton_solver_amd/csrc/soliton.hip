@@ -25,6 +25,7 @@ const TheoryOps* ops_bec();
 const TheoryOps* ops_mcsh();
 const TheoryOps* ops_fs();
 const TheoryOps* ops_st();
+const TheoryOps* ops_lc();
 }  // namespace ss
 
 using namespace ss;
@@ -41,6 +42,7 @@ static const TheoryOps* theory_ops(int id) {
     case SS_THEORY_MAXWELL_CHERN_SIMONS_HIGGS: return ops_mcsh();
     case SS_THEORY_FERROMAGNETIC_SC: return ops_fs();
     case SS_THEORY_SPIN_TRIPLET: return ops_st();
+    case SS_THEORY_LIQUID_CRYSTAL: return ops_lc();
     default: return nullptr;
   }
 }

@@ -11,6 +11,7 @@ from oracle.theories import chiral_magnet as o_cm
 from oracle.theories import ferromagnetic_superconductor as o_fs
 from oracle.theories import maxwell_chern_simons_higgs as o_mcsh
 from oracle.theories import ginzburg_landau as o_gl
+from oracle.theories import liquid_crystal as o_lc
 from oracle.theories import spin_triplet as o_st
 
 ORACLES = {
@@ -22,6 +23,7 @@ ORACLES = {
     "Anyon superconductor": o_mcsh,
     "Ferromagnetic superconductor": o_fs,
     "Spin triplet superconducting ferromagnet": o_st,
+    "Liquid crystal": o_lc,
 }
 
 
@@ -219,4 +221,25 @@ def st_field(theory, p_i, p_f, seed=0, amp=1e-2, ansatz="bloch"):
     V = np.zeros_like(F)
     theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
                                      config={"mode": "initial", "ansatz": ansatz, "noise": amp, "seed": seed})
+    return F
+
+
+def lc_params(theory, n=48, m=None, xsize=12.0, **kw):
+    """Physical defaults of the reference (twist, coup_PotE ~ 4.06, coup_eps ~ 22.1, e3/e1 = 2)."""
+    m = m or n
+    h = xsize / (n - 1)
+    base = dict(xlen=n, ylen=m, xsize=xsize, ysize=h * (m - 1), unit_magnetization=True)
+    base.update(kw)
+    return theory.params.default_params(**base)
+
+
+def lc_field(theory, p_i, p_f, seed=0, amp=1e-2, ansatz="bloch"):
+    """Skyrmion ansatz plus noise; Phi set to a smooth bump so the depolarisation terms act."""
+    nf, X, Y = int(p_i[4]), int(p_i[0]), int(p_i[1])
+    F = np.zeros((nf, X, Y))
+    V = np.zeros_like(F)
+    theory.initial_config.initialize(Field=F, Velocity=V, grid=grid_of(p_i, p_f), p_i_h=p_i, p_f_h=p_f,
+                                     config={"mode": "initial", "ansatz": ansatz, "noise": amp, "seed": seed})
+    g = grid_of(p_i, p_f)
+    F[3] = 0.2 * np.exp(-((g[0] - g[0].mean()) ** 2 + (g[1] - g[1].mean()) ** 2) / 5.0)
     return F

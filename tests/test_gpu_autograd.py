@@ -80,33 +80,39 @@ def test_gradcheck_on_patch_of_512_anisotropic():
     sim.close()
 
 
-def test_exact_flow_reaches_stationary_point_of_discrete_energy():
+def _as_relax_case():
+    """128x128 s+id vortex start in a 25x25 box, betas from seed 1, time step 0.3 h:
+    the anisotropic kinetic term (coefficient 2 gamma) puts the stiffest mode of
+    the exact-adjoint flow near the RK4 limit at the default 0.5 h."""
     th = ssa.load_theory("Anisotropic superconductor")
-    p = H.as_params(th, 64, 64, xsize=12.0, seed=1)
+    n, xs = 128, 25.0
+    p = H.as_params(th, n, n, xsize=xs, seed=1, time_step=0.3 * xs / (n - 1))
     p_i, p_f = H.packed(th, p)
-    F = H.as_field(th, p_i, p_f, seed=1, amp=1e-2)
+    return th, p, p_i, p_f, H.as_field(th, p_i, p_f, seed=1, amp=1e-2)
+
+
+def test_exact_flow_reaches_stationary_point_of_discrete_energy():
+    th, p, p_i, p_f, F = _as_relax_case()
     sim = ssa.Simulation(p, th, variation="exact_adjoint")
     sim.initialize({"mode": "array", "field": F})
     E0 = sim.compute_energy()
-    E, ea, n = sim.minimize(tol=1e-9, max_steps=40000, check_every=500)
+    E, ea, n = sim.minimize(tol=1e-7, max_steps=50000, check_every=1000)
     G = sag.exact_gradient(sim, torch.tensor(sim.get_field(), device="cuda")).cpu().numpy()
     print(f"[exact flow] E0={E0:.10f} E={E:.12f} steps={n} max|EG|={ea:.2e} max|dE/dphi|/gv={np.abs(G).max() / p_f[4]:.2e}")
     assert E < E0
-    assert ea < 1e-9
-    assert np.abs(G).max() / p_f[4] < 1e-8
+    assert ea < 1e-7
+    assert np.abs(G).max() / p_f[4] < 1e-7
     sim.close()
 
 
 @pytest.fixture(scope="module")
 def relaxed_anisotropic():
-    th = ssa.load_theory("Anisotropic superconductor")
-    p = H.as_params(th, 64, 64, xsize=12.0, seed=5)
-    p_i, p_f = H.packed(th, p)
-    F = H.as_field(th, p_i, p_f, seed=5, amp=1e-2)
+    th, p, p_i, p_f, F = _as_relax_case()
     sim = ssa.Simulation(p, th, variation="exact_adjoint")
     sim.initialize({"mode": "array", "field": F})
-    E, ea, n = sim.minimize(tol=1e-10, max_steps=60000, check_every=500)
-    print(f"[relaxed anisotropic 64x64] E*={E:.12f} max|EG|={ea:.2e} steps={n}", flush=True)
+    E, ea, n = sim.minimize(tol=1e-9, max_steps=80000, check_every=1000)
+    print(f"[relaxed anisotropic 128x128] E*={E:.12f} max|EG|={ea:.2e} steps={n}", flush=True)
+    assert ea < 1e-9
     Fstar = sim.get_field()
     grads = sag.energy_parameter_gradient(sim, torch.tensor(Fstar, device="cuda"),
                                           ["beta1", "beta2", "beta3", "beta4"])
@@ -126,7 +132,8 @@ def test_parameter_sensitivity_matches_rerelaxation(relaxed_anisotropic, name):
         q = p.with_(**{name: v + s * d})
         s2 = ssa.Simulation(q, th, variation="exact_adjoint")
         s2.initialize({"mode": "array", "field": Fstar})
-        E, _, _ = s2.minimize(tol=1e-10, max_steps=60000, check_every=500)
+        E, ea2, _ = s2.minimize(tol=1e-9, max_steps=80000, check_every=1000)
+        assert ea2 < 1e-9
         Es.append(E)
         s2.close()
     fd = (Es[0] - Es[1]) / (2 * d)

@@ -366,3 +366,28 @@ def test_spin_triplet_variation_energy_charges_and_steps():
     print(f"[spin triplet] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={np.abs(Fg - Fo).max():.2e}")
     assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
     assert np.abs(Fg - Fo).max() <= 1e-9
+
+
+@pytest.mark.parametrize("deformation,depol", [("twist", False), ("twist", True), ("splay-bend", True)])
+def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
+    name = "Liquid crystal"
+    th = ssa.load_theory(name)
+    p = H.lc_params(th, 62, 50, deformation=deformation, depol=depol)
+    p_i, p_f = H.packed(th, p)
+    F = H.lc_field(th, p_i, p_f, seed=8, ansatz="neel" if deformation == "splay-bend" else "bloch")
+    o = H.ORACLES[name]
+    sim = _sim(th, p, F)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    eg_ref = o.variation(F.copy(), D1, D2, p_i, p_f)
+    assert np.abs(sim.energy_gradient() - eg_ref).max() <= 1e-11 * np.abs(eg_ref).max()
+    assert H.rel(sim.compute_energy(), ocore.reference_sum(o.energy_density(F, D1, p_i, p_f))) < 1e-12
+    obs = sim.observables()
+    assert abs(obs["skyrmion_number"] - ocore.reference_sum(o.skyrmion_density(F, D1, p_i, p_f))) < 1e-11
+    el = o.electric_charge_density(F, D1, D2, p_i, p_f)
+    assert abs(obs["electric_charge"] - ocore.reference_sum(el)) < 1e-11 * max(1.0, np.abs(el).sum())
+    sim.close()
+    Es, errs, Fo, Vo, gE, gerr, Fg, Vg = _trajectory(name, th, p, F, np.zeros_like(F), 12, [5, 7])
+    print(f"[lc {deformation} depol={depol}] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={np.abs(Fg - Fo).max():.2e}")
+    assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
+    assert np.abs(Fg - Fo).max() <= 1e-9

@@ -570,3 +570,87 @@ def test_spin_triplet_vacuum_and_ansatz_charge():
     F = H.st_field(th, p_i, p_f, amp=0.0, ansatz="neel")
     D1 = ocore.first_derivatives(F, p_i, p_f)
     assert abs(ocore.reference_sum(o_st.skyrmion_density(F, D1, p_i, p_f).ravel()) + 1.0) < 2e-3
+
+
+def _lc_smooth(th, n, **kw):
+    p = H.lc_params(th, n, n, xsize=8.0, **kw)
+    p_i, p_f = H.packed(th, p)
+    g = H.grid_of(p_i, p_f)
+    F = np.zeros((4, n, n))
+    F[0] = 0.6 * np.sin(0.5 * g[0] + 0.2 * g[1])
+    F[1] = 0.5 * np.cos(0.3 * g[1] - 0.4 * g[0])
+    F[2] = 0.8 + 0.2 * np.sin(0.4 * g[0])
+    F[:3] /= np.sqrt((F[:3] ** 2).sum(0))
+    F[3] = 0.4 * np.sin(0.3 * g[0] + 0.5 * g[1])
+    b = _bump(p_i, p_f, w=0.35)
+    w = np.zeros_like(F)
+    w[:3] = np.stack([b * np.cos(0.6 * g[1]), 0.5 * b, b * (0.2 + 0.1 * g[0])])
+    w[:3] -= (w[:3] * F[:3]).sum(0) * F[:3]
+    return p_i, p_f, F, w
+
+
+@pytest.mark.parametrize("deformation", ["twist", "splay-bend"])
+def test_liquid_crystal_variation_is_functional_derivative_without_depol(deformation):
+    from oracle.theories import liquid_crystal as o_lc
+    th = ssa.load_theory("Liquid crystal")
+    errs = []
+    for n in (61, 121):
+        p_i, p_f, F, w = _lc_smooth(th, n, deformation=deformation)
+        errs.append(_dir_mismatch(o_lc, F, w, p_i, p_f))
+    print(deformation, "liquid crystal directional mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_liquid_crystal_depolarisation_director_terms_are_twice_their_derivative():
+    """Reference quirk kept: with depol on, the director variation's
+    depolarisation part (kernels.py:492-500) is 2 d/dn of the depolarisation
+    energy (kernels.py:203-209), so EG.w - dE = dE_depol at 4th order."""
+    from oracle.theories import liquid_crystal as o_lc
+    th = ssa.load_theory("Liquid crystal")
+    errs = []
+    eps = 1e-5
+    for n in (61, 121):
+        d = {}
+        for depol in (False, True):
+            p_i, p_f, F, w = _lc_smooth(th, n, depol=depol)
+            d[depol] = (ocore.theory_energy(o_lc, F + eps * w, p_i, p_f) - ocore.theory_energy(o_lc, F - eps * w, p_i, p_f)) / (2 * eps)
+        D1 = ocore.first_derivatives(F, p_i, p_f)
+        D2 = ocore.second_derivatives(F, p_i, p_f)
+        G = (o_lc.variation(F.copy(), D1, D2, p_i, p_f) * w).sum() * p_f[4]
+        dE_depol = d[True] - d[False]
+        assert abs(dE_depol) > 0.1 * abs(d[True])
+        errs.append(abs(G - d[True] - dE_depol) / abs(d[True]))
+    print("liquid crystal depol quirk mismatch (h, h/2):", errs)
+    assert errs[1] < 1e-3
+    assert errs[0] / errs[1] > 8.0
+
+
+def test_liquid_crystal_phi_equation_is_poisson_with_bound_charge():
+    """EG_Phi = -lap Phi + rho/coup_eps with rho = div P,
+    P_i = n_i div n - (e3/e1)(n.grad) n_i (in-plane i): checked against the
+    stencil divergence of P (4th order) and, exactly, against the electric
+    charge density of the observables (rho = -density/gv)."""
+    from oracle.theories import liquid_crystal as o_lc
+    th = ssa.load_theory("Liquid crystal")
+    errs = []
+    for n in (61, 121):
+        p_i, p_f, F, _ = _lc_smooth(th, n, depol=True)
+        D1 = ocore.first_derivatives(F, p_i, p_f)
+        D2 = ocore.second_derivatives(F, p_i, p_f)
+        EG = o_lc.variation(F.copy(), D1, D2, p_i, p_f)
+        gv, ceps, r = float(p_f[4]), float(p_f[8]), float(p_f[10]) / float(p_f[9])
+        rho = -o_lc.electric_charge_density(F, D1, D2, p_i, p_f) / gv
+        inner = ~ocore.band_mask(p_i)
+        lhs = EG[3] + D2[3, 0] + D2[3, 1]
+        assert np.abs((lhs - rho / ceps)[inner]).max() < 1e-12 * max(1.0, np.abs(rho).max())
+        divn = D1[0, 0] + D1[1, 1]
+        P = np.stack([F[i] * divn - r * (F[0] * D1[i, 0] + F[1] * D1[i, 1]) for i in (0, 1)])
+        DP = ocore.first_derivatives(np.concatenate([P, P]), p_i, p_f)
+        div_P = DP[0, 0] + DP[1, 1]
+        core = np.zeros_like(inner)
+        core[6:-6, 6:-6] = True
+        errs.append(np.abs((div_P - rho)[core]).max() / np.abs(rho[core]).max())
+    print("liquid crystal div P vs rho (h, h/2):", errs)
+    assert errs[1] < 1e-4
+    assert errs[0] / errs[1] > 8.0
