@@ -46,13 +46,18 @@ def chiral_workload(n: int, seed: int = 0, nsk: int = 8):
     return th, p, {"mode": "skyrmions", "ansatz": "anti", "skyrmions": sk, "noise": 1e-3, "seed": seed + 1}
 
 
-def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None):
+def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotune=True, tiles=None):
     import soliton_solver_amd as ssa
     sim = ssa.Simulation(p, th, precision=precision)
     if F0 is None:
         sim.initialize(cfg)
     else:
         sim.initialize({"mode": "array", "field": F0})
+    if tiles is not None:
+        sim.set_tile_variants(tiles)
+        sim.tiles = sim.tile_variants()
+    else:
+        sim.tiles = sim.autotune_tiles(reps=5) if autotune else sim.tile_variants()
     if graph_steps:
         sim.use_graphs(graph_steps)
     E = sim.compute_energy()
@@ -78,6 +83,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1024, help="side of the CPU-baseline sample lattice")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape of every pass")
+    ap.add_argument("--tiles", default="", help="tile variants of passes 1-4 for the FP32 form, e.g. 1,1,1,2 "
+                                                "(no autotuning)")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprof byte counters)")
@@ -111,7 +119,11 @@ def main():
     n = args.n
     th, p, cfg = chiral_workload(n)
     print(f"[bench] building {n}x{n} chiral workload", file=sys.stderr, flush=True)
-    sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, "fp32", args.steps, args.warmup, args.graph_steps)
+    sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, "fp32", args.steps, args.warmup, args.graph_steps,
+                                             autotune=not args.no_autotune,
+                                             tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None)
+    tiles32 = sim32.tiles
+    print(f"[bench] fp32 tile rows per pass {tiles32['rows']}", file=sys.stderr, flush=True)
     F0 = None
     N = n * n
     nf = 4
@@ -131,13 +143,16 @@ def main():
 
     result64 = None
     if not args.profile_only:
-        sim64, E64, err64, ea64, t64 = time_form(th, p, cfg, "fp64", args.steps, args.warmup, args.graph_steps)
+        sim64, E64, err64, ea64, t64 = time_form(th, p, cfg, "fp64", args.steps, args.warmup, args.graph_steps,
+                                                 autotune=not args.no_autotune)
+        tiles64 = sim64.tiles
         ups64 = N * args.steps / t64
         relE = abs(E32 - E64) / abs(E64)
         sim64.close()
         result64 = {"updates_per_s": ups64, "ms_per_step": t64 / args.steps * 1e3,
                     "hbm_frac_alg": ups64 * 16 * nf * 8 / (HBM_PEAK_GBS * 1e9),
-                    "energy": E64, "fp32_vs_fp64_relE_after_steps": relE}
+                    "energy": E64, "fp32_vs_fp64_relE_after_steps": relE,
+                    "tile_rows_per_pass": tiles64["rows"]}
         print(f"[bench] fp64 form: {t64 / args.steps * 1e3:.3f} ms/step, {ups64:.3e} updates/s, "
               f"relE(fp32 vs fp64)={relE:.2e}", file=sys.stderr, flush=True)
 
@@ -185,14 +200,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "tile_variants": tiles32["variants"],
         "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
                                "FP32 fields+stencils, FP64 energy sums and arrest test",
                    "lattice": [n, n], "fields": nf, "graph_steps": args.graph_steps,
+                   "tile_rows_per_pass": tiles32["rows"], "tile_shape": "64 y x rows x",
                    "parallelism": "single GPU"},
         "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_pass<ChiralMagnet, float, 4>",
+                     "kernel": f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>",
                      "alg_bytes_per_launch": alg_pass4, "avg_launch_ms": passes[4],
                      "step_kernels_gbs": step_gbs, "pass_ms": [float(x) for x in passes]},
         "fp64_form": result64,

@@ -273,6 +273,30 @@ class Simulation:
         self._check(_lib.lib().ss_time_passes(self._ctx, float(prev_energy), int(nsteps), out))
         return out
 
+    def autotune_tiles(self, reps: int = 10) -> dict:
+        """Time every fused pass in each tile-shape variant on the live buffers and keep
+        the fastest per pass (the relaxation state is unchanged).  Returns the chosen
+        variants, the rows per tile they give and the ms table [pass][variant]."""
+        chosen = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(12)
+        self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
+        info = self.tile_variants()
+        info["ms"] = ms.reshape(4, 3)
+        return info
+
+    def set_tile_variants(self, variants) -> None:
+        """Tile-shape variant (0, 1 or 2) of passes 1..4."""
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        if v.shape != (4,):
+            raise ValueError("tile variants: one entry per fused pass (4)")
+        self._check(_lib.lib().ss_set_tile_variants(self._ctx, v))
+
+    def tile_variants(self) -> dict:
+        v = np.zeros(4, dtype=np.int32)
+        rows = np.zeros(4, dtype=np.int32)
+        self._check(_lib.lib().ss_get_tile_variants(self._ctx, v, rows))
+        return {"variants": v.tolist(), "rows": rows.tolist()}
+
     def synchronize(self) -> None:
         self._check(_lib.lib().ss_synchronize(self._ctx))
 

@@ -49,9 +49,9 @@ struct Launch {
   int ntiles, ntiles_y, per_xcd, grid;
 };
 
-template <typename R, int NF>
+template <typename R, int NF, int TXR = 0>
 inline Launch make_launch(const Geom& g) {
-  using S = TileShape<R, NF>;
+  using S = TileShape<R, NF, TXR>;
   Launch L;
   const int ntx = (g.X + S::TX - 1) / S::TX;
   L.ntiles_y = (g.Y + TY - 1) / TY;
@@ -65,11 +65,11 @@ inline Launch make_launch(const Geom& g) {
 // outputs for the sites of the tile, and the block's partial (energy sum for
 // pass 1, signed max / max |EG| for pass 4) written to part0[t] / part1[t].
 // Shared by the single-lattice kernel and the batched kernel.
-template <class TH, typename R, int PASS>
-__device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, const PassBufs& b,
+template <class TH, typename R, int PASS, int TXR = 0>
+__device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* red, const PassBufs& b,
                                            const typename TH::TP& tp, const Geom& g, int x0, int y0, bool kill,
                                            int t) {
-  using S = TileShape<R, TH::NF>;
+  using S = TileShape<R, TH::NF, TXR>;
   constexpr int NF = TH::NF;
   const R* __restrict__ F = (const R*)b.F;
   const R* __restrict__ V = (const R*)b.V;
@@ -250,17 +250,17 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF>& tile, double* red, c
   }
 }
 
-template <class TH, typename R, int PASS>
+template <class TH, typename R, int PASS, int TXR = 0>
 __global__ void __launch_bounds__(kThreads)
 stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
-  using S = TileShape<R, TH::NF>;
-  __shared__ Tile<R, TH::NF> tile;
+  using S = TileShape<R, TH::NF, TXR>;
+  __shared__ Tile<R, TH::NF, TXR> tile;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
   const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
   const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
-  stage_tile<TH, R, PASS>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
+  stage_tile<TH, R, PASS, TXR>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
 }
 
 // Energy (kind < 0) or charge density ``kind`` of F, times gv: block partial
@@ -637,27 +637,45 @@ template <class TH>
 struct TheoryLaunchers {
   using TP = typename TH::TP;
 
-  static int ntiles(int prec, const Geom& g) {
-    return prec ? make_launch<float, TH::NF>(g).ntiles : make_launch<double, TH::NF>(g).ntiles;
+  // Tiles of one pass launch for tile-shape variant ``v`` (0 default rows, 1 and 2 halved).
+  template <typename R>
+  static int ntiles_r(int v, const Geom& g) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    return v == 2 ? make_launch<R, TH::NF, r2>(g).ntiles
+         : v == 1 ? make_launch<R, TH::NF, r1>(g).ntiles
+                  : make_launch<R, TH::NF, r0>(g).ntiles;
   }
 
-  template <typename R>
-  static hipError_t pass_r(int pass, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
-    const Launch L = make_launch<R, TH::NF>(g);
+  static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
+
+  template <typename R, int TXR>
+  static hipError_t pass_rv(int pass, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
+    const Launch L = make_launch<R, TH::NF, TXR>(g);
     dim3 grid(L.grid), block(kThreads);
     switch (pass) {
-      case 0: hipLaunchKernelGGL((stage_pass<TH, R, 0>), grid, block, 0, st, b, tp, g, L); break;
-      case 1: hipLaunchKernelGGL((stage_pass<TH, R, 1>), grid, block, 0, st, b, tp, g, L); break;
-      case 2: hipLaunchKernelGGL((stage_pass<TH, R, 2>), grid, block, 0, st, b, tp, g, L); break;
-      case 3: hipLaunchKernelGGL((stage_pass<TH, R, 3>), grid, block, 0, st, b, tp, g, L); break;
-      case 4: hipLaunchKernelGGL((stage_pass<TH, R, 4>), grid, block, 0, st, b, tp, g, L); break;
+      case 0: hipLaunchKernelGGL((stage_pass<TH, R, 0, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 1: hipLaunchKernelGGL((stage_pass<TH, R, 1, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 2: hipLaunchKernelGGL((stage_pass<TH, R, 2, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 3: hipLaunchKernelGGL((stage_pass<TH, R, 3, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 4: hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR>), grid, block, 0, st, b, tp, g, L); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
   }
 
-  static hipError_t pass(int prec, int pass, const PassBufs& b, const void* tp, const Geom& g, hipStream_t st) {
-    return prec ? pass_r<float>(pass, b, *(const TP*)tp, g, st) : pass_r<double>(pass, b, *(const TP*)tp, g, st);
+  template <typename R>
+  static hipError_t pass_r(int pass, int v, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
+    if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
+    return pass_rv<R, r0>(pass, b, tp, g, st);
+  }
+
+  // ``v``: tile-shape variant; passes 0 and the energy/charge kernels use variant 0.
+  static hipError_t pass(int prec, int pass, int v, const PassBufs& b, const void* tp, const Geom& g,
+                         hipStream_t st) {
+    return prec ? pass_r<float>(pass, v, b, *(const TP*)tp, g, st)
+                : pass_r<double>(pass, v, b, *(const TP*)tp, g, st);
   }
 
   template <typename R>
@@ -767,8 +785,8 @@ struct TheoryOps {
   int nf;
   int ncharges;
   int (*make_tp)(const int32_t*, int, const double*, int, void*, char*, int);
-  int (*ntiles)(int, const Geom&);
-  hipError_t (*pass)(int, int, const PassBufs&, const void*, const Geom&, hipStream_t);
+  int (*ntiles)(int, int, const Geom&);
+  hipError_t (*pass)(int, int, int, const PassBufs&, const void*, const Geom&, hipStream_t);
   hipError_t (*density)(int, int, const void*, double*, double*, const void*, const Geom&, hipStream_t);
   hipError_t (*adjoint)(int, const void*, double*, double*, double*, const void*, void*, int, const void*,
                         const Geom&, hipStream_t);

@@ -181,7 +181,9 @@ struct ss_ctx {
   void* V[2] = {nullptr, nullptr};
   void* l[3] = {nullptr, nullptr, nullptr};
   int cur = 0;
-  int ntiles = 0;
+  int ntiles = 0;          // capacity of each partials row (the most tiles over the shape variants)
+  int nt0 = 0;             // tiles of the default shape (energy, charge and pass-0 kernels)
+  int tv[5] = {0, 0, 0, 0, 0};  // tile-shape variant of each fused pass (index = pass)
   double* part = nullptr;  // 3 x ntiles
   DevState* st = nullptr;
   double* hist = nullptr;  // 2 x hist_cap
@@ -298,7 +300,12 @@ static int alloc_fields(ss_ctx* c) {
 }
 
 static int alloc_partials(ss_ctx* c) {
-  const int nt = c->ops->ntiles(c->prec, c->g);
+  int nt = 0;
+  for (int v = 0; v < 3; ++v) {
+    const int n = c->ops->ntiles(c->prec, v, c->g);
+    nt = n > nt ? n : nt;
+  }
+  c->nt0 = c->ops->ntiles(c->prec, 0, c->g);
   if (nt > c->ntiles || !c->part) {
     if (c->part) hipFree(c->part);
     c->part = nullptr;
@@ -352,25 +359,25 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = c->l[0];
   b.part0 = p0;
-  HIPCHK(c, c->ops->pass(c->prec, 1, b, c->tp, c->g, c->stream));
+  HIPCHK(c, c->ops->pass(c->prec, 1, c->tv[1], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, c->ntiles, c->st, histE,
-                     c->hist_cap, c->g.killkinen);
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0,
+                     c->ops->ntiles(c->prec, c->tv[1], c->g), c->st, histE, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = c->l[1];
-  HIPCHK(c, c->ops->pass(c->prec, 2, b, c->tp, c->g, c->stream));
+  HIPCHK(c, c->ops->pass(c->prec, 2, c->tv[2], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = c->l[2];
-  HIPCHK(c, c->ops->pass(c->prec, 3, b, c->tp, c->g, c->stream));
+  HIPCHK(c, c->ops->pass(c->prec, 3, c->tv[3], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = nullptr;
   b.part0 = p0;
   b.part1 = p1;
-  HIPCHK(c, c->ops->pass(c->prec, 4, b, c->tp, c->g, c->stream));
+  HIPCHK(c, c->ops->pass(c->prec, 4, c->tv[4], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1, c->ntiles,
-                     c->st, histErr, c->hist_cap);
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
+                     c->ops->ntiles(c->prec, c->tv[4], c->g), c->st, histErr, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   return 0;
@@ -404,7 +411,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   if (ensure_exact(c)) return -1;
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
                      c->hist, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
@@ -449,7 +456,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   if (ensure_exact(c)) return -1;
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
                      c->hist, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
@@ -468,7 +475,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
     v.F = c->temp;
     v.lout = c->G;
     v.st = c->st;
-    HIPCHK(c, c->ops->pass(c->prec, 0, v, c->tp, c->g, c->stream));
+    HIPCHK(c, c->ops->pass(c->prec, 0, 0, v, c->tp, c->g, c->stream));
     if (st < 3) {
       hipLaunchKernelGGL(k_scale_into, grid, dim3(256), 0, c->stream, (double*)c->l[st], (const double*)c->G, n,
                          -c->g.dt);
@@ -489,7 +496,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
 // kill applied to V in memory.
 static int launch_finish(ss_ctx* c) {
   HIPCHK(c, c->ops->density(c->prec, -1, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->st,
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
                      c->hist, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   const long long n = (long long)c->g.nf * c->g.N;
@@ -775,6 +782,79 @@ int ss_time_passes(ss_ctx* c, double prev_energy, int nsteps, double* avg_ms) {
   return rc;
 }
 
+int ss_set_tile_variants(ss_ctx* c, const int* variants) {
+  if (!c) return fail(nullptr, "null context");
+  if (!variants) return fail(c, "null variant array");
+  for (int p = 0; p < 4; ++p)
+    if (variants[p] < 0 || variants[p] > 2) return fail(c, "tile variant of pass %d is %d (valid: 0, 1, 2)", p + 1, variants[p]);
+  for (int p = 0; p < 4; ++p) c->tv[p + 1] = variants[p];
+  free_graphs(c);  // captured graphs hold the old launch shapes
+  return 0;
+}
+
+int ss_get_tile_variants(ss_ctx* c, int* variants, int* rows) {
+  if (!c) return fail(nullptr, "null context");
+  for (int p = 0; p < 4; ++p) {
+    if (variants) variants[p] = c->tv[p + 1];
+    if (rows) {
+      const int r = c->ops->tile_rows(c->prec) >> c->tv[p + 1];  // as variant_rows()
+      rows[p] = r < 4 ? 4 : r;
+    }
+  }
+  return 0;
+}
+
+// Times each fused pass in every tile-shape variant on the live buffers and
+// keeps the fastest per pass.  Only scratch outputs are written (l_k, the
+// partials and the next-parity F, V), so the state of the relaxation is
+// unchanged.
+int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
+  if (!c) return fail(nullptr, "null context");
+  if (reps < 1) return fail(c, "reps must be >= 1");
+  if (c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize)
+    return fail(c, "tile autotuning applies to the fused reference flow only");
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  PassBufs b = bufs_for(c, c->cur);
+  b.part0 = c->part;
+  b.part1 = c->part + c->ntiles;
+  int pick[5] = {0, 0, 0, 0, 0};
+  int rc = 0;
+  for (int p = 1; p <= 4 && rc == 0; ++p) {
+    b.lout = p <= 3 ? c->l[p - 1] : nullptr;
+    float best = 0.f;
+    for (int v = 0; v < 3 && rc == 0; ++v) {
+      hipError_t e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);  // warm-up
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        rc = hipfail(c, "tile autotuning of pass %d variant %d failed: %s", p, v, hipGetErrorString(e));
+        break;
+      }
+      t /= reps;
+      if (ms) ms[(p - 1) * 3 + v] = t;
+      if (v == 0 || t < best) {
+        best = t;
+        pick[p] = v;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  for (int p = 1; p <= 4; ++p) {
+    c->tv[p] = pick[p];
+    if (chosen) chosen[p - 1] = pick[p];
+  }
+  free_graphs(c);
+  return 0;
+}
+
 int ss_step_history(ss_ctx* c, double* energies, double* errs, int n) {
   if (!c) return fail(nullptr, "null context");
   if (n < 0 || n > c->last_nsteps) return fail(c, "history has %d entries (asked %d)", c->last_nsteps, n);
@@ -790,7 +870,7 @@ static int reduce_kind(ss_ctx* c, int kind, double* out) {
   if (!out) return fail(c, "null output");
   if (kind >= c->ops->ncharges) return fail(c, "theory %s has %d charges (asked %d)", c->ops->name, c->ops->ncharges, kind);
   HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->ntiles, c->scal);
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->scal);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -828,7 +908,7 @@ static int variation_into(ss_ctx* c, const void* field, void* out, int mode, hip
   b.F = field;
   b.lout = out;
   b.st = c->st;
-  HIPCHK(c, c->ops->pass(c->prec, 0, b, c->tp, c->g, st));
+  HIPCHK(c, c->ops->pass(c->prec, 0, 0, b, c->tp, c->g, st));
   return 0;
 }
 
@@ -862,7 +942,7 @@ int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* st
   if (!field_dev || !energy_dev) return fail(c, "null device pointer");
   hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
   HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, c->part, c->tp, c->g, st));
-  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->ntiles, energy_dev);
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->nt0, energy_dev);
   HIPCHK(c, hipGetLastError());
   return 0;
 }

@@ -14,24 +14,39 @@
 
 namespace ss {
 
-// Rows of x per tile, chosen so one tile (with halo) of every component fits
-// in about 48 KiB of LDS: several workgroups per CU stay resident.
+// Rows of x per tile.  TXR = 0 picks the largest of 32/16/8/4 rows whose
+// tile (with halo) of every component fits in about 48 KiB of LDS; a nonzero
+// TXR forces that many rows (a multiple of 4, for the tile-shape variants the
+// autotuner times: fewer rows, less LDS, more resident workgroups per CU).
 template <typename R, int NF>
+struct DefaultRows {
+  static constexpr int kBytesRow = NF * (TY + 4) * (int)sizeof(R);
+  static constexpr int value = ((32 + 4) * kBytesRow <= 49152) ? 32
+                             : ((16 + 4) * kBytesRow <= 49152) ? 16
+                             : ((8 + 4) * kBytesRow <= 49152) ? 8
+                             : 4;
+};
+
+// Rows of tile variant v (0, 1, 2): the default, then halved twice, at least 4.
+template <typename R, int NF>
+constexpr int variant_rows(int v) {
+  return (DefaultRows<R, NF>::value >> v) < 4 ? 4 : (DefaultRows<R, NF>::value >> v);
+}
+
+template <typename R, int NF, int TXR = 0>
 struct TileShape {
   static constexpr int kBytesRow = NF * (TY + 4) * (int)sizeof(R);
-  static constexpr int TX = ((32 + 4) * kBytesRow <= 49152) ? 32
-                          : ((16 + 4) * kBytesRow <= 49152) ? 16
-                          : ((8 + 4) * kBytesRow <= 49152) ? 8
-                          : 4;
-  static_assert((4 + 4) * kBytesRow <= 65536, "tile does not fit in 64 KiB of LDS");
+  static constexpr int TX = TXR ? TXR : DefaultRows<R, NF>::value;
+  static_assert(TX % kWaves == 0, "tile rows must split evenly over the waves");
+  static_assert((TX + 4) * kBytesRow <= 65536, "tile does not fit in 64 KiB of LDS");
   static constexpr int RX = TX + 4;  // rows with halo
   static constexpr int RY = TY + 4;  // columns with halo
   static constexpr int kRowsPerWave = TX / kWaves;
 };
 
-template <typename R, int NF>
+template <typename R, int NF, int TXR = 0>
 struct Tile {
-  using S = TileShape<R, NF>;
+  using S = TileShape<R, NF, TXR>;
   R v[NF][S::RX][S::RY];
 };
 
@@ -39,9 +54,9 @@ struct Tile {
 // ``val(a, off)`` returns the stage-field value at flat offset ``off`` of
 // component ``a``; sites outside the lattice are stored as 0 and are never
 // used by an unmasked stencil.
-template <typename R, int NF, typename ValFn>
-__device__ __forceinline__ void load_tile(Tile<R, NF>& t, const Geom& g, int x0, int y0, ValFn val) {
-  using S = TileShape<R, NF>;
+template <typename R, int NF, int TXR, typename ValFn>
+__device__ __forceinline__ void load_tile(Tile<R, NF, TXR>& t, const Geom& g, int x0, int y0, ValFn val) {
+  using S = TileShape<R, NF, TXR>;
   constexpr int kElems = S::RX * S::RY;
   for (int idx = threadIdx.x; idx < kElems; idx += kThreads) {
     const int r = idx / S::RY;
@@ -59,8 +74,8 @@ __device__ __forceinline__ void load_tile(Tile<R, NF>& t, const Geom& g, int x0,
 
 // Derivatives of every component at interior tile position (lx, ly).
 // T is the arithmetic type (R for the variation, double for energies).
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void tile_derivs(const Tile<R, NF>& t, int lx, int ly, const Geom& g,
+template <typename T, typename R, int NF, int TXR>
+__device__ __forceinline__ void tile_derivs(const Tile<R, NF, TXR>& t, int lx, int ly, const Geom& g,
                                             const SiteInfo& s, T f[NF], T d1[NF][2], T d2[NF][3]) {
   const int cx = lx + 2, cy = ly + 2;
   const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
@@ -97,8 +112,8 @@ __device__ __forceinline__ void tile_derivs(const Tile<R, NF>& t, int lx, int ly
 // Derivatives of one component from its x-window (x-2..x+2 at the site's
 // column, held in registers by a thread marching along x) plus the y and
 // diagonal neighbours read from LDS at tile row cx, column cy.
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void window_derivs(const Tile<R, NF>& t, int a, const R win[5], int cx, int cy,
+template <typename T, typename R, int NF, int TXR>
+__device__ __forceinline__ void window_derivs(const Tile<R, NF, TXR>& t, int a, const R win[5], int cx, int cy,
                                               const Geom& g, const SiteInfo& s, T& f, T d1[2], T d2[3]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
   const T c = T(win[2]);
@@ -123,8 +138,8 @@ __device__ __forceinline__ void window_derivs(const Tile<R, NF>& t, int a, const
 }
 
 // First derivatives of one component from the same window, in type T.
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void window_d1(const Tile<R, NF>& t, int a, const R win[5], int cx, int cy,
+template <typename T, typename R, int NF, int TXR>
+__device__ __forceinline__ void window_d1(const Tile<R, NF, TXR>& t, int a, const R win[5], int cx, int cy,
                                           const Geom& g, const SiteInfo& s, T& f, T d1[2]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23);
   const T xm2 = T(win[0]), xm1 = T(win[1]), xp1 = T(win[3]), xp2 = T(win[4]);
@@ -136,8 +151,8 @@ __device__ __forceinline__ void window_d1(const Tile<R, NF>& t, int a, const R w
 }
 
 // First derivatives only (energies and charges).
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void tile_d1(const Tile<R, NF>& t, int lx, int ly, const Geom& g,
+template <typename T, typename R, int NF, int TXR>
+__device__ __forceinline__ void tile_d1(const Tile<R, NF, TXR>& t, int lx, int ly, const Geom& g,
                                         const SiteInfo& s, T f[NF], T d1[NF][2]) {
   const int cx = lx + 2, cy = ly + 2;
   const T c12 = T(SS_C12), c23 = T(SS_C23);

