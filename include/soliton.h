@@ -130,14 +130,17 @@ int ss_synchronize(ss_ctx* ctx);
  * pass 3, pass 4, error finalize.  State advances like ss_step. */
 int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
 
-/* Tile shapes of the four fused passes (no counterpart in the reference,
- * whose kernels use fixed 8x8/16x16 thread blocks).  Variant 0 stages the
- * largest tile that fits ~48 KiB of LDS, variants 1 and 2 halve its rows
- * (less LDS, more resident workgroups per CU).  ss_autotune_tiles times every
- * pass in every variant ``reps`` times on the live buffers, keeps the fastest
- * per pass, and reports chosen[0..3] and ms[3*(pass-1)+variant]; it writes
- * only scratch buffers, so the relaxation state is unchanged.  Results of a
- * step do not depend on the variants beyond FP64 summation order. */
+/* Launch shapes of the four fused passes (no counterpart in the reference,
+ * whose kernels use fixed 8x8/16x16 thread blocks).  Variants 0-2 stage one
+ * LDS tile per workgroup: the largest that fits ~48 KiB, then its rows halved
+ * and quartered (less LDS, more resident workgroups per CU).  Variants 3-5
+ * march a workgroup down a strip of 64, 128 or 256 rows through an LDS ring
+ * with register prefetch (rows fetched once per strip).  ss_autotune_tiles
+ * times every pass in every variant ``reps`` times on the live buffers, keeps
+ * the fastest per pass, and reports chosen[0..3] and ms[6*(pass-1)+variant];
+ * it writes only scratch buffers, so the relaxation state is unchanged.
+ * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
+ * do not depend on the variants; energies only through FP64 summation order. */
 int ss_set_tile_variants(ss_ctx* ctx, const int* variants);
 int ss_get_tile_variants(ss_ctx* ctx, int* variants, int* rows);
 int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);

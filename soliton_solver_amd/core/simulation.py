@@ -273,19 +273,23 @@ class Simulation:
         self._check(_lib.lib().ss_time_passes(self._ctx, float(prev_energy), int(nsteps), out))
         return out
 
+    NUM_SHAPES = 6
+
     def autotune_tiles(self, reps: int = 10) -> dict:
-        """Time every fused pass in each tile-shape variant on the live buffers and keep
-        the fastest per pass (the relaxation state is unchanged).  Returns the chosen
-        variants, the rows per tile they give and the ms table [pass][variant]."""
+        """Time every fused pass in each shape variant on the live buffers and keep the
+        fastest per pass (the relaxation state is unchanged).  Variants 0-2 are LDS
+        tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows in the
+        streaming form.  Returns the chosen variants, their rows per tile/strip and the
+        ms table [pass][variant]."""
         chosen = np.zeros(4, dtype=np.int32)
-        ms = np.zeros(12)
+        ms = np.zeros(4 * self.NUM_SHAPES)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
-        info["ms"] = ms.reshape(4, 3)
+        info["ms"] = ms.reshape(4, self.NUM_SHAPES)
         return info
 
     def set_tile_variants(self, variants) -> None:
-        """Tile-shape variant (0, 1 or 2) of passes 1..4."""
+        """Shape variant (0..5, see ``autotune_tiles``) of passes 1..4."""
         v = np.ascontiguousarray(variants, dtype=np.int32)
         if v.shape != (4,):
             raise ValueError("tile variants: one entry per fused pass (4)")

@@ -28,22 +28,9 @@
 #include "common.h"
 #include "dual.h"
 #include "stencil.h"
+#include "stream.h"
 
 namespace ss {
-
-struct PassBufs {
-  const void* F;
-  const void* V;
-  const void* l1;
-  const void* l2;
-  const void* l3;
-  void* lout;      // pass 0: EG; passes 1-3: l_pass
-  void* Fout;      // pass 4
-  void* Vout;      // pass 4
-  double* part0;   // pass 1: energy partial per tile; pass 4: signed max partial
-  double* part1;   // pass 4: max |EG| partial
-  const DevState* st;
-};
 
 struct Launch {
   int ntiles, ntiles_y, per_xcd, grid;
@@ -637,13 +624,40 @@ template <class TH>
 struct TheoryLaunchers {
   using TP = typename TH::TP;
 
-  // Tiles of one pass launch for tile-shape variant ``v`` (0 default rows, 1 and 2 halved).
+  // Shape variants of the fused passes: 0..2 tiles with the default rows, halved,
+  // quartered (at least 4); 3..5 strips of 64, 128, 256 rows (streaming form).
+  static constexpr int kNumShapes = 6;
+  static int strip_rows(int v) { return 64 << (v - 3); }
+
+  // Partials (tiles or strips) of one pass launch in shape variant ``v``.
   template <typename R>
   static int ntiles_r(int v, const Geom& g) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (v >= 3) return make_slaunch<R, TH::NF>(g, strip_rows(v)).nblocks;
     return v == 2 ? make_launch<R, TH::NF, r2>(g).ntiles
          : v == 1 ? make_launch<R, TH::NF, r1>(g).ntiles
                   : make_launch<R, TH::NF, r0>(g).ntiles;
+  }
+
+  // Rows per tile (variants 0..2) or per strip (3..5).
+  static int shape_rows(int prec, int v) {
+    if (v >= 3) return strip_rows(v);
+    return prec ? variant_rows<float, TH::NF>(v) : variant_rows<double, TH::NF>(v);
+  }
+
+  template <typename R>
+  static hipError_t stream_r(int pass, int sx, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
+    const SLaunch L = make_slaunch<R, TH::NF>(g, sx);
+    dim3 grid(L.grid), block(kThreads);
+    switch (pass) {
+      case 0: hipLaunchKernelGGL((stream_pass<TH, R, 0>), grid, block, 0, st, b, tp, g, L); break;
+      case 1: hipLaunchKernelGGL((stream_pass<TH, R, 1>), grid, block, 0, st, b, tp, g, L); break;
+      case 2: hipLaunchKernelGGL((stream_pass<TH, R, 2>), grid, block, 0, st, b, tp, g, L); break;
+      case 3: hipLaunchKernelGGL((stream_pass<TH, R, 3>), grid, block, 0, st, b, tp, g, L); break;
+      case 4: hipLaunchKernelGGL((stream_pass<TH, R, 4>), grid, block, 0, st, b, tp, g, L); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
   }
 
   static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
@@ -666,6 +680,8 @@ struct TheoryLaunchers {
   template <typename R>
   static hipError_t pass_r(int pass, int v, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
+    if (v >= 3) return stream_r<R>(pass, strip_rows(v), b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
     return pass_rv<R, r0>(pass, b, tp, g, st);
@@ -793,6 +809,8 @@ struct TheoryOps {
   hipError_t (*combine)(int, const PassBufs&, const double*, const void*, const Geom&, hipStream_t, int*);
   int (*batch_inst_size)();
   int (*tile_rows)(int);
+  int (*shape_rows)(int, int);
+  int num_shapes;
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
@@ -810,6 +828,8 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::combine,                    \
                                       &ss::TheoryLaunchers<TH>::batch_inst_size,            \
                                       &ss::TheoryLaunchers<TH>::tile_rows,                  \
+                                      &ss::TheoryLaunchers<TH>::shape_rows,                 \
+                                      ss::TheoryLaunchers<TH>::kNumShapes,                  \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \

@@ -301,7 +301,7 @@ static int alloc_fields(ss_ctx* c) {
 
 static int alloc_partials(ss_ctx* c) {
   int nt = 0;
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < c->ops->num_shapes; ++v) {
     const int n = c->ops->ntiles(c->prec, v, c->g);
     nt = n > nt ? n : nt;
   }
@@ -786,7 +786,8 @@ int ss_set_tile_variants(ss_ctx* c, const int* variants) {
   if (!c) return fail(nullptr, "null context");
   if (!variants) return fail(c, "null variant array");
   for (int p = 0; p < 4; ++p)
-    if (variants[p] < 0 || variants[p] > 2) return fail(c, "tile variant of pass %d is %d (valid: 0, 1, 2)", p + 1, variants[p]);
+    if (variants[p] < 0 || variants[p] >= c->ops->num_shapes)
+      return fail(c, "shape variant of pass %d is %d (valid: 0 .. %d)", p + 1, variants[p], c->ops->num_shapes - 1);
   for (int p = 0; p < 4; ++p) c->tv[p + 1] = variants[p];
   free_graphs(c);  // captured graphs hold the old launch shapes
   return 0;
@@ -796,10 +797,7 @@ int ss_get_tile_variants(ss_ctx* c, int* variants, int* rows) {
   if (!c) return fail(nullptr, "null context");
   for (int p = 0; p < 4; ++p) {
     if (variants) variants[p] = c->tv[p + 1];
-    if (rows) {
-      const int r = c->ops->tile_rows(c->prec) >> c->tv[p + 1];  // as variant_rows()
-      rows[p] = r < 4 ? 4 : r;
-    }
+    if (rows) rows[p] = c->ops->shape_rows(c->prec, c->tv[p + 1]);
   }
   return 0;
 }
@@ -824,7 +822,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   for (int p = 1; p <= 4 && rc == 0; ++p) {
     b.lout = p <= 3 ? c->l[p - 1] : nullptr;
     float best = 0.f;
-    for (int v = 0; v < 3 && rc == 0; ++v) {
+    for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
       hipError_t e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);  // warm-up
       if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
       for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);
@@ -837,7 +835,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         break;
       }
       t /= reps;
-      if (ms) ms[(p - 1) * 3 + v] = t;
+      if (ms) ms[(p - 1) * c->ops->num_shapes + v] = t;
       if (v == 0 || t < best) {
         best = t;
         pick[p] = v;

@@ -1,0 +1,325 @@
+// Streaming form of the fused RK4 passes.
+//
+// A workgroup owns a strip of TY = 64 columns (y) by ``sx`` rows (x) and
+// marches down it RB rows at a time.  The stage field of the rows it needs
+// (RB computed rows plus the 2-row halo either side) lives in an LDS ring of
+// NR >= RB + 4 rows (row r in slot (r - x0 + 2) mod NR); after each group of
+// RB rows is computed, the next RB rows are written into slots no longer read.  The
+// global loads for those rows are issued one iteration ahead into registers,
+// so they are in flight while the current rows are computed.  Compared with
+// the tile form, rows are fetched once per strip instead of (TX + 4)/TX times,
+// and the ring is small enough for many resident workgroups per CU.
+//
+// Per site the arithmetic is the tile form's (same stencils, same order, same
+// theory calls), so the two forms give bitwise the same fields; only the order
+// of the FP64 partial sums differs.
+#pragma once
+
+#include "common.h"
+#include "stencil.h"
+
+namespace ss {
+
+// Buffers of one fused pass (tile and streaming forms).
+struct PassBufs {
+  const void* F;
+  const void* V;
+  const void* l1;
+  const void* l2;
+  const void* l3;
+  void* lout;      // pass 0: EG; passes 1-3: l_pass
+  void* Fout;      // pass 4
+  void* Vout;      // pass 4
+  double* part0;   // pass 1: energy partial per tile; pass 4: signed max partial
+  double* part1;   // pass 4: max |EG| partial
+  const DevState* st;
+};
+
+template <typename R, int NF>
+struct StreamShape {
+  static constexpr int RY = TY + 4;
+  static constexpr int kBytesRow = NF * RY * (int)sizeof(R);
+  // Two rows per wave and iteration when that ring fits 32 KiB, else one.
+  static constexpr int RPW = ((kWaves * 2 + 4) * kBytesRow <= 32768) ? 2 : 1;
+  static constexpr int RB = kWaves * RPW;  // rows computed per iteration
+  static constexpr int NR = (RB + 4 <= 8) ? 8 : 16;  // ring rows: a power of two >= RB + 4
+  static_assert(RB + 4 <= NR, "ring too small");
+  static_assert(NR * kBytesRow <= 65536, "ring does not fit in 64 KiB of LDS");
+};
+
+template <typename R, int NF>
+struct Ring {
+  using S = StreamShape<R, NF>;
+  R v[NF][S::NR][S::RY];
+};
+
+struct SLaunch {
+  int sx;       // rows per strip (a multiple of RB)
+  int nsy;      // strips along y
+  int nblocks;  // strips
+  int per_xcd, grid;
+};
+
+template <typename R, int NF>
+inline SLaunch make_slaunch(const Geom& g, int sx) {
+  SLaunch L;
+  L.sx = sx;
+  L.nsy = (g.Y + TY - 1) / TY;
+  L.nblocks = ((g.X + sx - 1) / sx) * L.nsy;
+  L.per_xcd = (L.nblocks + 7) / 8;
+  L.grid = 8 * L.per_xcd;
+  return L;
+}
+
+// Stencils of component a at ring column cy; sl[0..4] are the ring slots of
+// rows x-2 .. x+2.  Same formulas and masks as window_derivs.
+template <typename T, typename R, int NF>
+__device__ __forceinline__ void ring_derivs(const Ring<R, NF>& q, int a, const int sl[5], int cy, const Geom& g,
+                                            const SiteInfo& s, T& f, T d1[2], T d2[3]) {
+  const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
+  const T c = T(q.v[a][sl[2]][cy]);
+  const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
+  const T xp1 = T(q.v[a][sl[3]][cy]), xp2 = T(q.v[a][sl[4]][cy]);
+  const T ym2 = T(q.v[a][sl[2]][cy - 2]), ym1 = T(q.v[a][sl[2]][cy - 1]);
+  const T yp1 = T(q.v[a][sl[2]][cy + 1]), yp2 = T(q.v[a][sl[2]][cy + 2]);
+  const T sxx = c43 * ((xm1 - c) + (xp1 - c)) - c12 * ((xm2 - c) + (xp2 - c));
+  const T syy = c43 * ((ym1 - c) + (yp1 - c)) - c12 * ((ym2 - c) + (yp2 - c));
+  f = c;
+  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
+  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
+  d2[0] = s.inx ? sxx * T(g.inv_lsx2) : T(0);
+  d2[1] = s.iny ? syy * T(g.inv_lsy2) : T(0);
+  if (s.inxy) {
+    const T dm2 = T(q.v[a][sl[0]][cy - 2]), dm1 = T(q.v[a][sl[1]][cy - 1]);
+    const T dp1 = T(q.v[a][sl[3]][cy + 1]), dp2 = T(q.v[a][sl[4]][cy + 2]);
+    const T sdd = c43 * ((dm1 - c) + (dp1 - c)) - c12 * ((dm2 - c) + (dp2 - c));
+    d2[2] = sdd * T(g.inv_2gv) - sxx * T(g.inv_2lsx2) - syy * T(g.inv_2lsy2);
+  } else {
+    d2[2] = T(0);
+  }
+}
+
+template <typename T, typename R, int NF>
+__device__ __forceinline__ void ring_d1(const Ring<R, NF>& q, int a, const int sl[5], int cy, const Geom& g,
+                                        const SiteInfo& s, T& f, T d1[2]) {
+  const T c12 = T(SS_C12), c23 = T(SS_C23);
+  const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
+  const T xp1 = T(q.v[a][sl[3]][cy]), xp2 = T(q.v[a][sl[4]][cy]);
+  const T ym2 = T(q.v[a][sl[2]][cy - 2]), ym1 = T(q.v[a][sl[2]][cy - 1]);
+  const T yp1 = T(q.v[a][sl[2]][cy + 1]), yp2 = T(q.v[a][sl[2]][cy + 2]);
+  f = T(q.v[a][sl[2]][cy]);
+  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
+  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
+}
+
+// Raw inputs of the stage field for the RB rows of one group, per thread: RPW
+// interior sites (rows w*RPW + k, column lane) and, for threads < 4*RB, one
+// halo-column site (row tid/4, column 0, 1, 66 or 67 of the ring).
+template <typename R, int NF, int RPW>
+struct StreamPrefetch {
+  R f[RPW + 1][NF], v[RPW + 1][NF], l[RPW + 1][NF];
+};
+
+template <typename R, int NF>
+__device__ __forceinline__ void group_site(int k, int rb, int y0, int& r, int& c, int& x, int& y) {
+  using S = StreamShape<R, NF>;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (k < S::RPW) {
+    r = w * S::RPW + k;
+    c = lane + 2;
+  } else {
+    r = threadIdx.x >> 2;
+    const int qd = threadIdx.x & 3;
+    c = qd < 2 ? qd : TY + qd;
+  }
+  x = rb + r;
+  y = y0 - 2 + c;
+}
+
+// Issue the loads of the group of rows starting at rb (values of sites outside
+// the lattice are 0).  PASS selects the arrays: 0/1 F; 2 F, V; 3 F, V, l1; 4 F, V, l2.
+template <int PASS, typename R, int NF>
+__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NF>::RPW>& p, const PassBufs& b,
+                                           const Geom& g, int rb, int y0, bool kill) {
+  using S = StreamShape<R, NF>;
+  const R* __restrict__ F = (const R*)b.F;
+  const R* __restrict__ V = (const R*)b.V;
+  const R* __restrict__ L = (const R*)(PASS == 3 ? b.l1 : b.l2);
+#pragma unroll
+  for (int k = 0; k <= S::RPW; ++k) {
+    int r, c, x, y;
+    group_site<R, NF>(k, rb, y0, r, c, x, y);
+    const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
+    const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
+    const long long off = (long long)x * g.Y + y;
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const long long o = off + (long long)a * g.N;
+      p.f[k][a] = in ? F[o] : R(0);
+      p.v[k][a] = (PASS >= 2 && in && !kill) ? V[o] : R(0);
+      p.l[k][a] = (PASS >= 3 && in) ? L[o] : R(0);
+    }
+  }
+}
+
+template <int PASS, typename R>
+__device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
+  const R half = R(0.5);
+  if (PASS <= 1) return fo;
+  if (PASS == 2) return fo + half * (dt * v);
+  if (PASS == 3) return fo + half * (dt * (v + half * l));
+  return fo + dt * (v + half * l);
+}
+
+// Write a group into the ring; slot0 is the ring slot of its first row.
+template <int PASS, typename R, int NF>
+__device__ __forceinline__ void group_store(Ring<R, NF>& q, const StreamPrefetch<R, NF, StreamShape<R, NF>::RPW>& p,
+                                            int slot0, R dt) {
+  using S = StreamShape<R, NF>;
+#pragma unroll
+  for (int k = 0; k <= S::RPW; ++k) {
+    if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
+    int r, c, x, y;
+    group_site<R, NF>(k, 0, 0, r, c, x, y);
+    const int slot = (slot0 + r) & (S::NR - 1);
+#pragma unroll
+    for (int a = 0; a < NF; ++a) q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+  }
+}
+
+// One strip of one pass (PASS 0..4 as in stage_tile).  Partials: one per strip.
+template <class TH, typename R, int PASS>
+__device__ __forceinline__ void stream_strip(Ring<R, TH::NF>& q, double* red, const PassBufs& b,
+                                             const typename TH::TP& tp, const Geom& g, int xs, int sx, int y0,
+                                             bool kill, int t) {
+  using S = StreamShape<R, TH::NF>;
+  constexpr int NF = TH::NF, RB = S::RB, NR = S::NR, RPW = S::RPW;
+  const R dt = R(g.dt);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int y = y0 + lane;
+  const int cy = lane + 2;
+  const int nit = sx / RB;
+
+  // Prologue: rows xs-2 .. xs+RB+1 straight into ring slots 0 .. RB+3.  Row
+  // xs - 2 + j lives in slot j mod NR throughout.
+  for (int idx = threadIdx.x; idx < (RB + 4) * S::RY; idx += kThreads) {
+    const int r = idx / S::RY, c = idx - r * S::RY;
+    const int gx = xs - 2 + r, gy = y0 - 2 + c;
+    const bool in = gx >= 0 && gx < g.X && gy >= 0 && gy < g.Y;
+    const long long off = (long long)gx * g.Y + gy;
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      const long long o = off + (long long)a * g.N;
+      R val = R(0);
+      if (in) {
+        const R fo = ((const R*)b.F)[o];
+        const R v = (PASS >= 2 && !kill) ? ((const R*)b.V)[o] : R(0);
+        const R l = PASS == 3 ? ((const R*)b.l1)[o] : PASS == 4 ? ((const R*)b.l2)[o] : R(0);
+        val = stage_value<PASS, R>(fo, v, l, dt);
+      }
+      q.v[a][r][c] = val;
+    }
+  }
+  StreamPrefetch<R, NF, RPW> pf;
+  if (nit > 1) group_load<PASS, R, NF>(pf, b, g, xs + RB + 2, y0, kill);
+
+  double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
+  double acc1 = 0.0;
+  for (int it = 0; it < nit; ++it) {
+    __syncthreads();  // ring holds rows xs + it*RB - 2 .. xs + it*RB + RB + 1
+    const int c0 = xs + it * RB;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int x = c0 + w * RPW + k;
+      if (x < g.X && y < g.Y) {
+        int sl[5];
+        const int base = x - xs;  // slot of row x-2
+#pragma unroll
+        for (int d = 0; d < 5; ++d) sl[d] = (base + d) & (NR - 1);
+        const SiteInfo s = site_info(g, x, y);
+        R f[NF], d1[NF][2], d2[NF][3], eg[NF];
+#pragma unroll
+        for (int a = 0; a < NF; ++a) ring_derivs<R, R, NF>(q, a, sl, cy, g, s, f[a], d1[a], d2[a]);
+        TH::template variation<R>(tp, g, s, f, d1, d2, eg);
+        const long long off = (long long)x * g.Y + y;
+        if (PASS == 0) {
+          R* out = (R*)b.lout;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) out[off + a * g.N] = eg[a];
+        } else if (PASS <= 3) {
+          R* out = (R*)b.lout;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) out[off + a * g.N] = -dt * eg[a];
+        }
+        if (PASS == 1) {
+          double fd[NF], d1d[NF][2];
+#pragma unroll
+          for (int a = 0; a < NF; ++a) ring_d1<double, R, NF>(q, a, sl, cy, g, s, fd[a], d1d[a]);
+          acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+        }
+        if (PASS == 4) {
+          // F' = Temp4 + dt (l1 - 2 l2 + l3)/6 with Temp4 the ring centre.
+          const R* __restrict__ Vg = (const R*)b.V;
+          const R* __restrict__ l1 = (const R*)b.l1;
+          const R* __restrict__ l2 = (const R*)b.l2;
+          const R* __restrict__ l3 = (const R*)b.l3;
+          R Fn[NF], Vn[NF];
+          const R sixth = R(1.0 / 6.0);
+          const R dt6 = dt * sixth;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            const long long o = off + a * g.N;
+            const R Vc = kill ? R(0) : Vg[o];
+            const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
+            const R a4 = -dt * eg[a];
+            Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
+            Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
+            const double e = (double)eg[a];
+            acc0 = fmax(acc0, e);
+            acc1 = fmax(acc1, fabs(e));
+          }
+          if (TH::kUnitConstraint && g.unit_mag) {
+            TH::template normalize<R>(tp, Fn);
+            TH::template project<R>(tp, Vn, Fn);
+          }
+          R* Fo = (R*)b.Fout;
+          R* Vo = (R*)b.Vout;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            Fo[off + a * g.N] = Fn[a];
+            Vo[off + a * g.N] = Vn[a];
+          }
+        }
+      }
+    }
+    __syncthreads();  // every read of this iteration's rows is done
+    if (it + 1 < nit) {
+      // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
+      // no row of iteration it+1 other than themselves maps to
+      group_store<PASS, R, NF>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
+      if (it + 2 < nit) group_load<PASS, R, NF>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
+    }
+  }
+  if (PASS == 1) {
+    const double s = block_sum<kThreads>(acc0, red);
+    if (threadIdx.x == 0) b.part0[t] = s;
+  } else if (PASS == 4) {
+    const double m0 = block_max<kThreads>(acc0, red);
+    if (threadIdx.x == 0) b.part0[t] = m0;
+    const double m1 = block_max<kThreads>(acc1, red);
+    if (threadIdx.x == 0) b.part1[t] = m1;
+  }
+}
+
+template <class TH, typename R, int PASS>
+__global__ void __launch_bounds__(kThreads) stream_pass(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
+  __shared__ Ring<R, TH::NF> q;
+  __shared__ double red[kWaves];
+  const int t = xcd_tile(blockIdx.x, L.nblocks, L.per_xcd);
+  if (t < 0) return;  // whole block exits together, before any barrier
+  const int sxi = t / L.nsy, syi = t - sxi * L.nsy;
+  const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
+  stream_strip<TH, R, PASS>(q, red, b, tp, g, sxi * L.sx, L.sx, syi * TY, kill, t);
+}
+
+}  // namespace ss

@@ -393,9 +393,12 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     assert np.abs(Fg - Fo).max() <= 1e-9
 
 
-@pytest.mark.parametrize("variants", [[1, 1, 1, 1], [2, 2, 2, 2], [0, 2, 1, 2]])
+@pytest.mark.parametrize("variants", [[1, 1, 1, 1], [2, 2, 2, 2], [0, 2, 1, 2], [3, 3, 3, 3], [4, 4, 4, 4],
+                                      [5, 5, 5, 5], [0, 3, 4, 5]])
 def test_tile_shape_variants_match_oracle(variants):
-    """Every tile-shape variant of the fused passes gives the oracle's trajectory."""
+    """Every launch-shape variant of the fused passes (tiles and streaming strips,
+    lattice 70 x 130: partial strips and partial column blocks) gives the oracle's
+    trajectory."""
     th = ssa.load_theory("Chiral magnet")
     p = H.chiral_params(th, 70, 130)
     p_i, p_f = H.packed(th, p)
@@ -431,10 +434,41 @@ def test_autotune_leaves_state_unchanged_and_steps_agree():
     info = a.autotune_tiles(reps=3)
     print(f"[autotune 200x256 fp32] {info}")
     assert np.array_equal(a.get_field(), Fa) and np.array_equal(a.get_velocity(), Va)
-    assert all(v in (0, 1, 2) for v in info["variants"])
+    assert all(0 <= v < 6 for v in info["variants"])
     Ea, _, _ = a.steps(E0, 8)
     Eb, _, _ = b.steps(E0, 8)
     assert H.rel(Ea, Eb) < 1e-12
     assert np.abs(a.get_field() - b.get_field()).max() <= 1e-6
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("name", ["Ferromagnetic superconductor", "Spin triplet superconducting ferromagnet",
+                                  "Baby Skyrme model"])
+def test_streaming_form_matches_oracle_for_wide_theories(name):
+    """Streaming strips with one row per wave (8 and 10 FP64 fields) and the
+    unit-norm baby Skyrme model: same trajectory as the oracle."""
+    th = ssa.load_theory(name)
+    if name.startswith("Ferro"):
+        p = H.fs_params(th, 90, 70, eta2=0.05)
+        mk = lambda pi, pf: H.fs_field(th, pi, pf, seed=3)
+    elif name.startswith("Spin"):
+        p = H.st_params(th, 90, 70)
+        mk = lambda pi, pf: H.st_field(th, pi, pf, seed=3)
+    else:
+        p = H.bs_params(th, 90, 70, xsize=10.0)
+        mk = lambda pi, pf: H.bs_field(th, pi, pf, seed=3, k=2)
+    p_i, p_f = H.packed(th, p)
+    F = mk(p_i, p_f)
+    o = H.ORACLES[name]
+    Fo, Vo = F.copy(), np.zeros_like(F)
+    Es, _ = ocore.relax(o, Fo, Vo, p_i, p_f, 6, prev_energy=ocore.theory_energy(o, Fo, p_i, p_f))
+    sim = _sim(th, p, F)
+    sim.set_tile_variants([3, 4, 5, 3])
+    sim.steps(sim.compute_energy(), 6, history=True)
+    gE = np.array(sim.last_history[0])
+    dF = np.abs(sim.get_field() - Fo).max()
+    print(f"[stream {name}] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={dF:.2e}")
+    assert np.all(np.abs(gE - np.array(Es)) <= 1e-10 * np.abs(np.array(Es)))
+    assert dF <= 1e-9
+    sim.close()
