@@ -46,7 +46,7 @@ def chiral_workload(n: int, seed: int = 0, nsk: int = 8):
     return th, p, {"mode": "skyrmions", "ansatz": "anti", "skyrmions": sk, "noise": 1e-3, "seed": seed + 1}
 
 
-def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotune=True, tiles=None):
+def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotune=True, tiles=None, fused23=-1):
     import soliton_solver_amd as ssa
     sim = ssa.Simulation(p, th, precision=precision)
     if F0 is None:
@@ -55,6 +55,8 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
         sim.initialize({"mode": "array", "field": F0})
     if tiles is not None:
         sim.set_tile_variants(tiles)
+        if fused23 >= 0:
+            sim.set_fused_stages(bool(fused23))
         sim.tiles = sim.tile_variants()
     else:
         sim.tiles = sim.autotune_tiles(reps=5) if autotune else sim.tile_variants()
@@ -86,6 +88,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape of every pass")
     ap.add_argument("--tiles", default="", help="tile variants of passes 1-4 for the FP32 form, e.g. 1,1,1,2 "
                                                 "(no autotuning)")
+    ap.add_argument("--fused23", type=int, default=-1, help="with --tiles: 1 runs passes 2+3 as one sweep")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprof byte counters)")
@@ -121,9 +124,17 @@ def main():
     print(f"[bench] building {n}x{n} chiral workload", file=sys.stderr, flush=True)
     sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, "fp32", args.steps, args.warmup, args.graph_steps,
                                              autotune=not args.no_autotune,
-                                             tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None)
+                                             tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None,
+                                             fused23=args.fused23)
     tiles32 = sim32.tiles
     print(f"[bench] fp32 tile rows per pass {tiles32['rows']}", file=sys.stderr, flush=True)
+    if "ms" in tiles32:
+        for k, row in enumerate(tiles32["ms"]):
+            print(f"[bench]   pass {k + 1} ms by shape variant: " + " ".join(f"{x:.4f}" for x in row),
+                  file=sys.stderr, flush=True)
+        print(f"[bench]   fused passes 2+3 ms by strip variant: "
+              + " ".join(f"{x:.4f}" for x in tiles32["ms_fused23"]) + f"; fused: {tiles32['fused23']}",
+              file=sys.stderr, flush=True)
     F0 = None
     N = n * n
     nf = 4
@@ -201,10 +212,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "tile_variants": tiles32["variants"],
+        "fused23": int(tiles32.get("fused23", False)),
         "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
                                "FP32 fields+stencils, FP64 energy sums and arrest test",
                    "lattice": [n, n], "fields": nf, "graph_steps": args.graph_steps,
                    "tile_rows_per_pass": tiles32["rows"], "tile_shape": "64 y x rows x",
+                   "fused_passes_2_3": tiles32.get("fused23", False),
                    "parallelism": "single GPU"},
         "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

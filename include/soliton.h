@@ -137,13 +137,24 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * march a workgroup down a strip of 64, 128 or 256 rows through an LDS ring
  * with register prefetch (rows fetched once per strip).  ss_autotune_tiles
  * times every pass in every variant ``reps`` times on the live buffers, keeps
- * the fastest per pass, and reports chosen[0..3] and ms[6*(pass-1)+variant];
+ * the fastest per pass, and reports chosen[0..3] and ms[6*(pass-1)+variant]
+ * (ms holds 27 entries, see ss_set_fused_stages);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
  * do not depend on the variants; energies only through FP64 summation order. */
 int ss_set_tile_variants(ss_ctx* ctx, const int* variants);
 int ss_get_tile_variants(ss_ctx* ctx, int* variants, int* rows);
 int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
+
+/* Passes 2 and 3 of a step read the same inputs (F, V, l1), so they can run as
+ * one streaming sweep that builds both stage fields from one set of loads and
+ * writes l2 and l3 (no counterpart in the reference).  ss_autotune_tiles also
+ * times this fused sweep in the streaming variants (ms[24..26]) and switches it
+ * on when it beats the best pair of separate passes; ss_set_fused_stages
+ * forces it (on uses the pass-2 variant, at least 3) and fails when the two
+ * stage fields do not fit one LDS ring for the theory and precision. */
+int ss_set_fused_stages(ss_ctx* ctx, int on);
+int ss_get_fused_stages(ss_ctx* ctx, int* on);
 
 /* Device memory held by the context, in bytes. */
 int ss_memory_bytes(ss_ctx* ctx, size_t* bytes);

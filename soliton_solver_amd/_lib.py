@@ -61,6 +61,8 @@ SIGNATURES = {
     "ss_set_tile_variants": (C.c_int, [C.c_void_p, _i32p]),
     "ss_get_tile_variants": (C.c_int, [C.c_void_p, _i32p, _i32p]),
     "ss_autotune_tiles": (C.c_int, [C.c_void_p, C.c_int, _i32p, _f64p]),
+    "ss_set_fused_stages": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_get_fused_stages": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "ss_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "ss_last_error": (C.c_char_p, [C.c_void_p]),
     "ss_batch_create": (C.c_int, [C.c_int, C.c_int, _i32p, C.c_int, _f64p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),

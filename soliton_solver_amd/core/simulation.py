@@ -282,11 +282,16 @@ class Simulation:
         streaming form.  Returns the chosen variants, their rows per tile/strip and the
         ms table [pass][variant]."""
         chosen = np.zeros(4, dtype=np.int32)
-        ms = np.zeros(4 * self.NUM_SHAPES)
+        ms = np.zeros(4 * self.NUM_SHAPES + 3)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
-        info["ms"] = ms.reshape(4, self.NUM_SHAPES)
+        info["ms"] = ms[:4 * self.NUM_SHAPES].reshape(4, self.NUM_SHAPES)
+        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:]
         return info
+
+    def set_fused_stages(self, on: bool) -> None:
+        """Run passes 2 and 3 as one streaming sweep (see ``ss_set_fused_stages``)."""
+        self._check(_lib.lib().ss_set_fused_stages(self._ctx, int(bool(on))))
 
     def set_tile_variants(self, variants) -> None:
         """Shape variant (0..5, see ``autotune_tiles``) of passes 1..4."""
@@ -299,7 +304,9 @@ class Simulation:
         v = np.zeros(4, dtype=np.int32)
         rows = np.zeros(4, dtype=np.int32)
         self._check(_lib.lib().ss_get_tile_variants(self._ctx, v, rows))
-        return {"variants": v.tolist(), "rows": rows.tolist()}
+        on = C.c_int()
+        self._check(_lib.lib().ss_get_fused_stages(self._ctx, C.byref(on)))
+        return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value)}
 
     def synchronize(self) -> None:
         self._check(_lib.lib().ss_synchronize(self._ctx))

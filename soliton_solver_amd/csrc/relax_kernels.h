@@ -628,11 +628,17 @@ struct TheoryLaunchers {
   // quartered (at least 4); 3..5 strips of 64, 128, 256 rows (streaming form).
   static constexpr int kNumShapes = 6;
   static int strip_rows(int v) { return 64 << (v - 3); }
+  // The streaming form indexes with 32-bit element offsets; larger fields run
+  // the default tile shape instead.
+  static int effective_shape(int v, const Geom& g) {
+    return (v >= 3 && (long long)TH::NF * g.N >= (1LL << 31)) ? 0 : v;
+  }
 
   // Partials (tiles or strips) of one pass launch in shape variant ``v``.
   template <typename R>
   static int ntiles_r(int v, const Geom& g) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    v = effective_shape(v, g);
     if (v >= 3) return make_slaunch<R, TH::NF>(g, strip_rows(v)).nblocks;
     return v == 2 ? make_launch<R, TH::NF, r2>(g).ntiles
          : v == 1 ? make_launch<R, TH::NF, r1>(g).ntiles
@@ -655,9 +661,23 @@ struct TheoryLaunchers {
       case 2: hipLaunchKernelGGL((stream_pass<TH, R, 2>), grid, block, 0, st, b, tp, g, L); break;
       case 3: hipLaunchKernelGGL((stream_pass<TH, R, 3>), grid, block, 0, st, b, tp, g, L); break;
       case 4: hipLaunchKernelGGL((stream_pass<TH, R, 4>), grid, block, 0, st, b, tp, g, L); break;
+      case 5:
+        if constexpr (StreamShape<R, 2 * TH::NF>::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 23>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+  }
+
+  // Passes 2 and 3 fused into one streaming sweep (pass id 5) when both
+  // stage fields fit one ring and the field uses 32-bit offsets.
+  static int can_fuse23(int prec, const Geom& g) {
+    const bool fits = prec ? StreamShape<float, 2 * TH::NF>::kFits : StreamShape<double, 2 * TH::NF>::kFits;
+    return fits && effective_shape(3, g) == 3;
   }
 
   static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
@@ -681,6 +701,8 @@ struct TheoryLaunchers {
   static hipError_t pass_r(int pass, int v, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
+    v = effective_shape(v, g);
+    if (pass == 5) return v >= 3 ? stream_r<R>(pass, strip_rows(v), b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_r<R>(pass, strip_rows(v), b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -811,6 +833,7 @@ struct TheoryOps {
   int (*tile_rows)(int);
   int (*shape_rows)(int, int);
   int num_shapes;
+  int (*can_fuse23)(int, const Geom&);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
@@ -830,6 +853,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::tile_rows,                  \
                                       &ss::TheoryLaunchers<TH>::shape_rows,                 \
                                       ss::TheoryLaunchers<TH>::kNumShapes,                  \
+                                      &ss::TheoryLaunchers<TH>::can_fuse23,                 \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \

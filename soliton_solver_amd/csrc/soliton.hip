@@ -184,6 +184,7 @@ struct ss_ctx {
   int ntiles = 0;          // capacity of each partials row (the most tiles over the shape variants)
   int nt0 = 0;             // tiles of the default shape (energy, charge and pass-0 kernels)
   int tv[5] = {0, 0, 0, 0, 0};  // tile-shape variant of each fused pass (index = pass)
+  int fuse23 = 0;          // passes 2 and 3 as one streaming sweep (variant tv[2], at least 3)
   double* part = nullptr;  // 3 x ntiles
   DevState* st = nullptr;
   double* hist = nullptr;  // 2 x hist_cap
@@ -365,12 +366,20 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
                      c->ops->ntiles(c->prec, c->tv[1], c->g), c->st, histE, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  b.lout = c->l[1];
-  HIPCHK(c, c->ops->pass(c->prec, 2, c->tv[2], b, c->tp, c->g, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  b.lout = c->l[2];
-  HIPCHK(c, c->ops->pass(c->prec, 3, c->tv[3], b, c->tp, c->g, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  if (c->fuse23) {
+    b.lout = c->l[1];
+    b.lout2 = c->l[2];
+    HIPCHK(c, c->ops->pass(c->prec, 5, c->tv[2] < 3 ? 3 : c->tv[2], b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate pass 3
+  } else {
+    b.lout = c->l[1];
+    HIPCHK(c, c->ops->pass(c->prec, 2, c->tv[2], b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    b.lout = c->l[2];
+    HIPCHK(c, c->ops->pass(c->prec, 3, c->tv[3], b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  }
   b.lout = nullptr;
   b.part0 = p0;
   b.part1 = p1;
@@ -818,6 +827,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   b.part0 = c->part;
   b.part1 = c->part + c->ntiles;
   int pick[5] = {0, 0, 0, 0, 0};
+  float bestp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   int rc = 0;
   for (int p = 1; p <= 4 && rc == 0; ++p) {
     b.lout = p <= 3 ? c->l[p - 1] : nullptr;
@@ -841,6 +851,34 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         pick[p] = v;
       }
     }
+    bestp[p] = best;
+  }
+  const float sep23 = bestp[2] + bestp[3];
+  // passes 2 + 3 fused (streaming variants only) against the best separate pair
+  float best23 = 0.f;
+  int pick23 = -1;
+  if (rc == 0 && c->ops->can_fuse23(c->prec, c->g)) {
+    b.lout = c->l[1];
+    b.lout2 = c->l[2];
+    for (int v = 3; v < c->ops->num_shapes && rc == 0; ++v) {
+      hipError_t e = c->ops->pass(c->prec, 5, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, 5, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        rc = hipfail(c, "tile autotuning of fused passes 2+3 variant %d failed: %s", v, hipGetErrorString(e));
+        break;
+      }
+      t /= reps;
+      if (ms) ms[4 * c->ops->num_shapes + (v - 3)] = t;
+      if (pick23 < 0 || t < best23) {
+        best23 = t;
+        pick23 = v;
+      }
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -849,7 +887,28 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     c->tv[p] = pick[p];
     if (chosen) chosen[p - 1] = pick[p];
   }
+  c->fuse23 = 0;
+  if (pick23 >= 0 && best23 < sep23) {
+    c->fuse23 = 1;
+    c->tv[2] = pick23;
+  }
   free_graphs(c);
+  return 0;
+}
+
+int ss_set_fused_stages(ss_ctx* c, int on) {
+  if (!c) return fail(nullptr, "null context");
+  if (on && !c->ops->can_fuse23(c->prec, c->g))
+    return fail(c, "passes 2 and 3 of theory %s cannot share one sweep at this precision and size", c->ops->name);
+  c->fuse23 = on ? 1 : 0;
+  if (on && c->tv[2] < 3) c->tv[2] = 3;
+  free_graphs(c);
+  return 0;
+}
+
+int ss_get_fused_stages(ss_ctx* c, int* on) {
+  if (!c) return fail(nullptr, "null context");
+  if (on) *on = c->fuse23;
   return 0;
 }
 

@@ -27,7 +27,8 @@ struct PassBufs {
   const void* l1;
   const void* l2;
   const void* l3;
-  void* lout;      // pass 0: EG; passes 1-3: l_pass
+  void* lout;      // pass 0: EG; passes 1-3: l_pass; fused passes 2+3: l2
+  void* lout2;     // fused passes 2+3: l3
   void* Fout;      // pass 4
   void* Vout;      // pass 4
   double* part0;   // pass 1: energy partial per tile; pass 4: signed max partial
@@ -35,22 +36,25 @@ struct PassBufs {
   const DevState* st;
 };
 
-template <typename R, int NF>
+// Ring of NFR components (NF for one stage field, 2 NF when two stage fields
+// share a pass).
+template <typename R, int NFR>
 struct StreamShape {
   static constexpr int RY = TY + 4;
-  static constexpr int kBytesRow = NF * RY * (int)sizeof(R);
-  // Two rows per wave and iteration when that ring fits 32 KiB, else one.
-  static constexpr int RPW = ((kWaves * 2 + 4) * kBytesRow <= 32768) ? 2 : 1;
+  static constexpr int kBytesRow = NFR * RY * (int)sizeof(R);
+  // Two rows per wave and iteration when that 16-row ring fits 40 KiB, else one.
+  static constexpr int RPW = (16 * kBytesRow <= 40960) ? 2 : 1;
   static constexpr int RB = kWaves * RPW;  // rows computed per iteration
   static constexpr int NR = (RB + 4 <= 8) ? 8 : 16;  // ring rows: a power of two >= RB + 4
+  static constexpr bool kFits = NR * kBytesRow <= 65536;
   static_assert(RB + 4 <= NR, "ring too small");
-  static_assert(NR * kBytesRow <= 65536, "ring does not fit in 64 KiB of LDS");
 };
 
-template <typename R, int NF>
+template <typename R, int NFR>
 struct Ring {
-  using S = StreamShape<R, NF>;
-  R v[NF][S::NR][S::RY];
+  using S = StreamShape<R, NFR>;
+  static_assert(S::kFits, "ring does not fit in 64 KiB of LDS");
+  R v[NFR][S::NR][S::RY];
 };
 
 struct SLaunch {
@@ -71,10 +75,20 @@ inline SLaunch make_slaunch(const Geom& g, int sx) {
   return L;
 }
 
+// Stencil constants in the arithmetic type, converted once per thread.
+template <typename T>
+struct StencilK {
+  T ilx, ily, ilx2, ily2, i2gv, i2lx2, i2ly2;
+  __device__ __forceinline__ explicit StencilK(const Geom& g)
+      : ilx(T(g.inv_lsx)), ily(T(g.inv_lsy)), ilx2(T(g.inv_lsx2)), ily2(T(g.inv_lsy2)), i2gv(T(g.inv_2gv)),
+        i2lx2(T(g.inv_2lsx2)), i2ly2(T(g.inv_2lsy2)) {}
+};
+
 // Stencils of component a at ring column cy; sl[0..4] are the ring slots of
-// rows x-2 .. x+2.  Same formulas and masks as window_derivs.
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void ring_derivs(const Ring<R, NF>& q, int a, const int sl[5], int cy, const Geom& g,
+// rows x-2 .. x+2.  Same formulas and masks as window_derivs; INTERIOR (every
+// mask true) drops the selects.
+template <bool INTERIOR, typename T, typename R, int NFR>
+__device__ __forceinline__ void ring_derivs(const Ring<R, NFR>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                             const SiteInfo& s, T& f, T d1[2], T d2[3]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
   const T c = T(q.v[a][sl[2]][cy]);
@@ -84,23 +98,25 @@ __device__ __forceinline__ void ring_derivs(const Ring<R, NF>& q, int a, const i
   const T yp1 = T(q.v[a][sl[2]][cy + 1]), yp2 = T(q.v[a][sl[2]][cy + 2]);
   const T sxx = c43 * ((xm1 - c) + (xp1 - c)) - c12 * ((xm2 - c) + (xp2 - c));
   const T syy = c43 * ((ym1 - c) + (yp1 - c)) - c12 * ((ym2 - c) + (yp2 - c));
+  const T sx1 = (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * K.ilx;
+  const T sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * K.ily;
   f = c;
-  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
-  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
-  d2[0] = s.inx ? sxx * T(g.inv_lsx2) : T(0);
-  d2[1] = s.iny ? syy * T(g.inv_lsy2) : T(0);
-  if (s.inxy) {
+  d1[0] = (INTERIOR || s.inx) ? sx1 : T(0);
+  d1[1] = (INTERIOR || s.iny) ? sy1 : T(0);
+  d2[0] = (INTERIOR || s.inx) ? sxx * K.ilx2 : T(0);
+  d2[1] = (INTERIOR || s.iny) ? syy * K.ily2 : T(0);
+  if (INTERIOR || s.inxy) {
     const T dm2 = T(q.v[a][sl[0]][cy - 2]), dm1 = T(q.v[a][sl[1]][cy - 1]);
     const T dp1 = T(q.v[a][sl[3]][cy + 1]), dp2 = T(q.v[a][sl[4]][cy + 2]);
     const T sdd = c43 * ((dm1 - c) + (dp1 - c)) - c12 * ((dm2 - c) + (dp2 - c));
-    d2[2] = sdd * T(g.inv_2gv) - sxx * T(g.inv_2lsx2) - syy * T(g.inv_2lsy2);
+    d2[2] = sdd * K.i2gv - sxx * K.i2lx2 - syy * K.i2ly2;
   } else {
     d2[2] = T(0);
   }
 }
 
-template <typename T, typename R, int NF>
-__device__ __forceinline__ void ring_d1(const Ring<R, NF>& q, int a, const int sl[5], int cy, const Geom& g,
+template <bool INTERIOR, typename T, typename R, int NFR>
+__device__ __forceinline__ void ring_d1(const Ring<R, NFR>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                         const SiteInfo& s, T& f, T d1[2]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23);
   const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
@@ -108,8 +124,10 @@ __device__ __forceinline__ void ring_d1(const Ring<R, NF>& q, int a, const int s
   const T ym2 = T(q.v[a][sl[2]][cy - 2]), ym1 = T(q.v[a][sl[2]][cy - 1]);
   const T yp1 = T(q.v[a][sl[2]][cy + 1]), yp2 = T(q.v[a][sl[2]][cy + 2]);
   f = T(q.v[a][sl[2]][cy]);
-  d1[0] = s.inx ? (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * T(g.inv_lsx) : T(0);
-  d1[1] = s.iny ? (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * T(g.inv_lsy) : T(0);
+  const T sx1 = (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * K.ilx;
+  const T sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * K.ily;
+  d1[0] = (INTERIOR || s.inx) ? sx1 : T(0);
+  d1[1] = (INTERIOR || s.iny) ? sy1 : T(0);
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
@@ -120,9 +138,9 @@ struct StreamPrefetch {
   R f[RPW + 1][NF], v[RPW + 1][NF], l[RPW + 1][NF];
 };
 
-template <typename R, int NF>
-__device__ __forceinline__ void group_site(int k, int rb, int y0, int& r, int& c, int& x, int& y) {
-  using S = StreamShape<R, NF>;
+template <typename R, int NFR>
+__device__ __forceinline__ void group_site(int k, int& r, int& c) {
+  using S = StreamShape<R, NFR>;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (k < S::RPW) {
     r = w * S::RPW + k;
@@ -132,32 +150,45 @@ __device__ __forceinline__ void group_site(int k, int rb, int y0, int& r, int& c
     const int qd = threadIdx.x & 3;
     c = qd < 2 ? qd : TY + qd;
   }
-  x = rb + r;
-  y = y0 - 2 + c;
 }
 
-// Issue the loads of the group of rows starting at rb (values of sites outside
-// the lattice are 0).  PASS selects the arrays: 0/1 F; 2 F, V; 3 F, V, l1; 4 F, V, l2.
-template <int PASS, typename R, int NF>
-__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NF>::RPW>& p, const PassBufs& b,
+// Issue the loads of the group of rows starting at rb.  Element offsets are
+// 32-bit (the host keeps nf*X*Y below 2^31); a site outside the lattice loads
+// element 0 and is replaced by 0, so no load sits behind a branch.  PASS
+// selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2.
+template <int PASS, typename R, int NF, int NFR>
+__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
-  using S = StreamShape<R, NF>;
+  using S = StreamShape<R, NFR>;
   const R* __restrict__ F = (const R*)b.F;
   const R* __restrict__ V = (const R*)b.V;
-  const R* __restrict__ L = (const R*)(PASS == 3 ? b.l1 : b.l2);
+  const R* __restrict__ L = (const R*)(PASS == 4 ? b.l2 : b.l1);
+  const unsigned NP = (unsigned)g.N;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
-    int r, c, x, y;
-    group_site<R, NF>(k, rb, y0, r, c, x, y);
+    int r, c;
+    group_site<R, NFR>(k, r, c);
+    const int x = rb + r, y = y0 - 2 + c;
     const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
     const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
-    const long long off = (long long)x * g.Y + y;
+    const unsigned off = in ? (unsigned)(x * g.Y + y) : 0u;
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
-      const long long o = off + (long long)a * g.N;
-      p.f[k][a] = in ? F[o] : R(0);
-      p.v[k][a] = (PASS >= 2 && in && !kill) ? V[o] : R(0);
-      p.l[k][a] = (PASS >= 3 && in) ? L[o] : R(0);
+      const unsigned o = in ? off + (unsigned)a * NP : 0u;
+      const R fv = F[o];
+      p.f[k][a] = in ? fv : R(0);
+      if (PASS >= 2) {
+        const R vv = V[o];
+        p.v[k][a] = (in && !kill) ? vv : R(0);
+      } else {
+        p.v[k][a] = R(0);
+      }
+      if (PASS >= 3) {
+        const R lv = L[o];
+        p.l[k][a] = in ? lv : R(0);
+      } else {
+        p.l[k][a] = R(0);
+      }
     }
   }
 }
@@ -171,34 +202,141 @@ __device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
   return fo + dt * (v + half * l);
 }
 
-// Write a group into the ring; slot0 is the ring slot of its first row.
-template <int PASS, typename R, int NF>
-__device__ __forceinline__ void group_store(Ring<R, NF>& q, const StreamPrefetch<R, NF, StreamShape<R, NF>::RPW>& p,
+// Write a group into the ring; slot0 is the ring slot of its first row.  PASS
+// 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to NF..2NF-1.
+template <int PASS, typename R, int NF, int NFR>
+__device__ __forceinline__ void group_store(Ring<R, NFR>& q, const StreamPrefetch<R, NF, StreamShape<R, NFR>::RPW>& p,
                                             int slot0, R dt) {
-  using S = StreamShape<R, NF>;
+  using S = StreamShape<R, NFR>;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
     if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
-    int r, c, x, y;
-    group_site<R, NF>(k, 0, 0, r, c, x, y);
+    int r, c;
+    group_site<R, NFR>(k, r, c);
     const int slot = (slot0 + r) & (S::NR - 1);
 #pragma unroll
-    for (int a = 0; a < NF; ++a) q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+    for (int a = 0; a < NF; ++a) {
+      if (PASS == 23) {
+        q.v[a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+        q.v[NF + a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+      } else {
+        q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+      }
+    }
   }
 }
 
-// One strip of one pass (PASS 0..4 as in stage_tile).  Partials: one per strip.
-template <class TH, typename R, int PASS>
-__device__ __forceinline__ void stream_strip(Ring<R, TH::NF>& q, double* red, const PassBufs& b,
-                                             const typename TH::TP& tp, const Geom& g, int xs, int sx, int y0,
-                                             bool kill, int t) {
-  using S = StreamShape<R, TH::NF>;
-  constexpr int NF = TH::NF, RB = S::RB, NR = S::NR, RPW = S::RPW;
+// The RPW sites of this thread in one iteration (rows c0 + w*RPW + k), from
+// the stage field in ring components AOFF .. AOFF+NF-1.
+template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF>
+__device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBufs& b, const typename TH::TP& tp,
+                                            const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
+                                            int c0, int y, int cy, bool kill, double& acc0, double& acc1) {
+  using S = StreamShape<R, NFR>;
+  constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned NP = (unsigned)g.N;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < RPW; ++k) {
+    const int x = c0 + w * RPW + k;
+    if (INTERIOR || (x < g.X && y < g.Y)) {
+      int sl[5];
+      const int base = x - xs;  // slot of row x-2
+#pragma unroll
+      for (int d = 0; d < 5; ++d) sl[d] = (base + d) & (NR - 1);
+      SiteInfo s;
+      if (INTERIOR) {
+        s.x = x;
+        s.y = y;
+        s.inx = s.iny = s.inxy = true;
+        s.band = false;
+      } else {
+        s = site_info(g, x, y);
+      }
+      R f[NF], d1[NF][2], d2[NF][3], eg[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) ring_derivs<INTERIOR, R, R, NFR>(q, AOFF + a, sl, cy, K, s, f[a], d1[a], d2[a]);
+      TH::template variation<R>(tp, g, s, f, d1, d2, eg);
+      const unsigned off = (unsigned)(x * g.Y + y);
+      if (PASS == 0) {
+        R* out = (R*)b.lout;
+#pragma unroll
+        for (int a = 0; a < NF; ++a) out[off + (unsigned)a * NP] = eg[a];
+      } else if (PASS <= 3) {
+        R* out = (R*)b.lout;
+#pragma unroll
+        for (int a = 0; a < NF; ++a) out[off + (unsigned)a * NP] = -dt * eg[a];
+      }
+      if (PASS == 1) {
+        double fd[NF], d1d[NF][2];
+#pragma unroll
+        for (int a = 0; a < NF; ++a) ring_d1<INTERIOR, double, R, NFR>(q, AOFF + a, sl, cy, Kd, s, fd[a], d1d[a]);
+        acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+      }
+      if (PASS == 4) {
+        // F' = Temp4 + dt (l1 - 2 l2 + l3)/6 with Temp4 the ring centre.
+        const R* __restrict__ Vg = (const R*)b.V;
+        const R* __restrict__ l1 = (const R*)b.l1;
+        const R* __restrict__ l2 = (const R*)b.l2;
+        const R* __restrict__ l3 = (const R*)b.l3;
+        R Fn[NF], Vn[NF];
+        const R sixth = R(1.0 / 6.0);
+        const R dt6 = dt * sixth;
+#pragma unroll
+        for (int a = 0; a < NF; ++a) {
+          const unsigned o = off + (unsigned)a * NP;
+          const R vv = Vg[o];
+          const R Vc = kill ? R(0) : vv;
+          const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
+          const R a4 = -dt * eg[a];
+          Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
+          Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
+          const double e = (double)eg[a];
+          acc0 = fmax(acc0, e);
+          acc1 = fmax(acc1, fabs(e));
+        }
+        if (TH::kUnitConstraint && g.unit_mag) {
+          TH::template normalize<R>(tp, Fn);
+          TH::template project<R>(tp, Vn, Fn);
+        }
+        R* Fo = (R*)b.Fout;
+        R* Vo = (R*)b.Vout;
+#pragma unroll
+        for (int a = 0; a < NF; ++a) {
+          Fo[off + (unsigned)a * NP] = Fn[a];
+          Vo[off + (unsigned)a * NP] = Vn[a];
+        }
+      }
+    }
+  }
+}
+
+// One strip of one pass (PASS 0..4 as in stage_tile; 23 = passes 2 and 3 in
+// one sweep: both stage fields depend only on F, V and l1, so they are built
+// from the same loads into two halves of the ring and give l2 -> lout and
+// l3 -> lout2).  Partials: one per strip.
+template <class TH, typename R, int PASS>
+struct StreamRing {
+  static constexpr int NFR = PASS == 23 ? 2 * TH::NF : TH::NF;
+  using type = Ring<R, NFR>;
+};
+
+template <class TH, typename R, int PASS>
+__device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::type& q, double* red,
+                                             const PassBufs& b, const typename TH::TP& tp, const Geom& g, int xs,
+                                             int sx, int y0, bool kill, int t) {
+  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS>::NFR;
+  using S = StreamShape<R, NFR>;
+  constexpr int RB = S::RB, NR = S::NR, RPW = S::RPW;
+  const R dt = R(g.dt);
+  const int lane = threadIdx.x & 63;
   const int y = y0 + lane;
   const int cy = lane + 2;
   const int nit = sx / RB;
+  const StencilK<R> K(g);
+  const StencilK<double> Kd(g);
+  const unsigned NP = (unsigned)g.N;
 
   // Prologue: rows xs-2 .. xs+RB+1 straight into ring slots 0 .. RB+3.  Row
   // xs - 2 + j lives in slot j mod NR throughout.
@@ -206,98 +344,53 @@ __device__ __forceinline__ void stream_strip(Ring<R, TH::NF>& q, double* red, co
     const int r = idx / S::RY, c = idx - r * S::RY;
     const int gx = xs - 2 + r, gy = y0 - 2 + c;
     const bool in = gx >= 0 && gx < g.X && gy >= 0 && gy < g.Y;
-    const long long off = (long long)gx * g.Y + gy;
+    const unsigned off = in ? (unsigned)(gx * g.Y + gy) : 0u;
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
-      const long long o = off + (long long)a * g.N;
-      R val = R(0);
-      if (in) {
-        const R fo = ((const R*)b.F)[o];
-        const R v = (PASS >= 2 && !kill) ? ((const R*)b.V)[o] : R(0);
-        const R l = PASS == 3 ? ((const R*)b.l1)[o] : PASS == 4 ? ((const R*)b.l2)[o] : R(0);
-        val = stage_value<PASS, R>(fo, v, l, dt);
+      const unsigned o = in ? off + (unsigned)a * NP : 0u;
+      const R fo = ((const R*)b.F)[o];
+      const R vr = (PASS >= 2) ? ((const R*)b.V)[o] : R(0);
+      const R v = kill ? R(0) : vr;
+      const R l = (PASS == 3 || PASS == 23) ? ((const R*)b.l1)[o] : PASS == 4 ? ((const R*)b.l2)[o] : R(0);
+      if (PASS == 23) {
+        q.v[a][r][c] = in ? stage_value<2, R>(fo, v, l, dt) : R(0);
+        q.v[NF + a][r][c] = in ? stage_value<3, R>(fo, v, l, dt) : R(0);
+      } else {
+        q.v[a][r][c] = in ? stage_value<PASS, R>(fo, v, l, dt) : R(0);
       }
-      q.v[a][r][c] = val;
     }
   }
   StreamPrefetch<R, NF, RPW> pf;
-  if (nit > 1) group_load<PASS, R, NF>(pf, b, g, xs + RB + 2, y0, kill);
+  if (nit > 1) group_load<PASS, R, NF, NFR>(pf, b, g, xs + RB + 2, y0, kill);
 
+  PassBufs b2 = b, b3 = b;
+  b3.lout = b.lout2;
   double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
   double acc1 = 0.0;
   for (int it = 0; it < nit; ++it) {
     __syncthreads();  // ring holds rows xs + it*RB - 2 .. xs + it*RB + RB + 1
     const int c0 = xs + it * RB;
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-      const int x = c0 + w * RPW + k;
-      if (x < g.X && y < g.Y) {
-        int sl[5];
-        const int base = x - xs;  // slot of row x-2
-#pragma unroll
-        for (int d = 0; d < 5; ++d) sl[d] = (base + d) & (NR - 1);
-        const SiteInfo s = site_info(g, x, y);
-        R f[NF], d1[NF][2], d2[NF][3], eg[NF];
-#pragma unroll
-        for (int a = 0; a < NF; ++a) ring_derivs<R, R, NF>(q, a, sl, cy, g, s, f[a], d1[a], d2[a]);
-        TH::template variation<R>(tp, g, s, f, d1, d2, eg);
-        const long long off = (long long)x * g.Y + y;
-        if (PASS == 0) {
-          R* out = (R*)b.lout;
-#pragma unroll
-          for (int a = 0; a < NF; ++a) out[off + a * g.N] = eg[a];
-        } else if (PASS <= 3) {
-          R* out = (R*)b.lout;
-#pragma unroll
-          for (int a = 0; a < NF; ++a) out[off + a * g.N] = -dt * eg[a];
-        }
-        if (PASS == 1) {
-          double fd[NF], d1d[NF][2];
-#pragma unroll
-          for (int a = 0; a < NF; ++a) ring_d1<double, R, NF>(q, a, sl, cy, g, s, fd[a], d1d[a]);
-          acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
-        }
-        if (PASS == 4) {
-          // F' = Temp4 + dt (l1 - 2 l2 + l3)/6 with Temp4 the ring centre.
-          const R* __restrict__ Vg = (const R*)b.V;
-          const R* __restrict__ l1 = (const R*)b.l1;
-          const R* __restrict__ l2 = (const R*)b.l2;
-          const R* __restrict__ l3 = (const R*)b.l3;
-          R Fn[NF], Vn[NF];
-          const R sixth = R(1.0 / 6.0);
-          const R dt6 = dt * sixth;
-#pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            const long long o = off + a * g.N;
-            const R Vc = kill ? R(0) : Vg[o];
-            const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
-            const R a4 = -dt * eg[a];
-            Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-            Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
-            const double e = (double)eg[a];
-            acc0 = fmax(acc0, e);
-            acc1 = fmax(acc1, fabs(e));
-          }
-          if (TH::kUnitConstraint && g.unit_mag) {
-            TH::template normalize<R>(tp, Fn);
-            TH::template project<R>(tp, Vn, Fn);
-          }
-          R* Fo = (R*)b.Fout;
-          R* Vo = (R*)b.Vout;
-#pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            Fo[off + a * g.N] = Fn[a];
-            Vo[off + a * g.N] = Vn[a];
-          }
-        }
+    // every mask true for all rows and columns of this iteration
+    const bool interior = (c0 > g.halo) && (c0 + RB - 1 < g.X - g.halo) && (y0 > g.halo) && (y0 + TY - 1 < g.Y - g.halo);
+    if (PASS == 23) {
+      if (interior) {
+        stream_rows<TH, R, 2, true, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+        stream_rows<TH, R, 3, true, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+      } else {
+        stream_rows<TH, R, 2, false, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+        stream_rows<TH, R, 3, false, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
       }
+    } else if (interior) {
+      stream_rows<TH, R, PASS, true, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+    } else {
+      stream_rows<TH, R, PASS, false, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
     }
     __syncthreads();  // every read of this iteration's rows is done
     if (it + 1 < nit) {
       // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
       // no row of iteration it+1 other than themselves maps to
-      group_store<PASS, R, NF>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
-      if (it + 2 < nit) group_load<PASS, R, NF>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
+      group_store<PASS, R, NF, NFR>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
+      if (it + 2 < nit) group_load<PASS, R, NF, NFR>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
     }
   }
   if (PASS == 1) {
@@ -313,7 +406,7 @@ __device__ __forceinline__ void stream_strip(Ring<R, TH::NF>& q, double* red, co
 
 template <class TH, typename R, int PASS>
 __global__ void __launch_bounds__(kThreads) stream_pass(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
-  __shared__ Ring<R, TH::NF> q;
+  __shared__ typename StreamRing<TH, R, PASS>::type q;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, L.nblocks, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier

@@ -393,9 +393,10 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     assert np.abs(Fg - Fo).max() <= 1e-9
 
 
-@pytest.mark.parametrize("variants", [[1, 1, 1, 1], [2, 2, 2, 2], [0, 2, 1, 2], [3, 3, 3, 3], [4, 4, 4, 4],
-                                      [5, 5, 5, 5], [0, 3, 4, 5]])
-def test_tile_shape_variants_match_oracle(variants):
+@pytest.mark.parametrize("variants,fused", [([1, 1, 1, 1], False), ([2, 2, 2, 2], False), ([0, 2, 1, 2], False),
+                                            ([3, 3, 3, 3], False), ([4, 4, 4, 4], False), ([5, 5, 5, 5], False),
+                                            ([0, 3, 4, 5], False), ([0, 3, 0, 3], True), ([2, 5, 0, 4], True)])
+def test_tile_shape_variants_match_oracle(variants, fused):
     """Every launch-shape variant of the fused passes (tiles and streaming strips,
     lattice 70 x 130: partial strips and partial column blocks) gives the oracle's
     trajectory."""
@@ -409,8 +410,9 @@ def test_tile_shape_variants_match_oracle(variants):
     Es, _ = ocore.relax(o, Fo, Vo, p_i, p_f, 10, prev_energy=E0)
     sim = _sim(th, p, F)
     sim.set_tile_variants(variants)
+    sim.set_fused_stages(fused)
     info = sim.tile_variants()
-    assert info["variants"] == variants
+    assert info["variants"] == variants and info["fused23"] == fused
     E, _, _ = sim.steps(sim.compute_energy(), 10, history=True)
     gE = np.array(sim.last_history[0])
     dF = np.abs(sim.get_field() - Fo).max()
@@ -465,10 +467,22 @@ def test_streaming_form_matches_oracle_for_wide_theories(name):
     Es, _ = ocore.relax(o, Fo, Vo, p_i, p_f, 6, prev_energy=ocore.theory_energy(o, Fo, p_i, p_f))
     sim = _sim(th, p, F)
     sim.set_tile_variants([3, 4, 5, 3])
+    if name.startswith("Baby"):
+        sim.set_fused_stages(True)  # 3 fields: both FP64 stage fields fit one ring
     sim.steps(sim.compute_energy(), 6, history=True)
     gE = np.array(sim.last_history[0])
     dF = np.abs(sim.get_field() - Fo).max()
     print(f"[stream {name}] relE={H.rel(gE[-1], Es[-1]):.2e} max|dF|={dF:.2e}")
     assert np.all(np.abs(gE - np.array(Es)) <= 1e-10 * np.abs(np.array(Es)))
     assert dF <= 1e-9
+    sim.close()
+
+
+def test_fused_stages_rejected_when_two_rings_do_not_fit():
+    th = ssa.load_theory("Spin triplet superconducting ferromagnet")
+    p = H.st_params(th, 40, 40)
+    p_i, p_f = H.packed(th, p)
+    sim = _sim(th, p, H.st_field(th, p_i, p_f))
+    with pytest.raises(ValueError, match="cannot share one sweep"):
+        sim.set_fused_stages(True)
     sim.close()
