@@ -628,10 +628,11 @@ struct TheoryLaunchers {
   // quartered (at least 4); 3..5 strips of 64, 128, 256 rows (streaming form).
   static constexpr int kNumShapes = 6;
   static int strip_rows(int v) { return 64 << (v - 3); }
-  // The streaming form indexes with 32-bit element offsets; larger fields run
-  // the default tile shape instead.
+  // The streaming form addresses each plane with a 32-bit byte offset and the
+  // field with 32-bit element offsets; larger fields run the default tile shape.
   static int effective_shape(int v, const Geom& g) {
-    return (v >= 3 && (long long)TH::NF * g.N >= (1LL << 31)) ? 0 : v;
+    const bool ok = (long long)TH::NF * g.N < (1LL << 31) && g.N * (long long)sizeof(double) < (1LL << 32);
+    return (v >= 3 && !ok) ? 0 : v;
   }
 
   // Partials (tiles or strips) of one pass launch in shape variant ``v``.

@@ -84,6 +84,23 @@ struct StencilK {
         i2lx2(T(g.inv_2lsx2)), i2ly2(T(g.inv_2lsy2)) {}
 };
 
+// Element ``a*N + off`` of a field array as (plane base, 32-bit byte offset):
+// the base is uniform (scalar registers) and the offset a 32-bit VGPR, the
+// form a global load with a scalar base takes.  The host keeps every plane
+// below 4 GiB and nf*N below 2^31 for this form.
+template <typename R>
+__device__ __forceinline__ const char* plane(const void* base, int a, unsigned N) {
+  return (const char*)base + (size_t)a * N * sizeof(R);
+}
+template <typename R>
+__device__ __forceinline__ R ld(const char* plane_base, unsigned boff) {
+  return *(const R*)(plane_base + boff);
+}
+template <typename R>
+__device__ __forceinline__ void st(void* base, int a, unsigned N, unsigned boff, R v) {
+  *(R*)((char*)base + (size_t)a * N * sizeof(R) + boff) = v;
+}
+
 // Stencils of component a at ring column cy; sl[0..4] are the ring slots of
 // rows x-2 .. x+2.  Same formulas and masks as window_derivs; INTERIOR (every
 // mask true) drops the selects.
@@ -131,8 +148,12 @@ __device__ __forceinline__ void ring_d1(const Ring<R, NFR>& q, int a, const int 
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
-// interior sites (rows w*RPW + k, column lane) and, for threads < 4*RB, one
-// halo-column site (row tid/4, column 0, 1, 66 or 67 of the ring).
+// interior sites (column lane) and, for threads < 4*RB, one halo-column site
+// (row tid/4, column 0, 1, 66 or 67 of the ring).  A group starts 2 rows after
+// the first row computed in the next iteration, so the interior row of
+// (wave w, k) is (w*RPW + k - 2) mod RB: each thread loads the sites it
+// computes next iteration, except the first two compute rows (w*RPW + k < 2),
+// whose loaders hold them one iteration longer.
 template <typename R, int NF, int RPW>
 struct StreamPrefetch {
   R f[RPW + 1][NF], v[RPW + 1][NF], l[RPW + 1][NF];
@@ -143,7 +164,7 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
   using S = StreamShape<R, NFR>;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (k < S::RPW) {
-    r = w * S::RPW + k;
+    r = (w * S::RPW + k + S::RB - 2) & (S::RB - 1);
     c = lane + 2;
   } else {
     r = threadIdx.x >> 2;
@@ -160,9 +181,7 @@ template <int PASS, typename R, int NF, int NFR>
 __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
   using S = StreamShape<R, NFR>;
-  const R* __restrict__ F = (const R*)b.F;
-  const R* __restrict__ V = (const R*)b.V;
-  const R* __restrict__ L = (const R*)(PASS == 4 ? b.l2 : b.l1);
+  const void* L = PASS == 4 ? b.l2 : b.l1;
   const unsigned NP = (unsigned)g.N;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
@@ -171,20 +190,19 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
     const int x = rb + r, y = y0 - 2 + c;
     const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
     const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
-    const unsigned off = in ? (unsigned)(x * g.Y + y) : 0u;
+    const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
-      const unsigned o = in ? off + (unsigned)a * NP : 0u;
-      const R fv = F[o];
+      const R fv = ld<R>(plane<R>(b.F, a, NP), boff);
       p.f[k][a] = in ? fv : R(0);
       if (PASS >= 2) {
-        const R vv = V[o];
+        const R vv = ld<R>(plane<R>(b.V, a, NP), boff);
         p.v[k][a] = (in && !kill) ? vv : R(0);
       } else {
         p.v[k][a] = R(0);
       }
       if (PASS >= 3) {
-        const R lv = L[o];
+        const R lv = ld<R>(plane<R>(L, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
       } else {
         p.l[k][a] = R(0);
@@ -228,10 +246,12 @@ __device__ __forceinline__ void group_store(Ring<R, NFR>& q, const StreamPrefetc
 
 // The RPW sites of this thread in one iteration (rows c0 + w*RPW + k), from
 // the stage field in ring components AOFF .. AOFF+NF-1.
+// Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied).
 template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF>
 __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
-                                            int c0, int y, int cy, bool kill, double& acc0, double& acc1) {
+                                            int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
+                                            double& acc1) {
   using S = StreamShape<R, NFR>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
@@ -258,15 +278,13 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBuf
 #pragma unroll
       for (int a = 0; a < NF; ++a) ring_derivs<INTERIOR, R, R, NFR>(q, AOFF + a, sl, cy, K, s, f[a], d1[a], d2[a]);
       TH::template variation<R>(tp, g, s, f, d1, d2, eg);
-      const unsigned off = (unsigned)(x * g.Y + y);
+      const unsigned boff = (unsigned)(x * g.Y + y) * (unsigned)sizeof(R);
       if (PASS == 0) {
-        R* out = (R*)b.lout;
 #pragma unroll
-        for (int a = 0; a < NF; ++a) out[off + (unsigned)a * NP] = eg[a];
+        for (int a = 0; a < NF; ++a) st<R>(b.lout, a, NP, boff, eg[a]);
       } else if (PASS <= 3) {
-        R* out = (R*)b.lout;
 #pragma unroll
-        for (int a = 0; a < NF; ++a) out[off + (unsigned)a * NP] = -dt * eg[a];
+        for (int a = 0; a < NF; ++a) st<R>(b.lout, a, NP, boff, -dt * eg[a]);
       }
       if (PASS == 1) {
         double fd[NF], d1d[NF][2];
@@ -276,19 +294,15 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBuf
       }
       if (PASS == 4) {
         // F' = Temp4 + dt (l1 - 2 l2 + l3)/6 with Temp4 the ring centre.
-        const R* __restrict__ Vg = (const R*)b.V;
-        const R* __restrict__ l1 = (const R*)b.l1;
-        const R* __restrict__ l2 = (const R*)b.l2;
-        const R* __restrict__ l3 = (const R*)b.l3;
         R Fn[NF], Vn[NF];
         const R sixth = R(1.0 / 6.0);
         const R dt6 = dt * sixth;
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
-          const unsigned o = off + (unsigned)a * NP;
-          const R vv = Vg[o];
-          const R Vc = kill ? R(0) : vv;
-          const R a1 = l1[o], a2 = l2[o], a3 = l3[o];
+          const R Vc = kV[k * NF + a];
+          const R a1 = ld<R>(plane<R>(b.l1, a, NP), boff);
+          const R a2 = kL[k * NF + a];
+          const R a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
           Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
@@ -300,12 +314,10 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBuf
           TH::template normalize<R>(tp, Fn);
           TH::template project<R>(tp, Vn, Fn);
         }
-        R* Fo = (R*)b.Fout;
-        R* Vo = (R*)b.Vout;
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
-          Fo[off + (unsigned)a * NP] = Fn[a];
-          Vo[off + (unsigned)a * NP] = Vn[a];
+          st<R>(b.Fout, a, NP, boff, Fn[a]);
+          st<R>(b.Vout, a, NP, boff, Vn[a]);
         }
       }
     }
@@ -334,6 +346,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
   const int y = y0 + lane;
   const int cy = lane + 2;
   const int nit = sx / RB;
+  (void)kill;
   const StencilK<R> K(g);
   const StencilK<double> Kd(g);
   const unsigned NP = (unsigned)g.N;
@@ -344,19 +357,42 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     const int r = idx / S::RY, c = idx - r * S::RY;
     const int gx = xs - 2 + r, gy = y0 - 2 + c;
     const bool in = gx >= 0 && gx < g.X && gy >= 0 && gy < g.Y;
-    const unsigned off = in ? (unsigned)(gx * g.Y + gy) : 0u;
+    const unsigned boff = in ? (unsigned)(gx * g.Y + gy) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
-      const unsigned o = in ? off + (unsigned)a * NP : 0u;
-      const R fo = ((const R*)b.F)[o];
-      const R vr = (PASS >= 2) ? ((const R*)b.V)[o] : R(0);
+      const R fo = ld<R>(plane<R>(b.F, a, NP), boff);
+      const R vr = (PASS >= 2) ? ld<R>(plane<R>(b.V, a, NP), boff) : R(0);
       const R v = kill ? R(0) : vr;
-      const R l = (PASS == 3 || PASS == 23) ? ((const R*)b.l1)[o] : PASS == 4 ? ((const R*)b.l2)[o] : R(0);
+      const R l = (PASS == 3 || PASS == 23) ? ld<R>(plane<R>(b.l1, a, NP), boff)
+                : PASS == 4                 ? ld<R>(plane<R>(b.l2, a, NP), boff)
+                                            : R(0);
       if (PASS == 23) {
         q.v[a][r][c] = in ? stage_value<2, R>(fo, v, l, dt) : R(0);
         q.v[NF + a][r][c] = in ? stage_value<3, R>(fo, v, l, dt) : R(0);
       } else {
         q.v[a][r][c] = in ? stage_value<PASS, R>(fo, v, l, dt) : R(0);
+      }
+    }
+  }
+  // Pass 4: V and l2 of the sites this thread computes in the current
+  // iteration (keep) and, for the first two compute rows, in the next (hold).
+  const int w = threadIdx.x >> 6;
+  R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
+  if (PASS == 4) {
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int xk = xs + w * RPW + k, xh = xs + RB + w * RPW + k;
+      const bool ink = xk < g.X && y < g.Y, inh = xh < g.X && y < g.Y && (w * RPW + k < 2);
+      const unsigned bk = ink ? (unsigned)(xk * g.Y + y) * (unsigned)sizeof(R) : 0u;
+      const unsigned bh = inh ? (unsigned)(xh * g.Y + y) * (unsigned)sizeof(R) : 0u;
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        const R v0 = ld<R>(plane<R>(b.V, a, NP), bk), q0 = ld<R>(plane<R>(b.l2, a, NP), bk);
+        const R v1 = ld<R>(plane<R>(b.V, a, NP), bh), q1 = ld<R>(plane<R>(b.l2, a, NP), bh);
+        keepV[k][a] = (ink && !kill) ? v0 : R(0);
+        keepL[k][a] = ink ? q0 : R(0);
+        holdV[k][a] = (inh && !kill) ? v1 : R(0);
+        holdL[k][a] = inh ? q1 : R(0);
       }
     }
   }
@@ -374,22 +410,39 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     const bool interior = (c0 > g.halo) && (c0 + RB - 1 < g.X - g.halo) && (y0 > g.halo) && (y0 + TY - 1 < g.Y - g.halo);
     if (PASS == 23) {
       if (interior) {
-        stream_rows<TH, R, 2, true, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
-        stream_rows<TH, R, 3, true, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+        stream_rows<TH, R, 2, true, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 3, true, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
       } else {
-        stream_rows<TH, R, 2, false, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
-        stream_rows<TH, R, 3, false, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+        stream_rows<TH, R, 2, false, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 3, false, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
       }
     } else if (interior) {
-      stream_rows<TH, R, PASS, true, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+      stream_rows<TH, R, PASS, true, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                             acc1);
     } else {
-      stream_rows<TH, R, PASS, false, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, kill, acc0, acc1);
+      stream_rows<TH, R, PASS, false, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                              acc1);
     }
     __syncthreads();  // every read of this iteration's rows is done
     if (it + 1 < nit) {
       // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
       // no row of iteration it+1 other than themselves maps to
       group_store<PASS, R, NF, NFR>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
+      if (PASS == 4) {
+        // next iteration's sites: the group just stored, or (first two compute
+        // rows) the hold from the group before; the hold takes the new group.
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+          const bool lag = w * RPW + k < 2;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            keepV[k][a] = lag ? holdV[k][a] : pf.v[k][a];
+            keepL[k][a] = lag ? holdL[k][a] : pf.l[k][a];
+            holdV[k][a] = pf.v[k][a];
+            holdL[k][a] = pf.l[k][a];
+          }
+        }
+      }
       if (it + 2 < nit) group_load<PASS, R, NF, NFR>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
     }
   }
