@@ -75,6 +75,152 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
     return sim, E, err, ea, dt
 
 
+def relax_timed(sim, E, tol, max_steps, check_every, floor=False):
+    """Relax until max |EG| < tol (or, with ``floor``, until max |EG| stops
+    improving: three checks without a new minimum).  Returns (E, err_abs, steps, seconds)."""
+    import torch
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, best, stale = 0, np.inf, 0
+    ea = np.inf
+    while done < max_steps:
+        n = min(check_every, max_steps - done)
+        E, _, ea = sim.steps(E, n)
+        done += n
+        if done % (20 * check_every) == 0:
+            print(f"[bench]   {done} steps, E={E:.12f}, max|EG|={ea:.2e}", file=sys.stderr, flush=True)
+        if ea < tol:
+            break
+        if floor:
+            if ea < 0.98 * best:
+                best, stale = ea, 0
+            else:
+                stale += 1
+                if stale >= 3:
+                    break
+    sim.synchronize()
+    return E, ea, done, time.perf_counter() - t0
+
+
+def precision_workload(args):
+    """FP32 form relaxed to its floor + at most 200 FP64 polish steps, against the
+    plain FP64 form relaxed to the same max |EG|, from the same seeded field."""
+    import soliton_solver_amd as ssa
+    n = args.n
+    th, p, cfg = chiral_workload(n)
+    tol = args.tol
+    s64 = ssa.Simulation(p, th, precision="fp64")
+    s64.initialize(cfg)
+    F0 = s64.get_field()
+    s64.autotune_tiles(reps=3)
+    s64.use_graphs(args.graph_steps)
+    E0 = s64.compute_energy()
+    E64, ea64, n64, t64 = relax_timed(s64, E0, tol, args.max_steps, args.check_every)
+    print(f"[bench] fp64 form: {n64} steps, {t64:.2f} s, E={E64:.12f}, max|EG|={ea64:.2e}", file=sys.stderr,
+          flush=True)
+    F64 = s64.get_field()
+    s64.close()
+    s32 = ssa.Simulation(p, th, precision="fp32")
+    s32.initialize({"mode": "array", "field": F0})
+    s32.autotune_tiles(reps=3)
+    s32.use_graphs(args.graph_steps)
+    E32, ea32, n32, t32 = relax_timed(s32, s32.compute_energy(), tol, args.max_steps, args.check_every, floor=True)
+    print(f"[bench] fp32 phase: {n32} steps, {t32:.2f} s, E={E32:.12f}, max|EG|={ea32:.2e}", file=sys.stderr,
+          flush=True)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s32.set_precision("fp64")
+    s32.use_graphs(args.graph_steps)
+    Ep, eap, npol, _ = relax_timed(s32, s32.compute_energy(), tol, 200, 20)
+    tpol = time.perf_counter() - t0
+    print(f"[bench] fp64 polish: {npol} steps, {tpol:.2f} s, E={Ep:.12f}, max|EG|={eap:.2e}", file=sys.stderr,
+          flush=True)
+    Fp = s32.get_field()
+    s32.close()
+    dE = abs(Ep - E64) / abs(E64)
+    dm = float(np.abs(Fp[:3] - F64[:3]).max())
+    out = {
+        "metric": "time-to-converge, FP32 form + FP64 polish vs FP64 form",
+        "value": t64 / (t32 + tpol),
+        "unit": "speedup (FP64 form time / FP32+polish time)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
+                               f"relax to max|EG| < {tol:g}", "lattice": [n, n]},
+        "fp64_form": {"steps": n64, "seconds": t64, "energy": E64, "err_absmax": ea64},
+        "fp32_phase": {"steps": n32, "seconds": t32, "energy": E32, "err_absmax": ea32},
+        "fp64_polish": {"steps": npol, "seconds": tpol, "energy": Ep, "err_absmax": eap},
+        "relE_vs_fp64_form": dE,
+        "max_dm_vs_fp64_form": dm,
+        "target_relE": 1e-7,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batch_workload(args):
+    """Many baby-Skyrme relaxations of uneven sizes in one set of launches per RK4
+    pass: sizes uniform in {64, 128, 256, 512}^2, xsize 20, dt = 0.0957 h, pion
+    mass uniform in [0.1, 1], Skyrme coupling uniform in [0.5, 2], standard
+    potential, Neel hedgehog of charge 1..4 plus U(-1e-3, 1e-3), FP32 fields,
+    FP64 sums, graphs of 20 steps; run until every instance reaches max |EG| < tol."""
+    import torch
+    import soliton_solver_amd as ssa
+    from soliton_solver_amd.batch import BatchSimulation
+    th = ssa.load_theory("Baby Skyrme model")
+    rng = np.random.default_rng(args.seed)
+    ninst = args.instances
+    sizes = rng.choice([64, 128, 256, 512], size=ninst)
+    mpi = rng.uniform(0.1, 1.0, size=ninst)
+    kappa = rng.uniform(0.5, 2.0, size=ninst)
+    charge = rng.integers(1, 5, size=ninst)
+    params = []
+    for n, m, k in zip(sizes, mpi, kappa):
+        h = 20.0 / (int(n) - 1)
+        params.append(th.params.default_params(xlen=int(n), ylen=int(n), xsize=20.0, ysize=20.0, mpi=float(m),
+                                               kappa=float(k), unit_magnetization=True, time_step=0.0957 * h))
+    bs = BatchSimulation(th, params, precision="fp32")
+    bs.initialize([{"mode": "initial", "ansatz": "neel", "skyrmion_number": int(q), "noise": 1e-3, "seed": i}
+                   for i, q in enumerate(charge)])
+    bs.use_graphs(20)
+    sites = np.array([int(n) * int(n) for n in sizes], dtype=np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done, active = 0, ninst
+    while done < args.max_steps and active > 0:
+        chunk = min(args.check_every * 10, args.max_steps - done)
+        active = bs.run(chunk, args.tol, check_every=args.check_every)
+        done += chunk
+        st = bs.state()
+        print(f"[bench]   {done} steps: {active} of {ninst} instances active, "
+              f"max err {float(np.max(st['err_absmax'])):.2e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = bs.state()
+    updates = float((sites * st["steps"]).sum())
+    out = {
+        "metric": "grid-pt updates/s, batched relaxations (sum over instances of sites x steps / wall time)",
+        "value": updates / wall,
+        "unit": "grid-point updates/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{ninst} baby Skyrme relaxations, sizes U{{64,128,256,512}}^2, seed {args.seed}, "
+                               f"until max|EG| < {args.tol:g}", "instances": ninst,
+                   "total_sites": int(sites.sum()), "tiles": bs.num_tiles},
+        "time_to_converge_s": wall,
+        "converged": int((st["converged_step"] >= 0).sum()),
+        "steps_max": int(st["steps"].max()),
+        "steps_mean": float(st["steps"].mean()),
+    }
+    bs.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,13 +229,21 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--graph-steps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=1024, help="side of the CPU-baseline sample lattice")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (~10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape of every pass")
     ap.add_argument("--tiles", default="", help="tile variants of passes 1-4 for the FP32 form, e.g. 1,1,1,2 "
                                                 "(no autotuning)")
     ap.add_argument("--fused23", type=int, default=-1, help="with --tiles: 1 runs passes 2+3 as one sweep")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
+    ap.add_argument("--workload", choices=["throughput", "precision", "batch"], default="throughput",
+                    help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
+                         "batch: many baby-Skyrme relaxations of uneven sizes")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--tol", type=float, default=1e-6, help="precision workload: max |EG| to reach")
+    ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--check-every", type=int, default=500)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -118,6 +272,13 @@ def main():
         print("no GPU visible to HIP", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+
+    if args.workload == "precision":
+        precision_workload(args)
+        return
+    if args.workload == "batch":
+        batch_workload(args)
+        return
 
     n = args.n
     th, p, cfg = chiral_workload(n)
@@ -222,7 +383,9 @@ def main():
         "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>",
+                     "kernel": (f"stream_pass<ChiralMagnet, float, 4> (strips of {tiles32['rows'][3]} rows)"
+                                if tiles32["variants"][3] >= 3 else
+                                f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>"),
                      "alg_bytes_per_launch": alg_pass4, "avg_launch_ms": passes[4],
                      "step_kernels_gbs": step_gbs, "pass_ms": [float(x) for x in passes]},
         "fp64_form": result64,
