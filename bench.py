@@ -241,7 +241,7 @@ def main():
                          "batch: many baby-Skyrme relaxations of uneven sizes")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--tol", type=float, default=1e-6, help="precision workload: max |EG| to reach")
+    ap.add_argument("--tol", type=float, default=1e-3, help="precision workload: max |EG| to reach")
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--check-every", type=int, default=500)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

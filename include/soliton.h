@@ -104,6 +104,20 @@ int ss_num_charges(ss_ctx* ctx);
  * (Simulation.compute_energy_density, core/simulation.py:194-209). */
 int ss_density(ss_ctx* ctx, int kind, double* host_out, size_t n);
 
+/* Auxiliary per-site quantities of the reference's Simulation
+ * (core/simulation.py:211-386; theory kernels compute_norm_higgs*_point,
+ * compute_phase_difference_point, compute_supercurrent_point,
+ * compute_magnetic_field_kernel): |psi|^2 (HIGGS, or HIGGS1/HIGGS2 for two
+ * order parameters), cos of the phase difference, the supercurrent (2 or 3
+ * components) and the magnetic field (1 or 3 components, each the flux
+ * density times gv/(2 pi q) as the reference writes it).  ss_aux_components
+ * is 0 where the theory does not define the kind.  n = xlen*ylen values in
+ * idx_field(0, x, y) order, FP64, no grid-volume factor unless stated. */
+enum { SS_AUX_HIGGS = 0, SS_AUX_HIGGS1 = 1, SS_AUX_HIGGS2 = 2, SS_AUX_PHASE_DIFFERENCE = 3,
+       SS_AUX_SUPERCURRENT = 4, SS_AUX_MAGNETIC_FIELD = 5 };
+int ss_aux_components(ss_ctx* ctx, int kind);
+int ss_aux_density(ss_ctx* ctx, int kind, int comp, double* host_out, size_t n);
+
 /* Energy gradient EG of the current Field (the gradient kernel with
  * Temp = Field, core/integrator.py:15-48), FP64, reference layout. */
 int ss_variation(ss_ctx* ctx, int mode, double* host_eg, size_t n);

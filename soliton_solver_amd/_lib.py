@@ -27,6 +27,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 VAR_REFERENCE = 0
 VAR_EXACT_ADJOINT = 1
+AUX_HIGGS, AUX_HIGGS1, AUX_HIGGS2, AUX_PHASE_DIFFERENCE, AUX_SUPERCURRENT, AUX_MAGNETIC_FIELD = range(6)
 BUF_FIELD = 0
 BUF_VELOCITY = 1
 
@@ -62,6 +63,8 @@ SIGNATURES = {
     "ss_get_tile_variants": (C.c_int, [C.c_void_p, _i32p, _i32p]),
     "ss_autotune_tiles": (C.c_int, [C.c_void_p, C.c_int, _i32p, _f64p]),
     "ss_set_fused_stages": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_aux_components": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_aux_density": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
     "ss_get_fused_stages": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "ss_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "ss_last_error": (C.c_char_p, [C.c_void_p]),

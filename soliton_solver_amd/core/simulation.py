@@ -252,14 +252,88 @@ class Simulation:
                 obs[key] = f(self)
         return obs
 
+    # -- per-site densities (core/simulation.py:194-386) ---------------------
+    # The reference fills its shared ``en`` buffer; here each method also
+    # returns the values, shape (xlen, ylen), and keeps them in ``self.en``.
     def compute_energy_density(self) -> np.ndarray:
-        """Per-site energy times gv, shape (xlen, ylen) (core/simulation.py:194-209)."""
+        """Per-site energy times gv (core/simulation.py:194-209)."""
         return self.density(-1)
 
     def density(self, kind: int) -> np.ndarray:
+        """Per-site density times gv: kind -1 energy, >= 0 the theory's charge kinds."""
         out = np.empty(int(self.rp.dim_grid))
         self._check(_lib.lib().ss_density(self._ctx, int(kind), out, out.size))
+        self.en = out.reshape(int(self.rp.xlen), int(self.rp.ylen))
+        return self.en
+
+    def aux_components(self, kind: int) -> int:
+        return int(_lib.lib().ss_aux_components(self._ctx, int(kind)))
+
+    def aux_density(self, kind: int, comp: int = 0) -> np.ndarray:
+        """Auxiliary per-site quantity ``kind`` (``_lib.AUX_*``), component ``comp``."""
+        out = np.empty(int(self.rp.dim_grid))
+        self._check(_lib.lib().ss_aux_density(self._ctx, int(kind), int(comp), out, out.size))
         return out.reshape(int(self.rp.xlen), int(self.rp.ylen))
+
+    def _aux_field(self, kind: int, what: str) -> np.ndarray:
+        n = self.aux_components(kind)
+        if n == 0:
+            raise AttributeError(f"theory {self.theory.THEORY_SPEC.name!r} defines no {what}")
+        return np.stack([self.aux_density(kind, c) for c in range(n)])
+
+    def _charge_density(self, key: str) -> np.ndarray:
+        kinds = getattr(self.theory.kernels, "CHARGE_KINDS", {})
+        if key not in kinds:
+            raise AttributeError(f"theory {self.theory.THEORY_SPEC.name!r} defines no {key.replace('_', ' ')}")
+        return self.density(kinds[key])
+
+    def compute_higgs_density(self) -> np.ndarray:
+        """|psi|^2 per site (core/simulation.py:211-226)."""
+        self.en = self._aux_field(_lib.AUX_HIGGS, "Higgs norm")[0]
+        return self.en
+
+    def compute_higgs1_density(self) -> np.ndarray:
+        """|psi_1|^2 per site (core/simulation.py:228-243)."""
+        self.en = self._aux_field(_lib.AUX_HIGGS1, "first Higgs norm")[0]
+        return self.en
+
+    def compute_higgs2_density(self) -> np.ndarray:
+        """|psi_2|^2 per site (core/simulation.py:245-260)."""
+        self.en = self._aux_field(_lib.AUX_HIGGS2, "second Higgs norm")[0]
+        return self.en
+
+    def compute_phase_difference_density(self) -> np.ndarray:
+        """cos(arg psi_1 - arg psi_2) per site (core/simulation.py:262-277)."""
+        self.en = self._aux_field(_lib.AUX_PHASE_DIFFERENCE, "phase difference")[0]
+        return self.en
+
+    def compute_magnetic_flux_density(self, which: int) -> np.ndarray:
+        """Flux density ``which`` times gv/(2 pi q) per site (core/simulation.py:279-299)."""
+        kinds = getattr(self.theory.kernels, "CHARGE_KINDS", {})
+        key = {0: "vortex_number_0", 1: "vortex_number_1"}.get(int(which), "vortex_number")
+        return self._charge_density(key if key in kinds else "vortex_number")
+
+    def magnetic_field(self) -> np.ndarray:
+        """Magnetic flux density field, shape (components, xlen, ylen) (core/simulation.py:301-317)."""
+        self.MagneticFluxDensity = self._aux_field(_lib.AUX_MAGNETIC_FIELD, "magnetic field")
+        return self.MagneticFluxDensity
+
+    def compute_magnetic_charge_density(self) -> np.ndarray:
+        """core/simulation.py:319-334."""
+        return self._charge_density("magnetic_charge")
+
+    def compute_electric_charge_density(self) -> np.ndarray:
+        """core/simulation.py:336-351."""
+        return self._charge_density("electric_charge")
+
+    def compute_noether_charge_density(self) -> np.ndarray:
+        """core/simulation.py:353-368."""
+        return self._charge_density("noether_charge")
+
+    def supercurrent(self) -> np.ndarray:
+        """Supercurrent field, shape (components, xlen, ylen) (core/simulation.py:370-386)."""
+        self.Supercurrent = self._aux_field(_lib.AUX_SUPERCURRENT, "supercurrent")
+        return self.Supercurrent
 
     def energy_gradient(self, mode: str = "reference") -> np.ndarray:
         """EG of the current field (the theory's gradient kernel with Temp = Field)."""

@@ -59,6 +59,15 @@ __device__ __forceinline__ SiteInfo site_info(const Geom& g, int x, int y) {
 
 // Tile shape: TY consecutive y (one wavefront wide) by TX rows of x, plus a
 // 2-site halo on each side for the 5-point stencils.
+// Auxiliary per-site quantities of the reference's Simulation
+// (core/simulation.py:211-386): |psi|^2 of the order parameter(s), the cosine
+// of the phase difference, the supercurrent and the magnetic field.  A theory
+// reports how many components it has of each (0: not defined for it).
+enum AuxKind { kAuxHiggs = 0, kAuxHiggs1 = 1, kAuxHiggs2 = 2, kAuxPhaseDifference = 3, kAuxSupercurrent = 4,
+               kAuxMagneticField = 5, kNumAuxKinds = 6 };
+// density_pass kind codes >= kAuxBase select aux kind (code - kAuxBase) / 8, component (code - kAuxBase) % 8.
+constexpr int kAuxBase = 1000;
+
 constexpr int TY = 64;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;

@@ -279,6 +279,10 @@ density_pass(const void* Fv, double* dens, double* part, int kind, typename TH::
       if (kind < 0) {
         tile_d1<double, R, NF>(tile, lx, ly, g, s, fd, d1d);
         v = TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+      } else if (kind >= kAuxBase) {
+        double d2d[NF][3];
+        tile_derivs<double, R, NF>(tile, lx, ly, g, s, fd, d1d, d2d);
+        v = TH::template aux<double>((kind - kAuxBase) >> 3, (kind - kAuxBase) & 7, tp, g, s, fd, d1d, d2d);
       } else {
         double d2d[NF][3];
         tile_derivs<double, R, NF>(tile, lx, ly, g, s, fd, d1d, d2d);
@@ -767,6 +771,8 @@ struct TheoryLaunchers {
     return hipGetLastError();
   }
 
+  static int aux_components(int kind) { return TH::aux_components(kind); }
+
   static int make_tp(const int32_t* pi, int ni, const double* pf, int nfl, void* tp, char* msg, int cap) {
     static_assert(sizeof(TP) <= 512, "theory parameter block too large");
     return TH::make(pi, ni, pf, nfl, (TP*)tp, msg, cap);
@@ -835,6 +841,7 @@ struct TheoryOps {
   int (*shape_rows)(int, int);
   int num_shapes;
   int (*can_fuse23)(int, const Geom&);
+  int (*aux_components)(int);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
@@ -855,6 +862,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::shape_rows,                 \
                                       ss::TheoryLaunchers<TH>::kNumShapes,                  \
                                       &ss::TheoryLaunchers<TH>::can_fuse23,                 \
+                                      &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \

@@ -954,6 +954,27 @@ int ss_density(ss_ctx* c, int kind, double* host_out, size_t n) {
   return 0;
 }
 
+int ss_aux_components(ss_ctx* c, int kind) {
+  if (!c || kind < 0 || kind >= kNumAuxKinds) return 0;
+  return c->ops->aux_components(kind);
+}
+
+int ss_aux_density(ss_ctx* c, int kind, int comp, double* host_out, size_t n) {
+  if (!c) return fail(nullptr, "null context");
+  if (n != (size_t)c->g.N) return fail(c, "aux density needs xlen*ylen = %lld values (got %zu)", c->g.N, n);
+  if (!host_out) return fail(c, "null host pointer");
+  if (kind < 0 || kind >= kNumAuxKinds) return fail(c, "bad aux kind %d", kind);
+  const int nc = c->ops->aux_components(kind);
+  if (comp < 0 || comp >= nc)
+    return fail(c, "theory %s has %d components of aux kind %d (asked %d)", c->ops->name, nc, kind, comp);
+  if (!c->dens) HIPCHK(c, hipMalloc(&c->dens, sizeof(double) * (size_t)c->g.N));
+  HIPCHK(c, c->ops->density(c->prec, kAuxBase + 8 * kind + comp, c->F[c->cur], c->dens, c->part, c->tp, c->g,
+                            c->stream));
+  HIPCHK(c, hipMemcpyAsync(host_out, c->dens, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 static int variation_into(ss_ctx* c, const void* field, void* out, int mode, hipStream_t st) {
   if (mode == SS_VAR_EXACT_ADJOINT) {
     if (ensure_exact(c)) return -1;

@@ -149,6 +149,24 @@ struct AnisotropicSC {
     return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.Q));
   }
 
+  // |Delta_s|^2, |Delta_d|^2, cos(arg Delta_s - arg Delta_d), the 2-component
+  // supercurrent, and the magnetic field B3 times gv/(2 pi q)
+  // (compute_norm_higgs1/2_point, compute_phase_difference_point,
+  // compute_supercurrent_point, compute_magnetic_field_kernel).
+  __host__ __device__ static constexpr int aux_components(int kind) {
+    return (kind == kAuxHiggs1 || kind == kAuxHiggs2 || kind == kAuxPhaseDifference || kind == kAuxMagneticField) ? 1
+         : kind == kAuxSupercurrent ? 2 : 0;
+  }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    if (kind == kAuxHiggs1) return f[2] * f[2] + f[3] * f[3];
+    if (kind == kAuxHiggs2) return f[4] * f[4] + f[5] * f[5];
+    if (kind == kAuxPhaseDifference) return cos(atan2(f[3], f[2]) - atan2(f[5], f[4]));
+    if (kind == kAuxMagneticField) return charge<T>(0, tp, g, s, f, d1, d2) * T(g.gv);
+    return comp == 0 ? d2[1][2] - d2[0][1] : d2[0][2] - d2[1][0];
+  }
+
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {}
   template <typename T>

@@ -85,6 +85,14 @@ struct BoseEinsteinCondensate {
     return f[0] * f[0] + f[1] * f[1];
   }
 
+  // |psi|^2 (compute_norm_higgs_point, kernels.py).
+  __host__ __device__ static constexpr int aux_components(int kind) { return kind == kAuxHiggs ? 1 : 0; }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    return f[0] * f[0] + f[1] * f[1];
+  }
+
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {}
   template <typename T>

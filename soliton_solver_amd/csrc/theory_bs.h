@@ -164,6 +164,14 @@ struct BabySkyrme {
     return (f[0] * cx0 + f[1] * cx1 + f[2] * cx2) * T(1.0 / (4.0 * M_PI));
   }
 
+  // No auxiliary densities.
+  __host__ __device__ static constexpr int aux_components(int kind) { return 0; }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    return T(0);
+  }
+
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
     project(tp, eg, f);

@@ -140,6 +140,14 @@ struct ChiralMagnet {
     return -(d1[0][0] + d1[1][1]);
   }
 
+  // No auxiliary densities.
+  __host__ __device__ static constexpr int aux_components(int kind) { return 0; }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    return T(0);
+  }
+
   // Exact-adjoint flow convention: psi is a saddle field, flowed along
   // -coup_mu dE/dpsi as the reference's variation does (kernels.py:503); the
   // magnetisation gradient is projected onto the tangent space.

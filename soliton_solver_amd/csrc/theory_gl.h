@@ -83,6 +83,18 @@ struct GinzburgLandau {
     return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.q));
   }
 
+  // |psi|^2 (compute_norm_higgs_point) and the 2-component supercurrent
+  // (compute_supercurrent_point): (d_yx A2 - d_yy A1, d_xy A1 - d_xx A2).
+  __host__ __device__ static constexpr int aux_components(int kind) {
+    return kind == kAuxHiggs ? 1 : kind == kAuxSupercurrent ? 2 : 0;
+  }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    if (kind == kAuxHiggs) return f[2] * f[2] + f[3] * f[3];
+    return comp == 0 ? d2[1][2] - d2[0][1] : d2[0][2] - d2[1][0];
+  }
+
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {}
   template <typename T>

@@ -173,6 +173,14 @@ struct LiquidCrystal {
     return c;
   }
 
+  // No auxiliary densities.
+  __host__ __device__ static constexpr int aux_components(int kind) { return 0; }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    return T(0);
+  }
+
   // The energy is linear in Phi, so the exact-adjoint flow has no stationary
   // point in Phi when depol is on; the director gradient is projected.
   template <typename T>

@@ -134,6 +134,28 @@ struct SpinTripletSFM {
     return b / (T(2.0 * M_PI) * T(tp.q));
   }
 
+  // |psi1|^2, |psi2|^2 (compute_norm_higgs1/2_point), the 3-component supercurrent
+  // (d_yx A2 - d_yy A1, d_xy A1 - d_xx A2, -(d_xx + d_yy) A3) and the magnetic
+  // field: component a is the flux density ``which = a`` times gv/(2 pi q)
+  // (compute_supercurrent_point, compute_magnetic_field_kernel).
+  __host__ __device__ static constexpr int aux_components(int kind) {
+    return (kind == kAuxHiggs1 || kind == kAuxHiggs2) ? 1 : (kind == kAuxSupercurrent || kind == kAuxMagneticField) ? 3 : 0;
+  }
+  template <typename T>
+  __device__ __forceinline__ static T aux(int kind, int comp, const TP& tp, const Geom& g, const SiteInfo& s,
+                                          const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
+    if (kind == kAuxHiggs1) return f[6] * f[6] + f[7] * f[7];
+    if (kind == kAuxHiggs2) return f[8] * f[8] + f[9] * f[9];
+    if (kind == kAuxMagneticField) {
+      // flux kinds: 1 which=2, 2 which=0, 3 which=1
+      const int fk = comp == 2 ? 1 : comp + 2;
+      return charge<T>(fk, tp, g, s, f, d1, d2) * T(g.gv);
+    }
+    if (comp == 0) return d2[4][2] - d2[3][1];
+    if (comp == 1) return d2[3][2] - d2[4][0];
+    return -(d2[5][0] + d2[5][1]);
+  }
+
   template <typename T>
   __device__ __forceinline__ static void flow_fix(const TP& tp, const T f[NF], T eg[NF]) {
     project(tp, eg, f);

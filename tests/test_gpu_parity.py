@@ -486,3 +486,65 @@ def test_fused_stages_rejected_when_two_rings_do_not_fit():
     with pytest.raises(ValueError, match="cannot share one sweep"):
         sim.set_fused_stages(True)
     sim.close()
+
+
+_AUX_CASES = {
+    "Ginzburg-Landau superconductor": (lambda th: H.gl_params(th, 50, 42), lambda th, pi, pf: H.gl_field(th, pi, pf, seed=2),
+                                       {"higgs": (2, 3), "current": (0, 1, None)}),
+    "Anyon superconductor": (lambda th: H.mcsh_params(th, 50, 42), lambda th, pi, pf: H.mcsh_field(th, pi, pf, seed=2),
+                             {"higgs": (2, 3), "current": (0, 1, None)}),
+    "Bose-Einstein condensate": (lambda th: H.bec_params(th, 50, 42), lambda th, pi, pf: H.bec_field(th, pi, pf, seed=2),
+                                 {"higgs": (0, 1)}),
+    "Anisotropic superconductor": (lambda th: H.as_params(th, 50, 42, seed=2),
+                                   lambda th, pi, pf: H.as_field(th, pi, pf, seed=2, amp=5e-2),
+                                   {"higgs1": (2, 3), "higgs2": (4, 5), "phase": True, "current": (0, 1, None),
+                                    "field": [2]}),
+    "Ferromagnetic superconductor": (lambda th: H.fs_params(th, 50, 42), lambda th, pi, pf: H.fs_field(th, pi, pf, seed=2),
+                                     {"higgs": (6, 7), "current": (3, 4, 5), "field": [0, 1, 2]}),
+    "Spin triplet superconducting ferromagnet": (lambda th: H.st_params(th, 50, 42),
+                                                 lambda th, pi, pf: H.st_field(th, pi, pf, seed=2),
+                                                 {"higgs1": (6, 7), "higgs2": (8, 9), "current": (3, 4, 5),
+                                                  "field": [0, 1, 2]}),
+}
+
+
+@pytest.mark.parametrize("name", list(_AUX_CASES))
+def test_aux_densities_match_oracle(name):
+    """Higgs norms, phase difference, supercurrent and magnetic field (the
+    reference Simulation's auxiliary densities) against the oracle."""
+    from oracle import aux as oaux
+    mkp, mkf, spec = _AUX_CASES[name]
+    th = ssa.load_theory(name)
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    o = H.ORACLES[name]
+    sim = _sim(th, p, F)
+    D1 = ocore.first_derivatives(F, p_i, p_f)
+    D2 = ocore.second_derivatives(F, p_i, p_f)
+    tol = lambda ref: 1e-12 * max(1.0, float(np.abs(ref).max()))
+    if "higgs" in spec:
+        ref = oaux.higgs(F, *spec["higgs"])
+        assert np.abs(sim.compute_higgs_density() - ref).max() <= tol(ref)
+    for key, meth in (("higgs1", sim.compute_higgs1_density), ("higgs2", sim.compute_higgs2_density)):
+        if key in spec:
+            ref = oaux.higgs(F, *spec[key])
+            assert np.abs(meth() - ref).max() <= tol(ref)
+    if "phase" in spec:
+        ref = oaux.phase_difference(F)
+        assert np.abs(sim.compute_phase_difference_density() - ref).max() <= 1e-12
+    if "current" in spec:
+        ref = oaux.supercurrent(D2, *spec["current"])
+        got = sim.supercurrent()
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-11 * max(1.0, float(np.abs(ref).max()))
+    else:
+        with pytest.raises(AttributeError):
+            sim.supercurrent()
+    if "field" in spec:
+        ref = np.stack([o.vortex_density(F, D1, p_i, p_f, w) if o.vortex_density.__code__.co_argcount > 4
+                        else o.vortex_density(F, D1, p_i, p_f) for w in spec["field"]])
+        got = sim.magnetic_field()
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, float(np.abs(ref).max()))
+    sim.close()

@@ -58,7 +58,11 @@ def test_chiral_320_relaxes_below_1e_8():
     sim.initialize({"mode": "skyrmions", "ansatz": "anti", "skyrmions": [(5.0, 5.0, 1.0)], "noise": 1e-6,
                     "seed": 1})
     E0 = sim.compute_energy()
-    E, ea, n = sim.minimize(tol=1e-8, max_steps=200000, check_every=2000)
+    sim.use_graphs(20)
+    # the demagnetising potential is a saddle field: the energy rises while it
+    # relaxes, so the flow's velocity is killed nearly every step and the last
+    # decades take ~2e5 steps (measured: 6.9e-9 after 2.1e5)
+    E, ea, n = sim.minimize(tol=1e-8, max_steps=400000, check_every=5000)
     Q = sim.observables()["skyrmion_number"]
     print(f"[chiral 320] E0={E0:.10f} E={E:.12f} Q={Q:.10f} steps={n} max|EG|={ea:.2e}")
     assert ea < 1e-8

@@ -654,3 +654,30 @@ def test_liquid_crystal_phi_equation_is_poisson_with_bound_charge():
     print("liquid crystal div P vs rho (h, h/2):", errs)
     assert errs[1] < 1e-4
     assert errs[0] / errs[1] > 8.0
+
+
+def test_aux_oracle_supercurrent_is_curl_of_flux_and_phase_difference_is_normalised_dot():
+    """Supercurrent (d_y B3, -d_x B3, -lap A3) with B3 = d_x A2 - d_y A1, at 4th order
+    in the interior; cos of the phase difference = psi1.psi2/(|psi1||psi2|)."""
+    from oracle import aux as oaux
+    th = ssa.load_theory("Ferromagnetic superconductor")
+    errs = []
+    for n in (61, 121):
+        p_i, p_f, F, _ = _fs_smooth(th, n)
+        D1 = ocore.first_derivatives(F, p_i, p_f)
+        D2 = ocore.second_derivatives(F, p_i, p_f)
+        J = oaux.supercurrent(D2, 3, 4, 5)
+        B = D1[4, 0] - D1[3, 1]
+        Bd = ocore.first_derivatives(np.stack([B] * 8), p_i, p_f)[0]
+        ref = np.stack([Bd[1], -Bd[0]])
+        c = slice(8, -8)
+        errs.append(np.abs(J[:2, c, c] - ref[:, c, c]).max() / np.abs(ref[:, c, c]).max())
+        lap = D2[5, 0] + D2[5, 1]
+        assert np.array_equal(J[2], -lap)
+    print("supercurrent vs curl B (h, h/2):", errs)
+    assert errs[1] < 1e-4 and errs[0] / errs[1] > 8.0
+    rng = np.random.default_rng(0)
+    G = rng.normal(size=(6, 9, 9))
+    cosd = oaux.phase_difference(G)
+    dot = (G[2] * G[4] + G[3] * G[5]) / np.sqrt((G[2] ** 2 + G[3] ** 2) * (G[4] ** 2 + G[5] ** 2))
+    assert np.abs(cosd - dot).max() < 1e-12
