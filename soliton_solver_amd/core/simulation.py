@@ -382,6 +382,54 @@ class Simulation:
         self._check(_lib.lib().ss_get_fused_stages(self._ctx, C.byref(on)))
         return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value)}
 
+    # -- output bundle and resume (core/simulation.py:411-510, core/io.py) ---
+    def save_output(self, output_dir: str | None = None, precision: int = 32) -> str:
+        """Write the theory's output bundle (Field.dat, lattice files, densities)
+        in the reference's text format; returns the directory."""
+        import inspect
+        from pathlib import Path
+        if output_dir is None:
+            output_dir = str(Path(self.theory.io.__file__).resolve().parent / "results")
+        Path(output_dir).mkdir(parents=True, exist_ok=True)
+        kinds = getattr(self.theory.kernels, "CHARGE_KINDS", {})
+
+        def dens(key):
+            return self.density(kinds[key]).ravel() if key in kinds else None
+
+        def aux(kind):
+            n = self.aux_components(kind)
+            return np.stack([self.aux_density(kind, c) for c in range(n)]).ravel() if n else None
+
+        kwargs = dict(output_dir=output_dir, h_Field=self.get_field().ravel(),
+                      h_EnergyDensity=self.compute_energy_density().ravel(),
+                      h_MagneticFluxDensity=aux(_lib.AUX_MAGNETIC_FIELD),
+                      h_BaryonDensity=dens("skyrmion_number"),
+                      h_VortexDensity=dens("vortex_number"),
+                      h_Supercurrent=aux(_lib.AUX_SUPERCURRENT),
+                      h_HiggsDensity=aux(_lib.AUX_HIGGS),
+                      h_MagneticChargeDensity=dens("magnetic_charge"),
+                      h_ElectricChargeDensity=dens("electric_charge"),
+                      h_NoetherChargeDensity=dens("noether_charge"),
+                      h_Higgs1Density=aux(_lib.AUX_HIGGS1),
+                      h_Higgs2Density=aux(_lib.AUX_HIGGS2),
+                      h_grid=self.create_grid().ravel(),
+                      xlen=int(self.p_i_h[0]), ylen=int(self.p_i_h[1]),
+                      number_coordinates=int(self.p_i_h[3]), number_total_fields=int(self.p_i_h[4]),
+                      precision=precision)
+        accepted = set(inspect.signature(self.theory.io.output_data_bundle).parameters)
+        self.theory.io.output_data_bundle(**{k: v for k, v in kwargs.items() if k in accepted})
+        return output_dir
+
+    def load_output(self, output_dir: str, velocity_zero: bool = True) -> None:
+        """Resume from a bundle's Field.dat (velocity set to zero)."""
+        import os
+        from .io import read_field_dat
+        F = read_field_dat(os.path.join(output_dir, "Field.dat"), int(self.p_i_h[0]), int(self.p_i_h[1]),
+                           int(self.p_i_h[4]))
+        self.set_field(F)
+        if velocity_zero:
+            self.set_velocity(np.zeros_like(F))
+
     def synchronize(self) -> None:
         self._check(_lib.lib().ss_synchronize(self._ctx))
 

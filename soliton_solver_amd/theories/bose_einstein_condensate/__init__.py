@@ -2,7 +2,7 @@
 from __future__ import annotations
 
 from ..registry import TheorySpec
-from . import initial_config, kernels, observables, params
+from . import initial_config, io, kernels, observables, params
 
 THEORY_SPEC = TheorySpec(
     name="Bose-Einstein condensate",

@@ -3,7 +3,7 @@ soliton_solver/theories/ginzburg_landau_superconductor/__init__.py)."""
 from __future__ import annotations
 
 from ..registry import TheorySpec
-from . import initial_config, kernels, observables, params
+from . import initial_config, io, kernels, observables, params
 
 THEORY_SPEC = TheorySpec(
     name="Ginzburg-Landau superconductor",

@@ -3,7 +3,7 @@ soliton_solver/theories/spin_triplet_superconducting_magnet/__init__.py)."""
 from __future__ import annotations
 
 from ..registry import TheorySpec
-from . import initial_config, kernels, observables, params
+from . import initial_config, io, kernels, observables, params
 
 THEORY_SPEC = TheorySpec(
     name="Spin triplet superconducting ferromagnet",

@@ -1,10 +1,12 @@
-"""HIP path vs the FP64 oracle on identical seeded inputs (chiral magnet, Ginzburg-Landau).
+"""HIP path vs the FP64 oracle on identical seeded inputs, every theory.
 
 Tolerances (FP64 form): per-stage energy gradient within 1e-11 of max|EG|;
 energies and charges within 1e-12 relative for a given field; after N steps
 energy within 1e-10 relative and max |Delta F| stated (asserted <= 1e-9).
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 import pytest
@@ -548,3 +550,46 @@ def test_aux_densities_match_oracle(name):
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, float(np.abs(ref).max()))
     sim.close()
+
+
+_BUNDLE_CASES = {
+    "Chiral magnet": (lambda th: H.chiral_params(th, 30, 26), lambda th, pi, pf: H.chiral_field(th, pi, pf, seed=1)),
+    "Ginzburg-Landau superconductor": (lambda th: H.gl_params(th, 30, 26), lambda th, pi, pf: H.gl_field(th, pi, pf)),
+    "Baby Skyrme model": (lambda th: H.bs_params(th, 30, 26), lambda th, pi, pf: H.bs_field(th, pi, pf)),
+    "Anisotropic superconductor": (lambda th: H.as_params(th, 30, 26), lambda th, pi, pf: H.as_field(th, pi, pf)),
+    "Bose-Einstein condensate": (lambda th: H.bec_params(th, 30, 26), lambda th, pi, pf: H.bec_field(th, pi, pf)),
+    "Anyon superconductor": (lambda th: H.mcsh_params(th, 30, 26), lambda th, pi, pf: H.mcsh_field(th, pi, pf)),
+    "Ferromagnetic superconductor": (lambda th: H.fs_params(th, 30, 26), lambda th, pi, pf: H.fs_field(th, pi, pf)),
+    "Spin triplet superconducting ferromagnet": (lambda th: H.st_params(th, 30, 26),
+                                                 lambda th, pi, pf: H.st_field(th, pi, pf)),
+    "Liquid crystal": (lambda th: H.lc_params(th, 30, 26, depol=True), lambda th, pi, pf: H.lc_field(th, pi, pf)),
+}
+
+
+@pytest.mark.parametrize("name", list(_BUNDLE_CASES))
+def test_save_output_bundle_and_resume(tmp_path, name):
+    """save_output writes the theory's file set (reference names); the energy
+    density file holds the oracle's per-site energy; load_output restores the
+    field exactly."""
+    from oracle import io_format as oio
+    mkp, mkf = _BUNDLE_CASES[name]
+    th = ssa.load_theory(name)
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    sim = _sim(th, p, F)
+    out = sim.save_output(str(tmp_path), precision=32)
+    names = set(os.listdir(out))
+    expected = {fn for _, _, fn in th.io.BUNDLE_SPEC} | {"Field.dat", "LatticePoints.dat", "LatticeVectors.dat"}
+    assert names == expected
+    o = H.ORACLES[name]
+    e_ref = o.energy_density(F, ocore.first_derivatives(F, p_i, p_f), p_i, p_f)
+    got = np.array(open(os.path.join(out, "EnergyDensity.dat")).read().split(), dtype=np.float64).reshape(e_ref.shape)
+    assert np.abs(got - e_ref).max() <= 1e-12 * max(1.0, np.abs(e_ref).max())
+    assert open(os.path.join(out, "Field.dat")).read() == oio.field_dat_text(F.ravel(), F.shape[1], F.shape[2],
+                                                                              F.shape[0])
+    sim2 = _sim(th, p, np.zeros_like(F))
+    sim2.load_output(out)
+    assert np.array_equal(sim2.get_field(), F)
+    sim.close()
+    sim2.close()
