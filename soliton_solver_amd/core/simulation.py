@@ -64,6 +64,29 @@ class DeviceBuffer:
         return a if dtype is None else a.astype(dtype)
 
 
+def as_simulation(obj) -> "Simulation":
+    """The Simulation behind ``obj``: a Simulation, or its ``Field`` / ``Velocity``
+    buffer (the first argument of the reference's observables functions, e.g.
+    ``compute_energy(Field, d1fd1x, en, entmp, gridsum_partial, p_i_d, p_f_d,
+    p_i_h, p_f_h)`` in chiral_magnet/observables.py:15-57)."""
+    if isinstance(obj, Simulation):
+        return obj
+    if isinstance(obj, DeviceBuffer):
+        return obj._sim
+    raise TypeError(f"expected a Simulation or its Field buffer, got {type(obj).__name__}")
+
+
+def which_argument(args, default: int = 2) -> int:
+    """``which`` of compute_vortex_number: the only extra argument in
+    ``f(sim, which)``, the fifth in the reference's
+    ``f(Field, d1fd1x, en, entmp, gridsum_partial, which, p_i_d, p_f_d, p_i_h)``."""
+    if len(args) == 1:
+        return int(args[0])
+    if len(args) >= 5:
+        return int(args[4])
+    return default
+
+
 class Simulation:
     """Arrested-Newton-flow relaxation of one theory on one GPU."""
 
@@ -88,6 +111,11 @@ class Simulation:
         self.set_params(params)
         self.Field = DeviceBuffer(self, _lib.BUF_FIELD)
         self.Velocity = DeviceBuffer(self, _lib.BUF_VELOCITY)
+        # names of the reference's scratch buffers, for code that passes them
+        # to observables functions; the derivative and reduction scratch live
+        # in the native context
+        self.d1fd1x = self.d2fd2x = self.entmp = self.gridsum_partial = None
+        self.en = None
         self.grid = None
         self.last_history = None
 
@@ -114,6 +142,8 @@ class Simulation:
         p_i, p_f = self.theory.params.pack_device_params(self.rp)
         self.p_i_h = np.ascontiguousarray(p_i, dtype=np.int32)
         self.p_f_h = np.ascontiguousarray(p_f, dtype=np.float64)
+        # the reference's device copies; the native context holds its own
+        self.p_i_d, self.p_f_d = self.p_i_h, self.p_f_h
         L = _lib.lib()
         if self._ctx is None:
             ctx = C.c_void_p()

@@ -1,18 +1,24 @@
 """Maxwell-Chern-Simons-Higgs observables (reference: maxwell_chern_simons_higgs/observables.py:14-190)."""
+from ...core.simulation import as_simulation, which_argument
 from .kernels import CHARGE_KINDS
 
 
-def compute_energy(sim) -> float:
+def compute_energy(sim, *args) -> float:
+    sim = as_simulation(sim)
     return sim.compute_energy()
 
 
-def compute_vortex_number(sim, which: int = 2) -> float:
+def compute_vortex_number(sim, *args, which: int = 2) -> float:
+    which = which_argument(args, which)
+    sim = as_simulation(sim)
     return sim.compute_charge(CHARGE_KINDS["vortex_number"])
 
 
-def compute_electric_charge(sim) -> float:
+def compute_electric_charge(sim, *args) -> float:
+    sim = as_simulation(sim)
     return sim.compute_charge(CHARGE_KINDS["electric_charge"])
 
 
-def compute_noether_charge(sim) -> float:
+def compute_noether_charge(sim, *args) -> float:
+    sim = as_simulation(sim)
     return sim.compute_charge(CHARGE_KINDS["noether_charge"])

@@ -593,3 +593,27 @@ def test_save_output_bundle_and_resume(tmp_path, name):
     assert np.array_equal(sim2.get_field(), F)
     sim.close()
     sim2.close()
+
+
+def test_observables_accept_the_reference_call_convention():
+    """theory.observables functions called as the reference calls them
+    (Field and scratch buffers first, e.g. chiral_magnet/observables.py:15-99,
+    ferromagnetic_superconductor/observables.py compute_vortex_number(..., which, ...))."""
+    th = ssa.load_theory("Ferromagnetic superconductor")
+    p = H.fs_params(th, 40, 36)
+    p_i, p_f = H.packed(th, p)
+    sim = _sim(th, p, H.fs_field(th, p_i, p_f, seed=3))
+    ob = th.observables
+    E = ob.compute_energy(sim.Field, sim.d1fd1x, sim.en, sim.entmp, sim.gridsum_partial, sim.p_i_d, sim.p_f_d,
+                          sim.p_i_h, sim.p_f_h)
+    assert E == sim.compute_energy()
+    Q = ob.compute_skyrmion_number(sim.Field, sim.d1fd1x, sim.en, sim.entmp, sim.gridsum_partial, sim.p_i_d,
+                                   sim.p_f_d, sim.p_i_h)
+    assert Q == ob.compute_skyrmion_number(sim)
+    for which in (0, 1, 2):
+        ref_style = ob.compute_vortex_number(sim.Field, sim.d1fd1x, sim.en, sim.entmp, sim.gridsum_partial, which,
+                                             sim.p_i_d, sim.p_f_d, sim.p_i_h)
+        assert ref_style == ob.compute_vortex_number(sim, which)
+    with pytest.raises(TypeError):
+        ob.compute_energy(np.zeros(3))
+    sim.close()
