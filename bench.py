@@ -59,7 +59,7 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
             sim.set_fused_stages(bool(fused23))
         sim.tiles = sim.tile_variants()
     else:
-        sim.tiles = sim.autotune_tiles(reps=5) if autotune else sim.tile_variants()
+        sim.tiles = sim.autotune_tiles(reps=20) if autotune else sim.tile_variants()
     if graph_steps:
         sim.use_graphs(graph_steps)
     E = sim.compute_energy()

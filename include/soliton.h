@@ -104,6 +104,35 @@ int ss_num_charges(ss_ctx* ctx);
  * (Simulation.compute_energy_density, core/simulation.py:194-209). */
 int ss_density(ss_ctx* ctx, int kind, double* host_out, size_t n);
 
+/* Initial conditions generated on the device (the reference's
+ * create_ground_state_kernel / create_initial_configuration_kernel /
+ * create_skyrmion_kernel / create_vortex_kernel, e.g.
+ * chiral_magnet/initial_config.py:117-386): mode 0 ground state, 1 the
+ * theory's centred ansatz, 2 solitons (x, y, radius) composed by stereographic
+ * addition (unit-vector theories), 3 vortex psi = u tanh(scale r) e^{i N th}
+ * with A = 0 (Ginzburg-Landau).  extra[] carries theory-specific values (BEC:
+ * Thomas-Fermi radius, beta, vortex sign, core size; vortex seeds in
+ * solitons).  Sets Field, zeroes Velocity; fails for a mode the theory does
+ * not provide.  ss_add_noise adds U(-a, a) (or N(0, a^2)) from a
+ * counter-based generator of ``seed`` to components [begin, end) and then
+ * rescales the first renorm_ncomp components to norm renorm_norm (0: none);
+ * its stream is not NumPy's, so fields meant to match a host-seeded oracle
+ * are generated on the host and uploaded. */
+typedef struct ss_init_desc {
+  int mode;
+  int ansatz;              /* 0 Bloch, 1 Neel, 2 anti */
+  double skyrmion_number;
+  double rotation;
+  double scale;
+  double vortex_number;
+  double extra[4];
+  int nsol;
+  const double* solitons;  /* 3 * nsol values */
+} ss_init_desc;
+int ss_init_field(ss_ctx* ctx, const ss_init_desc* desc);
+int ss_add_noise(ss_ctx* ctx, int comp_begin, int comp_end, int gaussian, double amplitude, uint64_t seed,
+                 int renorm_ncomp, double renorm_norm);
+
 /* Auxiliary per-site quantities of the reference's Simulation
  * (core/simulation.py:211-386; theory kernels compute_norm_higgs*_point,
  * compute_phase_difference_point, compute_supercurrent_point,

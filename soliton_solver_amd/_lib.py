@@ -35,6 +35,25 @@ _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _dp = C.POINTER(C.c_double)
 
+INIT_GROUND, INIT_INITIAL, INIT_SKYRMIONS, INIT_VORTEX = range(4)
+
+
+class InitDesc(C.Structure):
+    """``ss_init_desc``: what ss_init_field generates on the device."""
+
+    _fields_ = [
+        ("mode", C.c_int),
+        ("ansatz", C.c_int),
+        ("skyrmion_number", C.c_double),
+        ("rotation", C.c_double),
+        ("scale", C.c_double),
+        ("vortex_number", C.c_double),
+        ("extra", C.c_double * 4),
+        ("nsol", C.c_int),
+        ("solitons", C.POINTER(C.c_double)),
+    ]
+
+
 # name -> (restype, argtypes); mirrors include/soliton.h one to one.
 SIGNATURES = {
     "ss_version": (C.c_int, []),
@@ -63,6 +82,8 @@ SIGNATURES = {
     "ss_get_tile_variants": (C.c_int, [C.c_void_p, _i32p, _i32p]),
     "ss_autotune_tiles": (C.c_int, [C.c_void_p, C.c_int, _i32p, _f64p]),
     "ss_set_fused_stages": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_init_field": (C.c_int, [C.c_void_p, C.POINTER(InitDesc)]),
+    "ss_add_noise": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_double]),
     "ss_aux_components": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_aux_density": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
     "ss_get_fused_stages": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

@@ -196,8 +196,15 @@ class Simulation:
         ``{"mode": "array", "field": F, "velocity": V}`` injects host arrays
         (shape ``(nf, xlen, ylen)`` or flat); every other mode is passed to the
         theory's ``initial_config.initialize``, which fills host arrays in place.
+        With ``"device": True`` the same configuration is generated on the
+        device instead (no host field, no upload); the noise then comes from
+        the device's counter-based generator seeded with ``"seed"``, not from
+        NumPy's, so only noise-free configurations agree with the host ones.
         """
         cfg = dict(init_config or {})
+        if cfg.get("device") and str(cfg.get("mode", "initial")).lower() != "array":
+            self.initialize_on_device(cfg)
+            return
         self.grid = self.create_grid()
         F = np.zeros(self.field_shape)
         V = np.zeros(self.field_shape)
@@ -211,6 +218,38 @@ class Simulation:
                                            p_f_h=self.p_f_h, config=cfg)
         self.set_field(F)
         self.set_velocity(V)
+
+    def initialize_on_device(self, init_config: dict | None = None) -> None:
+        """Generate the theory's initial configuration on the device (ss_init_field).
+
+        The theory's ``initial_config.device_plan`` translates the host
+        configuration into an ``ss_init_desc`` plus a list of noise passes
+        ``(comp_begin, comp_end, gaussian, amplitude, renorm_ncomp, renorm_norm)``.
+        """
+        cfg = dict(init_config or {})
+        plan = getattr(self.initial_config, "device_plan", None)
+        if plan is None:
+            raise NotImplementedError(f"theory {self.theory.THEORY_SPEC.name!r} has no device initial conditions")
+        self.grid = self.create_grid()
+        desc_kw, noise = plan(cfg, self.p_i_h, self.p_f_h)
+        d = _lib.InitDesc()
+        d.mode = int(desc_kw.get("mode", _lib.INIT_INITIAL))
+        d.ansatz = int(desc_kw.get("ansatz", 0))
+        d.skyrmion_number = float(desc_kw.get("skyrmion_number", 1.0))
+        d.rotation = float(desc_kw.get("rotation", 0.0))
+        d.scale = float(desc_kw.get("scale", 1.0))
+        d.vortex_number = float(desc_kw.get("vortex_number", 1.0))
+        for k, v in enumerate(desc_kw.get("extra", ())):
+            d.extra[k] = float(v)
+        sol = np.ascontiguousarray(np.asarray(desc_kw.get("solitons", ()), dtype=np.float64).reshape(-1))
+        d.nsol = sol.size // 3
+        d.solitons = sol.ctypes.data_as(C.POINTER(C.c_double)) if d.nsol else None
+        L = _lib.lib()
+        self._check(L.ss_init_field(self._ctx, C.byref(d)))
+        seed = int(cfg.get("seed", 0))
+        for k, (c0, c1, gauss, amp, rn, rv) in enumerate(noise):
+            self._check(L.ss_add_noise(self._ctx, int(c0), int(c1), int(gauss), float(amp),
+                                       (seed + 0x9E3779B97F4A7C15 * k) & 0xFFFFFFFFFFFFFFFF, int(rn), float(rv)))
 
     # -- the flow -----------------------------------------------------------
     def step(self, prev_energy: float) -> tuple[float, float]:

@@ -29,6 +29,7 @@
 #include "dual.h"
 #include "stencil.h"
 #include "stream.h"
+#include "init.h"
 
 namespace ss {
 
@@ -623,6 +624,19 @@ __global__ void __launch_bounds__(256) batch_fin_err(BatchBufs B, double tol) {
   }
 }
 
+// Initial condition of every site (theory init_site, FP64) stored in precision R.
+template <class TH, typename R>
+__global__ void __launch_bounds__(256) init_pass(void* Fv, InitArgs a, Geom g) {
+  R* F = (R*)Fv;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.N; i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / g.Y), y = (int)(i - (long long)x * g.Y);
+    double f[TH::NF];
+    TH::init_site(a, g, x, y, f);
+#pragma unroll
+    for (int c = 0; c < TH::NF; ++c) F[(long long)c * g.N + i] = R(f[c]);
+  }
+}
+
 // Host-side launchers for one theory.  ``prec`` 0 = FP64 storage, 1 = FP32.
 template <class TH>
 struct TheoryLaunchers {
@@ -773,6 +787,16 @@ struct TheoryLaunchers {
 
   static int aux_components(int kind) { return TH::aux_components(kind); }
 
+  static unsigned init_modes() { return TH::kInitModes; }
+
+  static hipError_t init(int prec, const InitArgs& a, void* F, const Geom& g, hipStream_t st) {
+    const long long nb = (g.N + 255) / 256;
+    const dim3 grid((unsigned)(nb < 8192 ? nb : 8192));
+    if (prec) hipLaunchKernelGGL((init_pass<TH, float>), grid, dim3(256), 0, st, F, a, g);
+    else hipLaunchKernelGGL((init_pass<TH, double>), grid, dim3(256), 0, st, F, a, g);
+    return hipGetLastError();
+  }
+
   static int make_tp(const int32_t* pi, int ni, const double* pf, int nfl, void* tp, char* msg, int cap) {
     static_assert(sizeof(TP) <= 512, "theory parameter block too large");
     return TH::make(pi, ni, pf, nfl, (TP*)tp, msg, cap);
@@ -842,6 +866,8 @@ struct TheoryOps {
   int num_shapes;
   int (*can_fuse23)(int, const Geom&);
   int (*aux_components)(int);
+  unsigned (*init_modes)();
+  hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
@@ -863,6 +889,8 @@ struct TheoryOps {
                                       ss::TheoryLaunchers<TH>::kNumShapes,                  \
                                       &ss::TheoryLaunchers<TH>::can_fuse23,                 \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
+                                      &ss::TheoryLaunchers<TH>::init_modes,                 \
+                                      &ss::TheoryLaunchers<TH>::init,                       \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \

@@ -32,6 +32,42 @@ using namespace ss;
 
 static thread_local std::string g_global_err;
 
+// Counter-based uniform in [0, 1): splitmix64 of (seed, counter), top 53 bits.
+__device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned long long ctr) {
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (ctr + 1ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Components [c0, c1) += U(-amp, amp) or N(0, amp^2) (Box-Muller on two
+// counters), then |first rn components| rescaled to rv where nonzero.
+template <typename R>
+__global__ void __launch_bounds__(256) k_noise(R* F, long long N, int c0, int c1, int gaussian, double amp,
+                                               unsigned long long seed, int rn, double rv) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    for (int c = c0; c < c1; ++c) {
+      const unsigned long long ctr = 2ull * ((unsigned long long)c * (unsigned long long)N + (unsigned long long)i);
+      double d;
+      if (gaussian) {
+        const double u1 = 1.0 - hash_uniform(seed, ctr), u2 = hash_uniform(seed, ctr + 1ull);
+        d = amp * sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+      } else {
+        d = amp * (2.0 * hash_uniform(seed, ctr) - 1.0);
+      }
+      F[(long long)c * N + i] = R((double)F[(long long)c * N + i] + d);
+    }
+    if (rn > 0) {
+      double s2 = 0.0;
+      for (int c = 0; c < rn; ++c) s2 += (double)F[(long long)c * N + i] * (double)F[(long long)c * N + i];
+      const double sn = sqrt(s2);
+      if (sn != 0.0)
+        for (int c = 0; c < rn; ++c) F[(long long)c * N + i] = R((double)F[(long long)c * N + i] * (rv / sn));
+    }
+  }
+}
+
 static const TheoryOps* theory_ops(int id) {
   switch (id) {
     case SS_THEORY_CHIRAL_MAGNET: return ops_chiral_magnet();
@@ -950,6 +986,61 @@ int ss_density(ss_ctx* c, int kind, double* host_out, size_t n) {
   if (!c->dens) HIPCHK(c, hipMalloc(&c->dens, sizeof(double) * (size_t)c->g.N));
   HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], c->dens, c->part, c->tp, c->g, c->stream));
   HIPCHK(c, hipMemcpyAsync(host_out, c->dens, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ss_init_field(ss_ctx* c, const ss_init_desc* d) {
+  if (!c) return fail(nullptr, "null context");
+  if (!d) return fail(c, "null init description");
+  if (d->mode < 0 || d->mode > 3 || !((c->ops->init_modes() >> d->mode) & 1u))
+    return fail(c, "init mode %d is not available on the device for theory %s", d->mode, c->ops->name);
+  if (d->ansatz < 0 || d->ansatz > 2) return fail(c, "bad ansatz %d (0 Bloch, 1 Neel, 2 anti)", d->ansatz);
+  if (d->nsol < 0 || (d->nsol > 0 && !d->solitons)) return fail(c, "bad soliton list");
+  InitArgs a{};
+  a.mode = d->mode;
+  a.ansatz = d->ansatz;
+  a.skN = d->skyrmion_number;
+  a.rot = d->rotation;
+  a.scale = d->scale;
+  a.vortex_number = d->vortex_number;
+  for (int k = 0; k < 4; ++k) a.p[k] = d->extra[k];
+  a.nsol = d->nsol;
+  for (int k = 0; k < c->nfl && k < kMaxPF; ++k) a.pf[k] = c->pf[k];
+  double* sol = nullptr;
+  if (d->nsol > 0) {
+    HIPCHK(c, hipMalloc(&sol, sizeof(double) * 3 * (size_t)d->nsol));
+    const hipError_t e = hipMemcpyAsync(sol, d->solitons, sizeof(double) * 3 * (size_t)d->nsol,
+                                        hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+      hipFree(sol);
+      return hipfail(c, "copy of the soliton list failed: %s", hipGetErrorString(e));
+    }
+  }
+  a.sol = sol;
+  hipError_t e = c->ops->init(c->prec, a, c->F[c->cur], c->g, c->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->V[c->cur], 0, elem_size(c->prec) * (size_t)c->g.nf * (size_t)c->g.N, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (sol) hipFree(sol);
+  if (e != hipSuccess) return hipfail(c, "device initialisation failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ss_add_noise(ss_ctx* c, int comp_begin, int comp_end, int gaussian, double amplitude, uint64_t seed,
+                 int renorm_ncomp, double renorm_norm) {
+  if (!c) return fail(nullptr, "null context");
+  if (comp_begin < 0 || comp_end > c->g.nf || comp_begin > comp_end) return fail(c, "bad component range");
+  if (renorm_ncomp < 0 || renorm_ncomp > c->g.nf) return fail(c, "bad renormalised component count");
+  const long long nb = (c->g.N + 255) / 256;
+  const dim3 grid((unsigned)(nb < 8192 ? nb : 8192));
+  if (c->prec)
+    hipLaunchKernelGGL((k_noise<float>), grid, dim3(256), 0, c->stream, (float*)c->F[c->cur], c->g.N, comp_begin,
+                       comp_end, gaussian, amplitude, (unsigned long long)seed, renorm_ncomp, renorm_norm);
+  else
+    hipLaunchKernelGGL((k_noise<double>), grid, dim3(256), 0, c->stream, (double*)c->F[c->cur], c->g.N, comp_begin,
+                       comp_end, gaussian, amplitude, (unsigned long long)seed, renorm_ncomp, renorm_norm);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -147,6 +148,32 @@ struct AnisotropicSC {
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
                                              const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.Q));
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 3u;
+  // Device initial conditions (anisotropic_superconductor/initial_config.py):
+  // ground A = 0, Delta_s = u1, Delta_d = u2 e^{i theta_diff}; initial vortex
+  // with profile tanh(r), windings N1, N2 of the two order parameters.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    const double u1 = a.pf[17], u2 = a.pf[18], thd = a.pf[19];
+    f[0] = f[1] = f[3] = 0.0;
+    f[2] = u1;
+    f[4] = u2 * cos(thd);
+    f[5] = u2 * sin(thd);
+    if (a.mode == kInitGround) return true;
+    if (a.mode != kInitInitial) return false;
+    const double rmax = a.pf[0] > a.pf[1] ? a.pf[0] : a.pf[1];
+    const double ainf = a.pf[22], N1 = a.pf[23], N2 = a.pf[24];
+    double fs, sth, cth, rr, th;
+    centred_vortex(g, x, y, 1.0, rmax, fs, sth, cth, rr, th);
+    f[0] = ainf * fs * sth / rr;
+    f[1] = -ainf * fs * cth / rr;
+    f[2] = u1 * fs * cos(N1 * th);
+    f[3] = u1 * fs * sin(N1 * th);
+    f[4] = u2 * fs * cos(N2 * th);
+    f[5] = u2 * fs * sin(N2 * th);
+    return true;
   }
 
   // |Delta_s|^2, |Delta_d|^2, cos(arg Delta_s - arg Delta_d), the 2-component

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -83,6 +84,32 @@ struct BoseEinsteinCondensate {
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
                                              const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     return f[0] * f[0] + f[1] * f[1];
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 3u;
+  // Device initial conditions (bose_einstein_condensate/initial_config.py):
+  // Thomas-Fermi amplitude sqrt(max(1/sqrt(beta pi) - r^2/(2 beta), 0)) inside
+  // radius p[0], times tanh(r_v/(sqrt 2 xi)) and the phase sign*atan2 of each
+  // vortex seed in sol (x, y, -); p[1] = beta, p[2] = sign, p[3] = xi.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    if (a.mode != kInitGround && a.mode != kInitInitial) return false;
+    double xc, yc;
+    lattice_centre(g, xc, yc);
+    const double gx = g.lsx * (double)x, gy = g.lsy * (double)y;
+    const double dx0 = gx - xc, dy0 = gy - yc, rtf = sqrt(dx0 * dx0 + dy0 * dy0);
+    const double beta = a.p[1];
+    const double tf = (rtf <= a.p[0]) ? 1.0 / sqrt(beta * M_PI) - 1.0 / (2.0 * beta) * rtf * rtf : 0.0;
+    const double amp = sqrt(fmax(tf, 0.0));
+    double phase = 0.0, core = 1.0;
+    for (int k = 0; k < a.nsol; ++k) {
+      const double dx = gx - a.sol[3 * k], dy = gy - a.sol[3 * k + 1];
+      phase = phase + a.p[2] * atan2(dy, dx);
+      core = core * tanh(sqrt(dx * dx + dy * dy) / (sqrt(2.0) * a.p[3]));
+    }
+    f[0] = amp * core * cos(phase);
+    f[1] = amp * core * sin(phase);
+    return true;
   }
 
   // |psi|^2 (compute_norm_higgs_point, kernels.py).

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -138,6 +139,14 @@ struct ChiralMagnet {
       return (f[0] * cx0 + f[1] * cx1 + f[2] * cx2) * T(1.0 / (4.0 * M_PI));
     }
     return -(d1[0][0] + d1[1][1]);
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 7u;
+  // Device initial conditions (chiral_magnet/initial_config.py): ground,
+  // centred texture, composed solitons; psi = 0.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    return unit_vector_init<NF>(a, g, x, y, f);
   }
 
   // No auxiliary densities.

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -127,6 +128,38 @@ struct FerromagneticSC {
     else if (kind == 3) b = -d1[5][0];
     else b = d1[4][0] - d1[3][1];
     return b / (T(2.0 * M_PI) * T(tp.q));
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 3u;
+  // Device initial conditions (ferromagnetic_superconductor/initial_config.py): ground m = (0, 0, M0), psi = u;
+  // initial: texture of radius max(xsize, ysize)/10 rescaled to |m| = M0, and a
+  // vortex A = ainf fs (-sin th, cos th)/r, psi = u fs e^{i N th} with the
+  // rotated angle and fs = tanh(0.2 r) (1 beyond max(xsize, ysize)).
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    const double M0 = a.pf[17], u1 = a.pf[11], ainf = a.pf[13], skN = a.pf[18];
+    for (int c = 0; c < NF; ++c) f[c] = 0.0;
+    f[2] = M0;
+    f[6] = u1;
+    if (a.mode == kInitGround) return true;
+    if (a.mode != kInitInitial) return false;
+    const double rmax = a.pf[0] > a.pf[1] ? a.pf[0] : a.pf[1];
+    double xc, yc, r, th;
+    lattice_centre(g, xc, yc);
+    polar(g, x, y, xc, yc, a.rot, r, th);
+    double m[3];
+    texture(texture_profile(r, rmax / 10.0), th, skN, a.ansatz, m);
+    rescale3(m, M0);
+    const double fs = (r > rmax) ? 1.0 : tanh(0.2 * r);
+    const double rr = fmax(r, 1e-12);
+    f[0] = m[0];
+    f[1] = m[1];
+    f[2] = m[2];
+    f[3] = -ainf * fs * sin(th) / rr;
+    f[4] = ainf * fs * cos(th) / rr;
+    f[6] = u1 * fs * cos(a.vortex_number * th);
+    f[7] = u1 * fs * sin(a.vortex_number * th);
+    return true;
   }
 
   // |psi|^2 (compute_norm_higgs_point), the 3-component supercurrent

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -81,6 +82,31 @@ struct GinzburgLandau {
   __device__ __forceinline__ static T charge(int kind, const TP& tp, const Geom& g, const SiteInfo& s,
                                              const T f[NF], const T d1[NF][2], const T d2[NF][3]) {
     return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.q));
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 11u;
+  // Device initial conditions (ginzburg_landau_superconductor/initial_config.py):
+  // ground (A = 0, psi = u1); vortex psi = u1 tanh(scale r) e^{i N th}, A = 0;
+  // initial: profile tanh(0.2 r) (1 beyond max(xsize, ysize)), A = ainf fs (sin, -cos)/r.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    const double u1 = a.pf[8], ainf = a.pf[10];
+    f[0] = f[1] = f[3] = 0.0;
+    f[2] = u1;
+    if (a.mode == kInitGround) return true;
+    if (a.mode != kInitVortex && a.mode != kInitInitial) return false;
+    const double rmax = a.pf[0] > a.pf[1] ? a.pf[0] : a.pf[1];
+    double fs, sth, cth, rr, th;
+    if (a.mode == kInitVortex) {
+      centred_vortex(g, x, y, a.scale, 1e300, fs, sth, cth, rr, th);
+    } else {
+      centred_vortex(g, x, y, 0.2, rmax, fs, sth, cth, rr, th);
+      f[0] = ainf * fs * sth / rr;
+      f[1] = -ainf * fs * cth / rr;
+    }
+    f[2] = u1 * fs * cos(a.vortex_number * th);
+    f[3] = u1 * fs * sin(a.vortex_number * th);
+    return true;
   }
 
   // |psi|^2 (compute_norm_higgs_point) and the 2-component supercurrent

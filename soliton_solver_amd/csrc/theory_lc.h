@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -171,6 +172,14 @@ struct LiquidCrystal {
       }
     }
     return c;
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 3u;
+  // Device initial conditions (liquid_crystal/initial_config.py): ground and
+  // the centred texture; Phi = 0.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    return a.mode != kInitSkyrmions && unit_vector_init<NF>(a, g, x, y, f);
   }
 
   // No auxiliary densities.

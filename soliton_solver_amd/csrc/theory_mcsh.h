@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "init.h"
 
 namespace ss {
 
@@ -81,6 +82,26 @@ struct MaxwellChernSimonsHiggs {
     if (kind == 0) return (d1[1][0] - d1[0][1]) / (T(2.0 * M_PI) * T(tp.q));
     if (kind == 1) return -(d2[4][0] + d2[4][1]);
     return T(tp.q) * T(tp.q) * f[4] * (f[2] * f[2] + f[3] * f[3]);
+  }
+
+  // Bit k set: init mode k (ground 0, initial 1, skyrmions 2, vortex 3) is available on the device.
+  static constexpr unsigned kInitModes = 3u;
+  // Device initial conditions (maxwell_chern_simons_higgs/initial_config.py):
+  // ground psi = u1; initial vortex as the Ginzburg-Landau one, A0 = 0.
+  __device__ __forceinline__ static bool init_site(const InitArgs& a, const Geom& g, int x, int y, double f[NF]) {
+    const double u1 = a.pf[9], ainf = a.pf[11];
+    for (int c = 0; c < NF; ++c) f[c] = 0.0;
+    f[2] = u1;
+    if (a.mode == kInitGround) return true;
+    if (a.mode != kInitInitial) return false;
+    const double rmax = a.pf[0] > a.pf[1] ? a.pf[0] : a.pf[1];
+    double fs, sth, cth, rr, th;
+    centred_vortex(g, x, y, 0.2, rmax, fs, sth, cth, rr, th);
+    f[0] = ainf * fs * sth / rr;
+    f[1] = -ainf * fs * cth / rr;
+    f[2] = u1 * fs * cos(a.vortex_number * th);
+    f[3] = u1 * fs * sin(a.vortex_number * th);
+    return true;
   }
 
   // |psi|^2 (compute_norm_higgs_point, psi at fields 2, 3) and the 2-component

@@ -46,3 +46,23 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
             for a in range(3):
                 Field[a] = Field[a] + rng.uniform(-amp, amp, size=Field[a].shape)
             A.unit_normalize(Field, 3)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise."""
+    cfg = cfg or {}
+    mode = _MODE.get(str(cfg.get("mode", "initial")).lower(), 1)
+    mode = mode if mode in (0, 2) else 1
+    sol = [(float(s[0]), float(s[1]), float(s[2])) for s in cfg.get("skyrmions", ())] if mode == 2 else []
+    desc = dict(mode=mode, ansatz=_ANSATZ.get(str(cfg.get("ansatz", "bloch")).lower(), 0),
+                skyrmion_number=float(cfg.get("skyrmion_number", p_f_h[8])),
+                rotation=float(cfg.get("skyrmion_rotation", 0.0)), solitons=sol)
+    amp = float(cfg.get("noise", 0.0))
+    gauss = 1 if str(cfg.get("noise_kind", "uniform")) == "gauss" else 0
+    noise = [(0, 3, gauss, amp, 3, 1.0)] if amp > 0.0 else []
+    return desc, noise

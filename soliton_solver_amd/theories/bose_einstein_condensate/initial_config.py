@@ -58,3 +58,27 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
     if a > 0.0:
         rng = np.random.default_rng(int(cfg.get("seed", 0)))
         Field += rng.uniform(-a, a, size=Field.shape)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise: Thomas-Fermi radius,
+    beta, vortex sign and core size in ``extra``, vortex seeds as the soliton list."""
+    cfg = cfg or {}
+    mode = str(cfg.get("mode", "initial")).lower()
+    beta, tf_radius = float(p_f_h[6]), float(p_f_h[7])
+    nv_signed = float(p_f_h[9]) if mode in ("ground", "uniform", "vacuum", "gs") else float(cfg.get("vortex_number", 1.0))
+    nv = int(abs(nv_signed))
+    sign = 1.0 if nv_signed > 0 else (-1.0 if nv_signed < 0 else 0.0)
+    X, Y = int(p_i_h[0]), int(p_i_h[1])
+    xc, yc = (0.0 + float(p_f_h[2]) * (X - 1)) / 2.0, (0.0 + float(p_f_h[3]) * (Y - 1)) / 2.0
+    spacing = tf_radius / 5.0
+    seeds = [(*_seed_position(n, xc, yc, spacing), 0.0) for n in range(nv)]
+    desc = dict(mode=1, extra=(tf_radius, beta, sign, spacing / 2.0), solitons=seeds)
+    amp = float(cfg.get("noise", 0.0))
+    noise = [(0, 2, 0, amp, 0, 0.0)] if amp > 0.0 else []
+    return desc, noise

@@ -49,3 +49,21 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
     sigma = float(cfg.get("noise", 0.0))
     if sigma > 0.0:
         A.gaussian_perturb(Field, np.random.default_rng(int(cfg.get("seed", 0))), sigma, 3)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise (noise: Gaussian on m, renormalised)."""
+    cfg = cfg or {}
+    mode = _MODE.get(str(cfg.get("mode", "initial")).lower(), 1)
+    mode = mode if mode in (0, 2) else 1
+    sol = [(float(s[0]), float(s[1]), float(s[2])) for s in cfg.get("skyrmions", ())] if mode == 2 else []
+    desc = dict(mode=mode, ansatz=_ANSATZ.get(str(cfg.get("ansatz", "bloch")).lower(), 0),
+                skyrmion_number=float(p_f_h[9]), rotation=float(cfg.get("skyrmion_rotation", 0.0)), solitons=sol)
+    sigma = float(cfg.get("noise", 0.0))
+    noise = [(0, 3, 1, sigma, 3, 1.0)] if sigma > 0.0 else []
+    return desc, noise

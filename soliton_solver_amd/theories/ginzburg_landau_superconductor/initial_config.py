@@ -62,3 +62,19 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
     Field[1] = -ainf * f * np.cos(th) / rr
     Field[2] = u1 * f * np.cos(N * th)
     Field[3] = u1 * f * np.sin(N * th)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise (noise on psi in the vortex mode only)."""
+    cfg = cfg or {}
+    mode = _MODE.get(str(cfg.get("mode", "initial")).lower(), 1)
+    desc = dict(mode=mode if mode in (0, 3) else 1, vortex_number=float(cfg.get("vortex_number", 1.0)),
+                scale=float(cfg.get("scale", 1.0)))
+    amp = float(cfg.get("noise", 0.0))
+    noise = [(2, 4, 0, amp, 0, 0.0)] if (amp > 0.0 and mode == 3) else []
+    return desc, noise

@@ -62,3 +62,19 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
         rng = np.random.default_rng(int(cfg.get("seed", 0)))
         Field += rng.uniform(-amp, amp, size=Field.shape)
         _unit(Field)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise."""
+    cfg = cfg or {}
+    mode = _MODE.get(str(cfg.get("mode", "initial")).lower(), 1)
+    desc = dict(mode=0 if mode == 0 else 1, ansatz=_ANSATZ.get(str(cfg.get("ansatz", "bloch")).lower(), 0),
+                skyrmion_number=float(p_f_h[11]), rotation=float(cfg.get("skyrmion_rotation", 0.0)))
+    amp = float(cfg.get("noise", 0.0))
+    noise = [(0, 4, 0, amp, 3, 1.0)] if amp > 0.0 else []
+    return desc, noise

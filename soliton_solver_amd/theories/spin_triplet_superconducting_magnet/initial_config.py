@@ -78,3 +78,19 @@ def initialize(*, Field, Velocity, grid, p_i_h, p_f_h, config=None):
         rng = np.random.default_rng(int(cfg.get("seed", 0)))
         Field += rng.uniform(-amp, amp, size=Field.shape)
         _rescale_m(Field, M0)
+
+
+# -- device generation ---------------------------------------------------
+_MODE = {"ground": 0, "uniform": 0, "vacuum": 0, "gs": 0, "skyrmions": 2, "vortex": 3}
+_ANSATZ = {"bloch": 0, "neel": 1, "anti": 2}
+
+
+def device_plan(cfg, p_i_h, p_f_h):
+    """The same configuration for ss_init_field / ss_add_noise (m rescaled to M0 after the noise)."""
+    cfg = cfg or {}
+    mode = _MODE.get(str(cfg.get("mode", "initial")).lower(), 1)
+    desc = dict(mode=0 if mode == 0 else 1, ansatz=_ANSATZ.get(str(cfg.get("ansatz", "bloch")).lower(), 0),
+                rotation=float(cfg.get("skyrmion_rotation", 0.0)))
+    amp = float(cfg.get("noise", 0.0))
+    noise = [(0, 10, 0, amp, 3, float(p_f_h[20]))] if amp > 0.0 else []
+    return desc, noise
