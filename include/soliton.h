@@ -169,8 +169,10 @@ int ss_use_graphs(ss_ctx* ctx, int steps_per_graph);
 int ss_synchronize(ss_ctx* ctx);
 
 /* Eager steps with a HIP event around every launch (on the context's
- * stream); avg_ms[0..5] = mean duration of pass 1, energy finalize, pass 2,
- * pass 3, pass 4, error finalize.  State advances like ss_step. */
+ * stream); avg_ms[0..5] = mean duration of pass 1, energy finalize, pass 2
+ * (or 2+3), pass 3, pass 4 (or 4+1), error finalize (with 4+1: and the energy
+ * finalize).  With fused 4+1 only the first step of a call runs pass 1.  State
+ * advances like ss_step. */
 int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
 
 /* Launch shapes of the four fused passes (no counterpart in the reference,
@@ -181,7 +183,7 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * with register prefetch (rows fetched once per strip).  ss_autotune_tiles
  * times every pass in every variant ``reps`` times on the live buffers, keeps
  * the fastest per pass, and reports chosen[0..3] and ms[6*(pass-1)+variant]
- * (ms holds 27 entries, see ss_set_fused_stages);
+ * (ms holds 30 entries, see ss_set_fused_stages);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
  * do not depend on the variants; energies only through FP64 summation order. */
@@ -189,15 +191,24 @@ int ss_set_tile_variants(ss_ctx* ctx, const int* variants);
 int ss_get_tile_variants(ss_ctx* ctx, int* variants, int* rows);
 int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
 
-/* Passes 2 and 3 of a step read the same inputs (F, V, l1), so they can run as
- * one streaming sweep that builds both stage fields from one set of loads and
- * writes l2 and l3 (no counterpart in the reference).  ss_autotune_tiles also
- * times this fused sweep in the streaming variants (ms[24..26]) and switches it
- * on when it beats the best pair of separate passes; ss_set_fused_stages
- * forces it (on uses the pass-2 variant, at least 3) and fails when the two
- * stage fields do not fit one LDS ring for the theory and precision. */
-int ss_set_fused_stages(ss_ctx* ctx, int on);
-int ss_get_fused_stages(ss_ctx* ctx, int* on);
+/* Fused sweeps (no counterpart in the reference).  SS_FUSE_23: passes 2 and
+ * 3 of a step read the same inputs (F, V, l1), so one streaming sweep builds
+ * both stage fields from one set of loads and writes l2 and l3.  SS_FUSE_41:
+ * the field after a step, F' = Temp4 + dt (l1 - 2 l2 + l3)/6 (then the unit
+ * constraint), is pointwise in the inputs of pass 4, so one sweep builds Temp4
+ * and F' from one set of loads, finishes the step (F', V', err partials) and
+ * runs pass 1 of the next step on F' (its l1 and energy); a call then runs
+ * pass 1 only in its first step, and l1 alternates between two buffers.
+ * ss_autotune_tiles times both sweeps in the streaming variants (ms[24..26]
+ * for 2+3, ms[27..29] for 4+1) and switches each on when it beats the best
+ * pair of separate passes; ss_set_fused_stages forces them (a fused sweep uses
+ * the variant of pass 2 or pass 4, at least 3) and fails when the two halves
+ * do not fit one LDS ring for the theory and precision.  Results equal the
+ * separate passes' up to the FP64 summation order of the energy. */
+#define SS_FUSE_23 1
+#define SS_FUSE_41 2
+int ss_set_fused_stages(ss_ctx* ctx, int flags);
+int ss_get_fused_stages(ss_ctx* ctx, int* flags);
 
 /* Device memory held by the context, in bytes. */
 int ss_memory_bytes(ss_ctx* ctx, size_t* bytes);

@@ -28,6 +28,8 @@ PREC_FP32 = 1
 VAR_REFERENCE = 0
 VAR_EXACT_ADJOINT = 1
 AUX_HIGGS, AUX_HIGGS1, AUX_HIGGS2, AUX_PHASE_DIFFERENCE, AUX_SUPERCURRENT, AUX_MAGNETIC_FIELD = range(6)
+FUSE_23 = 1
+FUSE_41 = 2
 BUF_FIELD = 0
 BUF_VELOCITY = 1
 

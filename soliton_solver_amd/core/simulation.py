@@ -422,19 +422,23 @@ class Simulation:
         """Time every fused pass in each shape variant on the live buffers and keep the
         fastest per pass (the relaxation state is unchanged).  Variants 0-2 are LDS
         tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows in the
-        streaming form.  Returns the chosen variants, their rows per tile/strip and the
-        ms table [pass][variant]."""
+        streaming form; the fused sweeps of passes 2+3 and 4+1 are timed in the
+        streaming variants.  Returns the chosen variants, their rows per tile/strip,
+        the fused flags and the ms tables ([pass][variant], fused per variant 3-5)."""
         chosen = np.zeros(4, dtype=np.int32)
-        ms = np.zeros(4 * self.NUM_SHAPES + 3)
+        ms = np.zeros(4 * self.NUM_SHAPES + 6)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
         info["ms"] = ms[:4 * self.NUM_SHAPES].reshape(4, self.NUM_SHAPES)
-        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:]
+        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:4 * self.NUM_SHAPES + 3]
+        info["ms_fused41"] = ms[4 * self.NUM_SHAPES + 3:]
         return info
 
-    def set_fused_stages(self, on: bool) -> None:
-        """Run passes 2 and 3 as one streaming sweep (see ``ss_set_fused_stages``)."""
-        self._check(_lib.lib().ss_set_fused_stages(self._ctx, int(bool(on))))
+    def set_fused_stages(self, fuse23: bool, fuse41: bool = False) -> None:
+        """Run passes 2+3 and/or pass 4 with the next step's pass 1 as single streaming
+        sweeps (see ``ss_set_fused_stages``)."""
+        flags = (_lib.FUSE_23 if fuse23 else 0) | (_lib.FUSE_41 if fuse41 else 0)
+        self._check(_lib.lib().ss_set_fused_stages(self._ctx, flags))
 
     def set_tile_variants(self, variants) -> None:
         """Shape variant (0..5, see ``autotune_tiles``) of passes 1..4."""
@@ -449,7 +453,8 @@ class Simulation:
         self._check(_lib.lib().ss_get_tile_variants(self._ctx, v, rows))
         on = C.c_int()
         self._check(_lib.lib().ss_get_fused_stages(self._ctx, C.byref(on)))
-        return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value)}
+        return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value & _lib.FUSE_23),
+                "fused41": bool(on.value & _lib.FUSE_41)}
 
     # -- output bundle and resume (core/simulation.py:411-510, core/io.py) ---
     def save_output(self, output_dir: str | None = None, precision: int = 32) -> str:

@@ -687,16 +687,29 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 6:
+        if constexpr (StreamShape<R, 2 * TH::NF, StreamRing<TH, R, 41>::RPM>::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 41>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
   }
 
-  // Passes 2 and 3 fused into one streaming sweep (pass id 5) when both
-  // stage fields fit one ring and the field uses 32-bit offsets.
+  // Passes 2 and 3 fused into one streaming sweep (pass id 5), and pass 4
+  // with pass 1 of the next step (pass id 6), when both halves fit one ring
+  // and the field uses 32-bit offsets.
   static int can_fuse23(int prec, const Geom& g) {
     const bool fits = prec ? StreamShape<float, 2 * TH::NF>::kFits : StreamShape<double, 2 * TH::NF>::kFits;
     return fits && effective_shape(3, g) == 3;
+  }
+  static int can_fuse41(int prec, const Geom& g) {
+    constexpr int rf = StreamRing<TH, float, 41>::RPM, rd = StreamRing<TH, double, 41>::RPM;
+    const bool fits = prec ? StreamShape<float, 2 * TH::NF, rf>::kFits : StreamShape<double, 2 * TH::NF, rd>::kFits;
+    return fits && effective_shape(3, g) == 3 && !TH::kL2Normalize;
   }
 
   static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
@@ -721,7 +734,7 @@ struct TheoryLaunchers {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
     v = effective_shape(v, g);
-    if (pass == 5) return v >= 3 ? stream_r<R>(pass, strip_rows(v), b, tp, g, st) : hipErrorInvalidValue;
+    if (pass == 5 || pass == 6) return v >= 3 ? stream_r<R>(pass, strip_rows(v), b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_r<R>(pass, strip_rows(v), b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -865,6 +878,7 @@ struct TheoryOps {
   int (*shape_rows)(int, int);
   int num_shapes;
   int (*can_fuse23)(int, const Geom&);
+  int (*can_fuse41)(int, const Geom&);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -888,6 +902,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::shape_rows,                 \
                                       ss::TheoryLaunchers<TH>::kNumShapes,                  \
                                       &ss::TheoryLaunchers<TH>::can_fuse23,                 \
+                                      &ss::TheoryLaunchers<TH>::can_fuse41,                 \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

@@ -221,6 +221,8 @@ struct ss_ctx {
   int nt0 = 0;             // tiles of the default shape (energy, charge and pass-0 kernels)
   int tv[5] = {0, 0, 0, 0, 0};  // tile-shape variant of each fused pass (index = pass)
   int fuse23 = 0;          // passes 2 and 3 as one streaming sweep (variant tv[2], at least 3)
+  int fuse41 = 0;          // pass 4 and the next step's pass 1 as one sweep (variant tv[4], at least 3)
+  void* l1alt = nullptr;   // with fuse41: l1 of the steps that start from parity 1
   double* part = nullptr;  // 3 x ntiles
   DevState* st = nullptr;
   double* hist = nullptr;  // 2 x hist_cap
@@ -318,6 +320,25 @@ static void free_fields(ss_ctx* c) {
     if (c->l[i]) hipFree(c->l[i]);
     c->l[i] = nullptr;
   }
+  if (c->l1alt) hipFree(c->l1alt);
+  c->l1alt = nullptr;
+}
+
+// The second l1 buffer of the fused 4+1 schedule: a step reads l1 of its
+// starting parity and writes the next step's l1 into the other one.
+static int ensure_l1alt(ss_ctx* c) {
+  if (c->l1alt) return 0;
+  const size_t bytes = (size_t)c->g.nf * (size_t)c->g.N * elem_size(c->prec);
+  HIPCHK(c, hipMalloc(&c->l1alt, bytes));
+  HIPCHK(c, hipMemsetAsync(c->l1alt, 0, bytes, c->stream));
+  return 0;
+}
+
+static void* l1_buf(ss_ctx* c, int parity) { return (c->fuse41 && parity) ? c->l1alt : c->l[0]; }
+
+// The fused 4+1 schedule is active (reference flow only).
+static bool use41(const ss_ctx* c) {
+  return c->fuse41 && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize;
 }
 
 static int alloc_fields(ss_ctx* c) {
@@ -332,6 +353,7 @@ static int alloc_fields(ss_ctx* c) {
     HIPCHK(c, hipMalloc(&c->l[i], bytes));
     HIPCHK(c, hipMemsetAsync(c->l[i], 0, bytes, c->stream));
   }
+  if (c->fuse41 && ensure_l1alt(c)) return -1;
   c->cur = 0;
   return 0;
 }
@@ -368,7 +390,7 @@ static PassBufs bufs_for(ss_ctx* c, int parity) {
   PassBufs b{};
   b.F = c->F[parity];
   b.V = c->V[parity];
-  b.l1 = c->l[0];
+  b.l1 = l1_buf(c, parity);
   b.l2 = c->l[1];
   b.l3 = c->l[2];
   b.Fout = c->F[parity ^ 1];
@@ -381,27 +403,41 @@ static int launch_step_exact(ss_ctx* c, int parity);
 static int launch_step_l2(ss_ctx* c, int parity);
 
 // Launches of one step from buffer parity ``parity``.  With ``ev`` non-null an
-// event is recorded before the first and after every launch.
-static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
+// event is recorded before the first and after every launch (7 events: pass 1,
+// energy finalize, pass 2 or 2+3, pass 3, pass 4 or 4+1, err finalize).
+//
+// Separate schedule: pass 1 (l1, energy of F) -> arrest decision of the
+// previous step -> passes 2, 3 -> pass 4 -> err.  Fused 4+1 schedule: pass 1
+// runs only in the first step of a call (``first``); every step ends with the
+// 4+1 sweep, which also gives l1 and the energy of the new field, so the
+// arrest decision follows it directly.
+static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) {
     if (ev) return fail(c, "pass timing is available for the fused reference flow only");
     return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
+  const bool f41 = use41(c);
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
   double* p1 = c->part + c->ntiles;
+  double* p2 = c->part + 2 * c->ntiles;
   double* histE = c->hist;
   double* histErr = c->hist + c->hist_cap;
   int e = 0;
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  b.lout = c->l[0];
-  b.part0 = p0;
-  HIPCHK(c, c->ops->pass(c->prec, 1, c->tv[1], b, c->tp, c->g, c->stream));
-  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0,
-                     c->ops->ntiles(c->prec, c->tv[1], c->g), c->st, histE, c->hist_cap, c->g.killkinen);
-  HIPCHK(c, hipGetLastError());
-  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  if (!f41 || first) {
+    b.lout = l1_buf(c, parity);
+    b.part0 = p0;
+    HIPCHK(c, c->ops->pass(c->prec, 1, c->tv[1], b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0,
+                       c->ops->ntiles(c->prec, c->tv[1], c->g), c->st, histE, c->hist_cap, c->g.killkinen);
+    HIPCHK(c, hipGetLastError());
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  } else if (ev) {
+    HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  }
   if (c->fuse23) {
     b.lout = c->l[1];
     b.lout2 = c->l[2];
@@ -416,9 +452,26 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev) {
     HIPCHK(c, c->ops->pass(c->prec, 3, c->tv[3], b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   }
-  b.lout = nullptr;
+  b.lout2 = nullptr;
   b.part0 = p0;
   b.part1 = p1;
+  if (f41) {
+    const int v4 = c->tv[4] < 3 ? 3 : c->tv[4];
+    b.lout = l1_buf(c, parity ^ 1);
+    b.part2 = p2;
+    HIPCHK(c, c->ops->pass(c->prec, 6, v4, b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    const int nt = c->ops->ntiles(c->prec, v4, c->g);
+    hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1, nt, c->st,
+                       histErr, c->hist_cap);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p2, nt, c->st, histE,
+                       c->hist_cap, c->g.killkinen);
+    HIPCHK(c, hipGetLastError());
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    return 0;
+  }
+  b.lout = nullptr;
   HIPCHK(c, c->ops->pass(c->prec, 4, c->tv[4], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
@@ -538,12 +591,15 @@ static int launch_step_l2(ss_ctx* c, int parity) {
 }
 
 // After the last step of a call: E(F_final), the last kill decision, and the
-// kill applied to V in memory.
-static int launch_finish(ss_ctx* c) {
-  HIPCHK(c, c->ops->density(c->prec, -1, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
-                     c->hist, c->hist_cap, c->g.killkinen);
-  HIPCHK(c, hipGetLastError());
+// kill applied to V in memory.  In the fused 4+1 schedule the last step has
+// already given E(F_final) and the decision.
+static int launch_finish(ss_ctx* c, int nsteps) {
+  if (!(use41(c) && nsteps > 0)) {
+    HIPCHK(c, c->ops->density(c->prec, -1, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
+                       c->hist, c->hist_cap, c->g.killkinen);
+    HIPCHK(c, hipGetLastError());
+  }
   const long long n = (long long)c->g.nf * c->g.N;
   if (c->prec)
     hipLaunchKernelGGL(k_cond_zero<float>, dim3(1024), dim3(256), 0, c->stream, (float*)c->V[c->cur], n, c->st);
@@ -557,7 +613,7 @@ static int build_graph(ss_ctx* c, int parity) {
   hipGraph_t gr = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int s = 0; s < c->graph_steps && rc == 0; ++s) rc = launch_step(c, parity ^ (s & 1), nullptr);
+  for (int s = 0; s < c->graph_steps && rc == 0; ++s) rc = launch_step(c, parity ^ (s & 1), nullptr, false);
   hipError_t e = hipStreamEndCapture(c->stream, &gr);
   if (rc) {
     if (gr) hipGraphDestroy(gr);
@@ -722,6 +778,8 @@ int ss_set_precision(ss_ctx* c, int prec) {
   free_graphs(c);
   free_fields(c);
   c->prec = prec;
+  if (c->fuse23 && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
+  if (c->fuse41 && !c->ops->can_fuse41(prec, c->g)) c->fuse41 = 0;
   if (alloc_fields(c)) return -1;
   HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->V[0], nV, nb, hipMemcpyDeviceToDevice, c->stream));
@@ -778,7 +836,21 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, c->stream, c->st, prev_energy);
   HIPCHK(c, hipGetLastError());
   int done = 0;
-  if (!ev && c->graph_steps > 0 && nsteps >= c->graph_steps) {
+  if (use41(c) && nsteps > 0) {
+    // the first step carries pass 1 and stays outside the graphs
+    if (launch_step(c, c->cur, ev, true)) return -1;
+    if (ev) {
+      HIPCHK(c, hipEventSynchronize(ev[6]));
+      for (int k = 0; k < 6; ++k) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        acc_ms[k] += ms;
+      }
+    }
+    c->cur ^= 1;
+    done = 1;
+  }
+  if (!ev && c->graph_steps > 0 && nsteps - done >= c->graph_steps) {
     while (nsteps - done >= c->graph_steps) {
       if (!c->graph[c->cur] && build_graph(c, c->cur)) return -1;
       HIPCHK(c, hipGraphLaunch(c->graph[c->cur], c->stream));
@@ -786,7 +858,7 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
     }
   }
   for (; done < nsteps; ++done) {
-    if (launch_step(c, c->cur, ev)) return -1;
+    if (launch_step(c, c->cur, ev, !use41(c))) return -1;
     if (ev) {
       HIPCHK(c, hipEventSynchronize(ev[6]));
       for (int k = 0; k < 6; ++k) {
@@ -797,7 +869,7 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
     }
     c->cur ^= 1;
   }
-  if (launch_finish(c)) return -1;
+  if (launch_finish(c, nsteps)) return -1;
   c->last_nsteps = nsteps;
   return 0;
 }
@@ -916,6 +988,36 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
       }
     }
   }
+  // pass 4 fused with the next step's pass 1 against the best separate pair;
+  // the next step's l1 goes to the l1 buffer of the other parity (scratch)
+  const float sep41 = bestp[4] + bestp[1];
+  float best41 = 0.f;
+  int pick41 = -1;
+  if (rc == 0 && c->ops->can_fuse41(c->prec, c->g)) rc = ensure_l1alt(c);
+  if (rc == 0 && c->ops->can_fuse41(c->prec, c->g)) {
+    b.lout = (b.l1 == c->l[0]) ? c->l1alt : c->l[0];
+    b.lout2 = nullptr;
+    b.part2 = c->part + 2 * c->ntiles;
+    for (int v = 3; v < c->ops->num_shapes && rc == 0; ++v) {
+      hipError_t e = c->ops->pass(c->prec, 6, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, 6, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        rc = hipfail(c, "tile autotuning of fused passes 4+1 variant %d failed: %s", v, hipGetErrorString(e));
+        break;
+      }
+      t /= reps;
+      if (ms) ms[4 * c->ops->num_shapes + 3 + (v - 3)] = t;
+      if (pick41 < 0 || t < best41) {
+        best41 = t;
+        pick41 = v;
+      }
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) return rc;
@@ -928,23 +1030,34 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     c->fuse23 = 1;
     c->tv[2] = pick23;
   }
+  c->fuse41 = 0;
+  if (pick41 >= 0 && best41 < sep41) {
+    c->fuse41 = 1;
+    c->tv[4] = pick41;
+  }
   free_graphs(c);
   return 0;
 }
 
-int ss_set_fused_stages(ss_ctx* c, int on) {
+int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (!c) return fail(nullptr, "null context");
-  if (on && !c->ops->can_fuse23(c->prec, c->g))
+  if (flags & ~(SS_FUSE_23 | SS_FUSE_41)) return fail(c, "unknown fused-stage flags 0x%x", flags);
+  if ((flags & SS_FUSE_23) && !c->ops->can_fuse23(c->prec, c->g))
     return fail(c, "passes 2 and 3 of theory %s cannot share one sweep at this precision and size", c->ops->name);
-  c->fuse23 = on ? 1 : 0;
-  if (on && c->tv[2] < 3) c->tv[2] = 3;
+  if ((flags & SS_FUSE_41) && !c->ops->can_fuse41(c->prec, c->g))
+    return fail(c, "pass 4 and pass 1 of theory %s cannot share one sweep at this precision and size", c->ops->name);
+  if ((flags & SS_FUSE_41) && ensure_l1alt(c)) return -1;
+  c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
+  c->fuse41 = (flags & SS_FUSE_41) ? 1 : 0;
+  if (c->fuse23 && c->tv[2] < 3) c->tv[2] = 3;
+  if (c->fuse41 && c->tv[4] < 3) c->tv[4] = 3;
   free_graphs(c);
   return 0;
 }
 
-int ss_get_fused_stages(ss_ctx* c, int* on) {
+int ss_get_fused_stages(ss_ctx* c, int* flags) {
   if (!c) return fail(nullptr, "null context");
-  if (on) *on = c->fuse23;
+  if (flags) *flags = (c->fuse23 ? SS_FUSE_23 : 0) | (c->fuse41 ? SS_FUSE_41 : 0);
   return 0;
 }
 
@@ -1125,7 +1238,7 @@ int ss_synchronize(ss_ctx* c) {
 int ss_memory_bytes(ss_ctx* c, size_t* bytes) {
   if (!c || !bytes) return fail(c, "null argument");
   const size_t f = (size_t)c->g.nf * (size_t)c->g.N * elem_size(c->prec);
-  *bytes = 7 * f + sizeof(double) * 3 * (size_t)c->ntiles + sizeof(double) * 2 * (size_t)c->hist_cap +
+  *bytes = (c->l1alt ? 8 : 7) * f + sizeof(double) * 3 * (size_t)c->ntiles + sizeof(double) * 2 * (size_t)c->hist_cap +
            (c->dens ? sizeof(double) * (size_t)c->g.N : 0);
   return 0;
 }

@@ -27,32 +27,35 @@ struct PassBufs {
   const void* l1;
   const void* l2;
   const void* l3;
-  void* lout;      // pass 0: EG; passes 1-3: l_pass; fused passes 2+3: l2
+  void* lout;      // pass 0: EG; passes 1-3: l_pass; fused passes 2+3: l2; fused 4+1: l1 of the next step
   void* lout2;     // fused passes 2+3: l3
   void* Fout;      // pass 4
   void* Vout;      // pass 4
   double* part0;   // pass 1: energy partial per tile; pass 4: signed max partial
   double* part1;   // pass 4: max |EG| partial
+  double* part2;   // fused 4+1: energy partial of the new field
   const DevState* st;
 };
 
 // Ring of NFR components (NF for one stage field, 2 NF when two stage fields
 // share a pass).
-template <typename R, int NFR>
+// RPM: most rows per wave and iteration (1 where the register prefetch of two
+// rows would not fit the register file).
+template <typename R, int NFR, int RPM = 2>
 struct StreamShape {
   static constexpr int RY = TY + 4;
   static constexpr int kBytesRow = NFR * RY * (int)sizeof(R);
   // Two rows per wave and iteration when that 16-row ring fits 40 KiB, else one.
-  static constexpr int RPW = (16 * kBytesRow <= 40960) ? 2 : 1;
+  static constexpr int RPW = (RPM >= 2 && 16 * kBytesRow <= 40960) ? 2 : 1;
   static constexpr int RB = kWaves * RPW;  // rows computed per iteration
   static constexpr int NR = (RB + 4 <= 8) ? 8 : 16;  // ring rows: a power of two >= RB + 4
   static constexpr bool kFits = NR * kBytesRow <= 65536;
   static_assert(RB + 4 <= NR, "ring too small");
 };
 
-template <typename R, int NFR>
+template <typename R, int NFR, int RPM = 2>
 struct Ring {
-  using S = StreamShape<R, NFR>;
+  using S = StreamShape<R, NFR, RPM>;
   static_assert(S::kFits, "ring does not fit in 64 KiB of LDS");
   R v[NFR][S::NR][S::RY];
 };
@@ -104,8 +107,8 @@ __device__ __forceinline__ void st(void* base, int a, unsigned N, unsigned boff,
 // Stencils of component a at ring column cy; sl[0..4] are the ring slots of
 // rows x-2 .. x+2.  Same formulas and masks as window_derivs; INTERIOR (every
 // mask true) drops the selects.
-template <bool INTERIOR, typename T, typename R, int NFR>
-__device__ __forceinline__ void ring_derivs(const Ring<R, NFR>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
+template <bool INTERIOR, typename T, typename R, int NFR, int RPM>
+__device__ __forceinline__ void ring_derivs(const Ring<R, NFR, RPM>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                             const SiteInfo& s, T& f, T d1[2], T d2[3]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
   const T c = T(q.v[a][sl[2]][cy]);
@@ -132,8 +135,8 @@ __device__ __forceinline__ void ring_derivs(const Ring<R, NFR>& q, int a, const 
   }
 }
 
-template <bool INTERIOR, typename T, typename R, int NFR>
-__device__ __forceinline__ void ring_d1(const Ring<R, NFR>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
+template <bool INTERIOR, typename T, typename R, int NFR, int RPM>
+__device__ __forceinline__ void ring_d1(const Ring<R, NFR, RPM>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                         const SiteInfo& s, T& f, T d1[2]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23);
   const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
@@ -157,11 +160,12 @@ __device__ __forceinline__ void ring_d1(const Ring<R, NFR>& q, int a, const int 
 template <typename R, int NF, int RPW>
 struct StreamPrefetch {
   R f[RPW + 1][NF], v[RPW + 1][NF], l[RPW + 1][NF];
+  R la[RPW + 1][NF], lc[RPW + 1][NF];  // fused 4+1 only: l1 and l3
 };
 
-template <typename R, int NFR>
+template <typename R, int NFR, int RPM>
 __device__ __forceinline__ void group_site(int k, int& r, int& c) {
-  using S = StreamShape<R, NFR>;
+  using S = StreamShape<R, NFR, RPM>;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (k < S::RPW) {
     r = (w * S::RPW + k + S::RB - 2) & (S::RB - 1);
@@ -176,17 +180,19 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
 // Issue the loads of the group of rows starting at rb.  Element offsets are
 // 32-bit (the host keeps nf*X*Y below 2^31); a site outside the lattice loads
 // element 0 and is replaced by 0, so no load sits behind a branch.  PASS
-// selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2.
-template <int PASS, typename R, int NF, int NFR>
-__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR>::RPW>& p, const PassBufs& b,
+// selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 41 F, V,
+// l2, l1, l3.
+template <int PASS, typename R, int NF, int NFR, int RPM>
+__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, RPM>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
-  using S = StreamShape<R, NFR>;
-  const void* L = PASS == 4 ? b.l2 : b.l1;
+  using S = StreamShape<R, NFR, RPM>;
+  const void* L = (PASS == 4 || PASS == 41) ? b.l2 : b.l1;
   const unsigned NP = (unsigned)g.N;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
+    if (PASS == 41 && k == S::RPW) break;  // the 4+1 sweep loads its halo-column sites when it stores them
     int r, c;
-    group_site<R, NFR>(k, r, c);
+    group_site<R, NFR, RPM>(k, r, c);
     const int x = rb + r, y = y0 - 2 + c;
     const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
     const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
@@ -207,8 +213,30 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
       } else {
         p.l[k][a] = R(0);
       }
+      if (PASS == 41) {
+        const R l1v = ld<R>(plane<R>(b.l1, a, NP), boff), l3v = ld<R>(plane<R>(b.l3, a, NP), boff);
+        p.la[k][a] = in ? l1v : R(0);
+        p.lc[k][a] = in ? l3v : R(0);
+      }
     }
   }
+}
+
+// The field after a step at one site, from the inputs of pass 4 (pointwise:
+// F' = Temp4 + dt (l1 - 2 l2 + l3)/6, then the unit constraint), the same
+// arithmetic as the combine of stream_rows.  ``in``: the site is in the
+// lattice (outside it the value stays 0).
+template <class TH, typename R>
+__device__ __forceinline__ void new_field(const typename TH::TP& tp, const Geom& g, bool in, const R* f, const R* v,
+                                          const R* l1, const R* l2, const R* l3, R dt, R* T4, R* Fn) {
+  constexpr int NF = TH::NF;
+  const R dt6 = dt * R(1.0 / 6.0);
+#pragma unroll
+  for (int a = 0; a < NF; ++a) {
+    T4[a] = f[a] + dt * (v[a] + R(0.5) * l2[a]);
+    Fn[a] = T4[a] + dt6 * (l1[a] - R(2) * l2[a] + l3[a]);
+  }
+  if (TH::kUnitConstraint && g.unit_mag && in) TH::template normalize<R>(tp, Fn);
 }
 
 template <int PASS, typename R>
@@ -222,15 +250,16 @@ __device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
 
 // Write a group into the ring; slot0 is the ring slot of its first row.  PASS
 // 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to NF..2NF-1.
-template <int PASS, typename R, int NF, int NFR>
-__device__ __forceinline__ void group_store(Ring<R, NFR>& q, const StreamPrefetch<R, NF, StreamShape<R, NFR>::RPW>& p,
-                                            int slot0, R dt) {
-  using S = StreamShape<R, NFR>;
+template <int PASS, typename R, int NF, int NFR, int RPM>
+__device__ __forceinline__ void group_store(Ring<R, NFR, RPM>& q,
+                                            const StreamPrefetch<R, NF, StreamShape<R, NFR, RPM>::RPW>& p, int slot0,
+                                            R dt) {
+  using S = StreamShape<R, NFR, RPM>;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
     if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
     int r, c;
-    group_site<R, NFR>(k, r, c);
+    group_site<R, NFR, RPM>(k, r, c);
     const int slot = (slot0 + r) & (S::NR - 1);
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
@@ -244,19 +273,87 @@ __device__ __forceinline__ void group_store(Ring<R, NFR>& q, const StreamPrefetc
   }
 }
 
+// Fused 4+1: Temp4 into ring components 0..NF-1 and the new field F' into
+// NF..2NF-1, both from the group's loads (rows from rb, columns from y0 - 2);
+// the halo-column sites (threads < 4 RB) are loaded here, not prefetched,
+// which keeps five arrays of one site out of the prefetch registers.
+template <class TH, typename R, int NFR, int RPM>
+__device__ __forceinline__ void group_store41(Ring<R, NFR, RPM>& q,
+                                              const StreamPrefetch<R, TH::NF, StreamShape<R, NFR, RPM>::RPW>& p,
+                                              int slot0, R dt, const typename TH::TP& tp, const PassBufs& b,
+                                              const Geom& g, int rb, int y0, bool kill) {
+  using S = StreamShape<R, NFR, RPM>;
+  constexpr int NF = TH::NF;
+  const unsigned NP = (unsigned)g.N;
+#pragma unroll
+  for (int k = 0; k <= S::RPW; ++k) {
+    if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
+    int r, c;
+    group_site<R, NFR, RPM>(k, r, c);
+    const int slot = (slot0 + r) & (S::NR - 1);
+    const int x = rb + r, y = y0 - 2 + c;
+    const bool in = x >= 0 && x < g.X && y >= 0 && y < g.Y;
+    R T4[NF], Fn[NF];
+    if (k < S::RPW) {
+      new_field<TH, R>(tp, g, in, p.f[k], p.v[k], p.la[k], p.l[k], p.lc[k], dt, T4, Fn);
+    } else {
+      const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
+      R f[NF], v[NF], l1[NF], l2[NF], l3[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        const R f0 = ld<R>(plane<R>(b.F, a, NP), boff), v0 = ld<R>(plane<R>(b.V, a, NP), boff);
+        const R a1 = ld<R>(plane<R>(b.l1, a, NP), boff), a2 = ld<R>(plane<R>(b.l2, a, NP), boff);
+        const R a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
+        f[a] = in ? f0 : R(0);
+        v[a] = (in && !kill) ? v0 : R(0);
+        l1[a] = in ? a1 : R(0);
+        l2[a] = in ? a2 : R(0);
+        l3[a] = in ? a3 : R(0);
+      }
+      new_field<TH, R>(tp, g, in, f, v, l1, l2, l3, dt, T4, Fn);
+    }
+#pragma unroll
+    for (int a = 0; a < NF; ++a) {
+      q.v[a][slot][c] = T4[a];
+      q.v[NF + a][slot][c] = Fn[a];
+    }
+  }
+}
+
 // The RPW sites of this thread in one iteration (rows c0 + w*RPW + k), from
 // the stage field in ring components AOFF .. AOFF+NF-1.
-// Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied).
-template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF>
-__device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBufs& b, const typename TH::TP& tp,
+// Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied);
+// with FOFF >= 0 it takes the new field F' from ring components FOFF ..
+// FOFF+NF-1 (the 4+1 sweep, which builds F' there) instead of forming it.
+template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int RPM>
+__device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
                                             double& acc1) {
-  using S = StreamShape<R, NFR>;
+  using S = StreamShape<R, NFR, RPM>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
   const unsigned NP = (unsigned)g.N;
   const int w = threadIdx.x >> 6;
+  // Pass 4 in FP32: l1 and l3 of this thread's sites, issued before the
+  // stencils so their latency overlaps the variation (the stores of the loop
+  // below would otherwise keep the compiler from hoisting them).  In FP64 the
+  // registers are not there to spare, and they are loaded at the combine.
+  constexpr bool kPre = PASS == 4 && sizeof(R) == 4;
+  R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
+  if (kPre) {
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int x = c0 + w * RPW + k;
+      const bool in = INTERIOR || (x < g.X && y < g.Y);
+      const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        pl1[k][a] = ld<R>(plane<R>(b.l1, a, NP), boff);
+        pl3[k][a] = ld<R>(plane<R>(b.l3, a, NP), boff);
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < RPW; ++k) {
     const int x = c0 + w * RPW + k;
@@ -300,18 +397,21 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBuf
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
           const R Vc = kV[k * NF + a];
-          const R a1 = ld<R>(plane<R>(b.l1, a, NP), boff);
+          const R a1 = kPre ? pl1[kPre ? k : 0][a] : ld<R>(plane<R>(b.l1, a, NP), boff);
           const R a2 = kL[k * NF + a];
-          const R a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
+          const R a3 = kPre ? pl3[kPre ? k : 0][a] : ld<R>(plane<R>(b.l3, a, NP), boff);
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
+          if (FOFF >= 0)
+            Fn[a] = q.v[FOFF + a][sl[2]][cy];
+          else
+            Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
           const double e = (double)eg[a];
           acc0 = fmax(acc0, e);
           acc1 = fmax(acc1, fabs(e));
         }
         if (TH::kUnitConstraint && g.unit_mag) {
-          TH::template normalize<R>(tp, Fn);
+          if (FOFF < 0) TH::template normalize<R>(tp, Fn);
           TH::template project<R>(tp, Vn, Fn);
         }
 #pragma unroll
@@ -327,19 +427,26 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR>& q, const PassBuf
 // One strip of one pass (PASS 0..4 as in stage_tile; 23 = passes 2 and 3 in
 // one sweep: both stage fields depend only on F, V and l1, so they are built
 // from the same loads into two halves of the ring and give l2 -> lout and
-// l3 -> lout2).  Partials: one per strip.
+// l3 -> lout2; 41 = pass 4 and pass 1 of the next step in one sweep: the new
+// field F' = Temp4 + dt (l1 - 2 l2 + l3)/6 is pointwise in the inputs of pass
+// 4, so Temp4 and F' are built from the same loads into the two halves of the
+// ring, pass 4 runs on the first half (F', V' -> Fout, Vout) and pass 1 on the
+// second (l1 of the next step -> lout, its energy partial -> part2)).
+// Partials: one per strip.
 template <class TH, typename R, int PASS>
 struct StreamRing {
-  static constexpr int NFR = PASS == 23 ? 2 * TH::NF : TH::NF;
-  using type = Ring<R, NFR>;
+  static constexpr int NFR = (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
+  // the 4+1 sweep prefetches five arrays: one row per wave and iteration
+  static constexpr int RPM = PASS == 41 ? 1 : 2;
+  using type = Ring<R, NFR, RPM>;
 };
 
 template <class TH, typename R, int PASS>
 __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::type& q, double* red,
                                              const PassBufs& b, const typename TH::TP& tp, const Geom& g, int xs,
                                              int sx, int y0, bool kill, int t) {
-  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS>::NFR;
-  using S = StreamShape<R, NFR>;
+  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS>::NFR, RPM = StreamRing<TH, R, PASS>::RPM;
+  using S = StreamShape<R, NFR, RPM>;
   constexpr int RB = S::RB, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
   const int lane = threadIdx.x & 63;
@@ -358,6 +465,27 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     const int gx = xs - 2 + r, gy = y0 - 2 + c;
     const bool in = gx >= 0 && gx < g.X && gy >= 0 && gy < g.Y;
     const unsigned boff = in ? (unsigned)(gx * g.Y + gy) * (unsigned)sizeof(R) : 0u;
+    if constexpr (PASS == 41) {
+      R fo[NF], vv[NF], l1v[NF], l2v[NF], l3v[NF], T4[NF], Fn[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        const R f0 = ld<R>(plane<R>(b.F, a, NP), boff), v0 = ld<R>(plane<R>(b.V, a, NP), boff);
+        const R a1 = ld<R>(plane<R>(b.l1, a, NP), boff), a2 = ld<R>(plane<R>(b.l2, a, NP), boff);
+        const R a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
+        fo[a] = in ? f0 : R(0);
+        vv[a] = (in && !kill) ? v0 : R(0);
+        l1v[a] = in ? a1 : R(0);
+        l2v[a] = in ? a2 : R(0);
+        l3v[a] = in ? a3 : R(0);
+      }
+      new_field<TH, R>(tp, g, in, fo, vv, l1v, l2v, l3v, dt, T4, Fn);
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        q.v[a][r][c] = T4[a];
+        q.v[NF + a][r][c] = Fn[a];
+      }
+      continue;
+    }
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       const R fo = ld<R>(plane<R>(b.F, a, NP), boff);
@@ -378,7 +506,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
   // iteration (keep) and, for the first two compute rows, in the next (hold).
   const int w = threadIdx.x >> 6;
   R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
-  if (PASS == 4) {
+  if (PASS == 4 || PASS == 41) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int xk = xs + w * RPW + k, xh = xs + RB + w * RPW + k;
@@ -397,12 +525,12 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     }
   }
   StreamPrefetch<R, NF, RPW> pf;
-  if (nit > 1) group_load<PASS, R, NF, NFR>(pf, b, g, xs + RB + 2, y0, kill);
+  if (nit > 1) group_load<PASS, R, NF, NFR, RPM>(pf, b, g, xs + RB + 2, y0, kill);
 
   PassBufs b2 = b, b3 = b;
   b3.lout = b.lout2;
-  double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
-  double acc1 = 0.0;
+  double acc0 = (PASS == 4 || PASS == 41) ? -1.0e300 : 0.0;
+  double acc1 = 0.0, acc2 = 0.0, unused = 0.0;
   for (int it = 0; it < nit; ++it) {
     __syncthreads();  // ring holds rows xs + it*RB - 2 .. xs + it*RB + RB + 1
     const int c0 = xs + it * RB;
@@ -410,25 +538,39 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     const bool interior = (c0 > g.halo) && (c0 + RB - 1 < g.X - g.halo) && (y0 > g.halo) && (y0 + TY - 1 < g.Y - g.halo);
     if (PASS == 23) {
       if (interior) {
-        stream_rows<TH, R, 2, true, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
-        stream_rows<TH, R, 3, true, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 2, true, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 3, true, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
       } else {
-        stream_rows<TH, R, 2, false, NFR, 0>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
-        stream_rows<TH, R, 3, false, NFR, NF>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 2, false, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 3, false, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+      }
+    } else if (PASS == 41) {
+      // pass 4 from Temp4 (ring 0..NF-1), then pass 1 of the next step from F' (ring NF..2NF-1)
+      if (interior) {
+        stream_rows<TH, R, 4, true, NFR, 0, NF>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                            acc1);
+        stream_rows<TH, R, 1, true, NFR, NF, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc2, unused);
+      } else {
+        stream_rows<TH, R, 4, false, NFR, 0, NF>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                             acc1);
+        stream_rows<TH, R, 1, false, NFR, NF, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc2, unused);
       }
     } else if (interior) {
-      stream_rows<TH, R, PASS, true, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+      stream_rows<TH, R, PASS, true, NFR, 0, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
                                              acc1);
     } else {
-      stream_rows<TH, R, PASS, false, NFR, 0>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+      stream_rows<TH, R, PASS, false, NFR, 0, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
                                               acc1);
     }
     __syncthreads();  // every read of this iteration's rows is done
     if (it + 1 < nit) {
       // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
       // no row of iteration it+1 other than themselves maps to
-      group_store<PASS, R, NF, NFR>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
-      if (PASS == 4) {
+      if constexpr (PASS == 41)
+        group_store41<TH, R, NFR, RPM>(q, pf, (it * RB + RB + 4) & (NR - 1), dt, tp, b, g, c0 + RB + 2, y0, kill);
+      else
+        group_store<PASS, R, NF, NFR, RPM>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
+      if (PASS == 4 || PASS == 41) {
         // next iteration's sites: the group just stored, or (first two compute
         // rows) the hold from the group before; the hold takes the new group.
 #pragma unroll
@@ -443,17 +585,21 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
           }
         }
       }
-      if (it + 2 < nit) group_load<PASS, R, NF, NFR>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
+      if (it + 2 < nit) group_load<PASS, R, NF, NFR, RPM>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
     }
   }
   if (PASS == 1) {
     const double s = block_sum<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = s;
-  } else if (PASS == 4) {
+  } else if (PASS == 4 || PASS == 41) {
     const double m0 = block_max<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = m0;
     const double m1 = block_max<kThreads>(acc1, red);
     if (threadIdx.x == 0) b.part1[t] = m1;
+    if (PASS == 41) {
+      const double e = block_sum<kThreads>(acc2, red);
+      if (threadIdx.x == 0) b.part2[t] = e;
+    }
   }
 }
 

@@ -395,13 +395,15 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     assert np.abs(Fg - Fo).max() <= 1e-9
 
 
-@pytest.mark.parametrize("variants,fused", [([1, 1, 1, 1], False), ([2, 2, 2, 2], False), ([0, 2, 1, 2], False),
-                                            ([3, 3, 3, 3], False), ([4, 4, 4, 4], False), ([5, 5, 5, 5], False),
-                                            ([0, 3, 4, 5], False), ([0, 3, 0, 3], True), ([2, 5, 0, 4], True)])
-def test_tile_shape_variants_match_oracle(variants, fused):
+@pytest.mark.parametrize("variants,fused,fused41", [
+    ([1, 1, 1, 1], False, False), ([2, 2, 2, 2], False, False), ([0, 2, 1, 2], False, False),
+    ([3, 3, 3, 3], False, False), ([4, 4, 4, 4], False, False), ([5, 5, 5, 5], False, False),
+    ([0, 3, 4, 5], False, False), ([0, 3, 0, 3], True, False), ([2, 5, 0, 4], True, False),
+    ([0, 1, 2, 3], False, True), ([1, 3, 0, 5], True, True), ([2, 4, 4, 4], True, True)])
+def test_tile_shape_variants_match_oracle(variants, fused, fused41):
     """Every launch-shape variant of the fused passes (tiles and streaming strips,
-    lattice 70 x 130: partial strips and partial column blocks) gives the oracle's
-    trajectory."""
+    lattice 70 x 130: partial strips and partial column blocks), with and without
+    the fused 2+3 and 4+1 sweeps, gives the oracle's trajectory."""
     th = ssa.load_theory("Chiral magnet")
     p = H.chiral_params(th, 70, 130)
     p_i, p_f = H.packed(th, p)
@@ -412,9 +414,9 @@ def test_tile_shape_variants_match_oracle(variants, fused):
     Es, _ = ocore.relax(o, Fo, Vo, p_i, p_f, 10, prev_energy=E0)
     sim = _sim(th, p, F)
     sim.set_tile_variants(variants)
-    sim.set_fused_stages(fused)
+    sim.set_fused_stages(fused, fused41)
     info = sim.tile_variants()
-    assert info["variants"] == variants and info["fused23"] == fused
+    assert info["variants"] == variants and info["fused23"] == fused and info["fused41"] == fused41
     E, _, _ = sim.steps(sim.compute_energy(), 10, history=True)
     gE = np.array(sim.last_history[0])
     dF = np.abs(sim.get_field() - Fo).max()
@@ -470,7 +472,7 @@ def test_streaming_form_matches_oracle_for_wide_theories(name):
     sim = _sim(th, p, F)
     sim.set_tile_variants([3, 4, 5, 3])
     if name.startswith("Baby"):
-        sim.set_fused_stages(True)  # 3 fields: both FP64 stage fields fit one ring
+        sim.set_fused_stages(True, True)  # 3 fields: both FP64 halves fit one ring
     sim.steps(sim.compute_energy(), 6, history=True)
     gE = np.array(sim.last_history[0])
     dF = np.abs(sim.get_field() - Fo).max()
@@ -480,6 +482,81 @@ def test_streaming_form_matches_oracle_for_wide_theories(name):
     sim.close()
 
 
+def _relax_calls(sim, E, counts):
+    """Consecutive calls of ``counts`` steps; returns the energies and errs of every step."""
+    hE, hR = [], []
+    for n in counts:
+        E, _, _ = sim.steps(E, n, history=True)
+        if n:
+            hE += list(sim.last_history[0])
+            hR += list(sim.last_history[1])
+    return E, np.array(hE), np.array(hR)
+
+
+@pytest.mark.parametrize("precision,graphs", [("fp64", 0), ("fp64", 4), ("fp32", 0), ("fp32", 6)])
+def test_fused41_matches_separate_passes(precision, graphs):
+    """The 4+1 schedule (pass 1 only in the first step of a call, l1 in two
+    buffers) against the separate passes over calls of 0, 1, 2, 7, 13 and 1
+    steps, eager and in graphs: the same energies, errs and fields (FP64 energy
+    sums differ only in order)."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 150, 200)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=8, rough=True)
+    a = _sim(th, p, F, precision=precision)
+    b = _sim(th, p, F, precision=precision)
+    a.set_tile_variants([3, 3, 3, 4])
+    b.set_tile_variants([3, 3, 3, 4])
+    a.set_fused_stages(True, True)
+    b.set_fused_stages(True, False)
+    if graphs:
+        a.use_graphs(graphs)
+        b.use_graphs(graphs)
+    counts = [0, 1, 2, 7, 13, 1]
+    E0 = a.compute_energy()
+    Ea, hEa, hRa = _relax_calls(a, E0, counts)
+    Eb, hEb, hRb = _relax_calls(b, E0, counts)
+    dF = np.abs(a.get_field() - b.get_field()).max()
+    dV = np.abs(a.get_velocity() - b.get_velocity()).max()
+    print(f"[fused41 {precision} graphs={graphs}] relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e} max|dV|={dV:.2e} "
+          f"max hist relE={np.max(np.abs(hEa - hEb) / np.abs(hEb)):.2e}")
+    tE, tF, tR = (1e-12, 1e-12, 1e-10) if precision == "fp64" else (1e-7, 1e-5, 1e-4)
+    assert len(hEa) == sum(counts)
+    assert np.all(np.abs(hEa - hEb) <= tE * np.abs(hEb))
+    assert np.all(np.abs(hRa - hRb) <= tR * np.abs(hRb).max())
+    assert H.rel(Ea, Eb) < tE
+    assert dF <= tF and dV <= tF
+    a.close()
+    b.close()
+
+
+def test_fused41_survives_precision_switch_and_autotune():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 128, 160)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=2)
+    a = _sim(th, p, F, precision="fp32")
+    info = a.autotune_tiles(reps=2)
+    assert len(info["ms_fused41"]) == 3 and np.all(info["ms_fused41"] > 0)
+    a.set_tile_variants([0, 3, 3, 3])
+    a.set_fused_stages(True, True)
+    b = _sim(th, p, F, precision="fp32")
+    b.set_tile_variants([0, 3, 3, 3])
+    E0 = a.compute_energy()
+    Ea, _, _ = a.steps(E0, 5)
+    Eb, _, _ = b.steps(E0, 5)
+    a.set_precision("fp64")
+    b.set_precision("fp64")
+    assert a.tile_variants()["fused41"]
+    Ea, _, _ = a.steps(Ea, 6)
+    Eb, _, _ = b.steps(Eb, 6)
+    # FP32 steps of the two schedules agree to FP32 rounding, which the FP64 steps carry on
+    assert H.rel(Ea, Eb) < 1e-7
+    assert np.abs(a.get_field() - b.get_field()).max() <= 1e-5
+    a.close()
+    b.close()
+
+
 def test_fused_stages_rejected_when_two_rings_do_not_fit():
     th = ssa.load_theory("Spin triplet superconducting ferromagnet")
     p = H.st_params(th, 40, 40)
@@ -487,6 +564,8 @@ def test_fused_stages_rejected_when_two_rings_do_not_fit():
     sim = _sim(th, p, H.st_field(th, p_i, p_f))
     with pytest.raises(ValueError, match="cannot share one sweep"):
         sim.set_fused_stages(True)
+    with pytest.raises(ValueError, match="cannot share one sweep"):
+        sim.set_fused_stages(False, True)
     sim.close()
 
 
