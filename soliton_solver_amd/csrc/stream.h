@@ -437,7 +437,7 @@ template <class TH, typename R, int PASS>
 struct StreamRing {
   static constexpr int NFR = (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
   // the 4+1 sweep prefetches five arrays: one row per wave and iteration
-  static constexpr int RPM = PASS == 41 ? 1 : 2;
+  static constexpr int RPM = (PASS == 41 || PASS == 4) ? 1 : 2;
   using type = Ring<R, NFR, RPM>;
 };
 

@@ -189,9 +189,10 @@ struct BabySkyrme {
   __device__ __forceinline__ static void normalize(const TP& tp, T f[NF]) {
     const T n = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
     if (n != T(0)) {
-      f[0] = f[0] / n;
-      f[1] = f[1] / n;
-      f[2] = f[2] / n;
+      const T inv = T(1) / n;
+      f[0] = f[0] * inv;
+      f[1] = f[1] * inv;
+      f[2] = f[2] * inv;
     }
   }
 

@@ -17,6 +17,7 @@ struct ChiralMagnet {
 
   struct TP {
     double coupK, couph, coupmu;   // p_f[6..8]
+    double inv_2mu;                // 1/(2 coupmu)
     double D00, D01, D10, D11;     // DMI matrix by flag (kernels.py:171-183)
     int demag;                     // p_i[15]
   };
@@ -35,6 +36,7 @@ struct ChiralMagnet {
     tp->coupK = pf[6];
     tp->couph = pf[7];
     tp->coupmu = pf[8];
+    tp->inv_2mu = 1.0 / (2.0 * pf[8]);
     tp->D00 = tp->D01 = tp->D10 = tp->D11 = 0.0;
     if (pi[11]) { tp->D00 = -1.0; tp->D11 = -1.0; }
     else if (pi[12]) { tp->D01 = -1.0; tp->D10 = 1.0; }
@@ -119,7 +121,7 @@ struct ChiralMagnet {
           }
         }
       }
-      if (tp.demag) e -= f[3] * d1[i][i] + d1[3][i] * d1[3][i] / (T(2) * T(tp.coupmu));
+      if (tp.demag) e -= f[3] * d1[i][i] + d1[3][i] * d1[3][i] * T(tp.inv_2mu);
     }
     const T mz = f[2];
     e += T(tp.couph) * (T(1) - mz);
@@ -174,8 +176,9 @@ struct ChiralMagnet {
     for (int a = 0; a < NMAG; ++a) s2 += f[a] * f[a];
     const T n = sqrt(s2);
     if (n != T(0)) {
+      const T inv = T(1) / n;
 #pragma unroll
-      for (int a = 0; a < NMAG; ++a) f[a] = f[a] / n;
+      for (int a = 0; a < NMAG; ++a) f[a] = f[a] * inv;
     }
   }
 

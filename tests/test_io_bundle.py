@@ -62,3 +62,17 @@ def test_bundle_core_files(tmp_path):
     assert open(os.path.join(tmp_path, "LatticePoints.dat")).read() == "4\t3\t"
     assert open(os.path.join(tmp_path, "LatticeVectors.dat")).read() == "1.5\t0\t0\t0.5\t"
     assert open(os.path.join(tmp_path, "yGrid.dat")).read() == oio.density_dat_text(g.ravel(), 1, X, Y)
+
+
+def test_oracle_text_is_the_correctly_rounded_decimal():
+    """Each value the oracle writers emit is the double's exact binary value
+    rounded to ``precision`` significant decimal digits (round half even), as
+    exact Decimal arithmetic gives it, and parses back to the same double at 32."""
+    from decimal import ROUND_HALF_EVEN, Context, Decimal
+    values = [0.1, -1.0 / 3.0, 6.02214076e23, 5e-324, 1.0, 123456789.125, -2.5e-300, 2.0 ** 70 + 1.0]
+    for x in values:
+        for p in (32, 17, 8):
+            text = oio.values_dat_text([x], p)
+            assert text.endswith("\t") and text.count("\t") == 1
+            assert Decimal(text[:-1]) == Context(prec=p, rounding=ROUND_HALF_EVEN).plus(Decimal(x))
+        assert float(oio.values_dat_text([x], 32)[:-1]) == x
