@@ -1025,13 +1025,14 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     c->tv[p] = pick[p];
     if (chosen) chosen[p - 1] = pick[p];
   }
+  // a fused sweep replaces the separate passes only when at least 3 % faster
   c->fuse23 = 0;
-  if (pick23 >= 0 && best23 < sep23) {
+  if (pick23 >= 0 && best23 < 0.97f * sep23) {
     c->fuse23 = 1;
     c->tv[2] = pick23;
   }
   c->fuse41 = 0;
-  if (pick41 >= 0 && best41 < sep41) {
+  if (pick41 >= 0 && best41 < 0.97f * sep41) {
     c->fuse41 = 1;
     c->tv[4] = pick41;
   }

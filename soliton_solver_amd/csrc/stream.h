@@ -322,14 +322,15 @@ __device__ __forceinline__ void group_store41(Ring<R, NFR, RPM>& q,
 
 // The RPW sites of this thread in one iteration (rows c0 + w*RPW + k), from
 // the stage field in ring components AOFF .. AOFF+NF-1.
-// Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied);
+// Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied), and
+// in the 4+1 sweep also l1 and l3 from kA / kC;
 // with FOFF >= 0 it takes the new field F' from ring components FOFF ..
 // FOFF+NF-1 (the 4+1 sweep, which builds F' there) instead of forming it.
 template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int RPM>
 __device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
-                                            double& acc1) {
+                                            double& acc1, const R* kA = nullptr, const R* kC = nullptr) {
   using S = StreamShape<R, NFR, RPM>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
@@ -339,7 +340,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const Pa
   // stencils so their latency overlaps the variation (the stores of the loop
   // below would otherwise keep the compiler from hoisting them).  In FP64 the
   // registers are not there to spare, and they are loaded at the combine.
-  constexpr bool kPre = PASS == 4 && sizeof(R) == 4;
+  constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF < 0;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
   if (kPre) {
 #pragma unroll
@@ -397,9 +398,9 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const Pa
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
           const R Vc = kV[k * NF + a];
-          const R a1 = kPre ? pl1[kPre ? k : 0][a] : ld<R>(plane<R>(b.l1, a, NP), boff);
+          const R a1 = FOFF >= 0 ? kA[k * NF + a] : kPre ? pl1[kPre ? k : 0][a] : ld<R>(plane<R>(b.l1, a, NP), boff);
           const R a2 = kL[k * NF + a];
-          const R a3 = kPre ? pl3[kPre ? k : 0][a] : ld<R>(plane<R>(b.l3, a, NP), boff);
+          const R a3 = FOFF >= 0 ? kC[k * NF + a] : kPre ? pl3[kPre ? k : 0][a] : ld<R>(plane<R>(b.l3, a, NP), boff);
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
           if (FOFF >= 0)
@@ -437,7 +438,7 @@ template <class TH, typename R, int PASS>
 struct StreamRing {
   static constexpr int NFR = (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
   // the 4+1 sweep prefetches five arrays: one row per wave and iteration
-  static constexpr int RPM = (PASS == 41 || PASS == 4) ? 1 : 2;
+  static constexpr int RPM = PASS == 41 ? 1 : 2;
   using type = Ring<R, NFR, RPM>;
 };
 
@@ -506,6 +507,8 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
   // iteration (keep) and, for the first two compute rows, in the next (hold).
   const int w = threadIdx.x >> 6;
   R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
+  // 4+1 sweep: l1 and l3 of the same sites (its front loads carry them already)
+  R keepA[RPW][NF], keepC[RPW][NF], holdA[RPW][NF], holdC[RPW][NF];
   if (PASS == 4 || PASS == 41) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
@@ -521,6 +524,14 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
         keepL[k][a] = ink ? q0 : R(0);
         holdV[k][a] = (inh && !kill) ? v1 : R(0);
         holdL[k][a] = inh ? q1 : R(0);
+        if (PASS == 41) {
+          const R a0 = ld<R>(plane<R>(b.l1, a, NP), bk), c0v = ld<R>(plane<R>(b.l3, a, NP), bk);
+          const R a1 = ld<R>(plane<R>(b.l1, a, NP), bh), c1v = ld<R>(plane<R>(b.l3, a, NP), bh);
+          keepA[k][a] = ink ? a0 : R(0);
+          keepC[k][a] = ink ? c0v : R(0);
+          holdA[k][a] = inh ? a1 : R(0);
+          holdC[k][a] = inh ? c1v : R(0);
+        }
       }
     }
   }
@@ -548,11 +559,11 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
       // pass 4 from Temp4 (ring 0..NF-1), then pass 1 of the next step from F' (ring NF..2NF-1)
       if (interior) {
         stream_rows<TH, R, 4, true, NFR, 0, NF>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                            acc1);
+                                                acc1, &keepA[0][0], &keepC[0][0]);
         stream_rows<TH, R, 1, true, NFR, NF, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc2, unused);
       } else {
         stream_rows<TH, R, 4, false, NFR, 0, NF>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                             acc1);
+                                                 acc1, &keepA[0][0], &keepC[0][0]);
         stream_rows<TH, R, 1, false, NFR, NF, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc2, unused);
       }
     } else if (interior) {
@@ -582,6 +593,12 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
             keepL[k][a] = lag ? holdL[k][a] : pf.l[k][a];
             holdV[k][a] = pf.v[k][a];
             holdL[k][a] = pf.l[k][a];
+            if (PASS == 41) {
+              keepA[k][a] = lag ? holdA[k][a] : pf.la[k][a];
+              keepC[k][a] = lag ? holdC[k][a] : pf.lc[k][a];
+              holdA[k][a] = pf.la[k][a];
+              holdC[k][a] = pf.lc[k][a];
+            }
           }
         }
       }
