@@ -179,11 +179,13 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * whose kernels use fixed 8x8/16x16 thread blocks).  Variants 0-2 stage one
  * LDS tile per workgroup: the largest that fits ~48 KiB, then its rows halved
  * and quartered (less LDS, more resident workgroups per CU).  Variants 3-5
- * march a workgroup down a strip of 64, 128 or 256 rows through an LDS ring
- * with register prefetch (rows fetched once per strip).  ss_autotune_tiles
- * times every pass in every variant ``reps`` times on the live buffers, keeps
- * the fastest per pass, and reports chosen[0..3] and ms[6*(pass-1)+variant]
- * (ms holds 30 entries, see ss_set_fused_stages);
+ * march a workgroup down a strip of 64, 128 or 256 rows, 64 columns wide,
+ * through an LDS ring with register prefetch (rows fetched once per strip);
+ * variants 6-8 are the same strips 128 columns wide (half the y-halo reads
+ * per site; where a theory's rings do not fit that wide, they run as 3-5).
+ * ss_autotune_tiles times every pass in every variant ``reps`` times on the
+ * live buffers, keeps the fastest per pass, and reports chosen[0..3] and
+ * ms[9*(pass-1)+variant] (ms holds 48 entries, see ss_set_fused_stages);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
  * do not depend on the variants; energies only through FP64 summation order. */
@@ -199,8 +201,8 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * and F' from one set of loads, finishes the step (F', V', err partials) and
  * runs pass 1 of the next step on F' (its l1 and energy); a call then runs
  * pass 1 only in its first step, and l1 alternates between two buffers.
- * ss_autotune_tiles times both sweeps in the streaming variants (ms[24..26]
- * for 2+3, ms[27..29] for 4+1) and switches each on when it beats the best
+ * ss_autotune_tiles times both sweeps in the streaming variants (ms[36..41]
+ * for 2+3, ms[42..47] for 4+1, variants 3..8) and switches each on when it beats the best
  * pair of separate passes; ss_set_fused_stages forces them (a fused sweep uses
  * the variant of pass 2 or pass 4, at least 3) and fails when the two halves
  * do not fit one LDS ring for the theory and precision.  Results equal the

@@ -416,22 +416,23 @@ class Simulation:
         self._check(_lib.lib().ss_time_passes(self._ctx, float(prev_energy), int(nsteps), out))
         return out
 
-    NUM_SHAPES = 6
+    NUM_SHAPES = 9
 
     def autotune_tiles(self, reps: int = 10) -> dict:
         """Time every fused pass in each shape variant on the live buffers and keep the
         fastest per pass (the relaxation state is unchanged).  Variants 0-2 are LDS
-        tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows in the
-        streaming form; the fused sweeps of passes 2+3 and 4+1 are timed in the
-        streaming variants.  Returns the chosen variants, their rows per tile/strip,
+        tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows and
+        64 columns in the streaming form, 6-8 the same strips 128 columns wide; the
+        fused sweeps of passes 2+3 and 4+1 are timed in the streaming variants.  Returns the chosen variants, their rows per tile/strip,
         the fused flags and the ms tables ([pass][variant], fused per variant 3-5)."""
         chosen = np.zeros(4, dtype=np.int32)
-        ms = np.zeros(4 * self.NUM_SHAPES + 6)
+        nst = self.NUM_SHAPES - 3
+        ms = np.zeros(4 * self.NUM_SHAPES + 2 * nst)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
         info["ms"] = ms[:4 * self.NUM_SHAPES].reshape(4, self.NUM_SHAPES)
-        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:4 * self.NUM_SHAPES + 3]
-        info["ms_fused41"] = ms[4 * self.NUM_SHAPES + 3:]
+        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:4 * self.NUM_SHAPES + nst]
+        info["ms_fused41"] = ms[4 * self.NUM_SHAPES + nst:]
         return info
 
     def set_fused_stages(self, fuse23: bool, fuse41: bool = False) -> None:
@@ -441,7 +442,7 @@ class Simulation:
         self._check(_lib.lib().ss_set_fused_stages(self._ctx, flags))
 
     def set_tile_variants(self, variants) -> None:
-        """Shape variant (0..5, see ``autotune_tiles``) of passes 1..4."""
+        """Shape variant (0..8, see ``autotune_tiles``) of passes 1..4."""
         v = np.ascontiguousarray(variants, dtype=np.int32)
         if v.shape != (4,):
             raise ValueError("tile variants: one entry per fused pass (4)")

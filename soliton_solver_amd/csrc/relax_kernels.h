@@ -643,53 +643,67 @@ struct TheoryLaunchers {
   using TP = typename TH::TP;
 
   // Shape variants of the fused passes: 0..2 tiles with the default rows, halved,
-  // quartered (at least 4); 3..5 strips of 64, 128, 256 rows (streaming form).
-  static constexpr int kNumShapes = 6;
-  static int strip_rows(int v) { return 64 << (v - 3); }
+  // quartered (at least 4); 3..5 strips one wavefront (64 columns) wide and
+  // 6..8 strips two wavefronts (128 columns) wide, of 64, 128, 256 rows
+  // (streaming form).
+  static constexpr int kNumShapes = 9;
+  static int strip_rows(int v) { return 64 << ((v - 3) % 3); }
+  static int strip_cw(int v) { return v >= 6 ? 2 : 1; }
+  // Two-wavefront strips are offered when every streaming ring of the theory
+  // that fits one wavefront wide also fits two wide.
+  template <typename R>
+  static constexpr bool wide_fits() {
+    constexpr int NF = TH::NF;
+    return StreamShape<R, NF, 6>::kFits && (!StreamShape<R, 2 * NF, 2>::kFits || StreamShape<R, 2 * NF, 6>::kFits) &&
+           (!StreamShape<R, 2 * NF, 1>::kFits || StreamShape<R, 2 * NF, 5>::kFits);
+  }
   // The streaming form addresses each plane with a 32-bit byte offset and the
   // field with 32-bit element offsets; larger fields run the default tile shape.
+  template <typename R>
   static int effective_shape(int v, const Geom& g) {
     const bool ok = (long long)TH::NF * g.N < (1LL << 31) && g.N * (long long)sizeof(double) < (1LL << 32);
-    return (v >= 3 && !ok) ? 0 : v;
+    if (v >= 3 && !ok) return 0;
+    if (v >= 6 && !wide_fits<R>()) return v - 3;
+    return v;
   }
 
   // Partials (tiles or strips) of one pass launch in shape variant ``v``.
   template <typename R>
   static int ntiles_r(int v, const Geom& g) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
-    v = effective_shape(v, g);
-    if (v >= 3) return make_slaunch<R, TH::NF>(g, strip_rows(v)).nblocks;
+    v = effective_shape<R>(v, g);
+    if (v >= 3) return make_slaunch<R, TH::NF>(g, strip_rows(v), TY * strip_cw(v)).nblocks;
     return v == 2 ? make_launch<R, TH::NF, r2>(g).ntiles
          : v == 1 ? make_launch<R, TH::NF, r1>(g).ntiles
                   : make_launch<R, TH::NF, r0>(g).ntiles;
   }
 
-  // Rows per tile (variants 0..2) or per strip (3..5).
+  // Rows per tile (variants 0..2) or per strip (3..8).
   static int shape_rows(int prec, int v) {
     if (v >= 3) return strip_rows(v);
     return prec ? variant_rows<float, TH::NF>(v) : variant_rows<double, TH::NF>(v);
   }
 
-  template <typename R>
+  template <typename R, int CW>
   static hipError_t stream_r(int pass, int sx, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
-    const SLaunch L = make_slaunch<R, TH::NF>(g, sx);
+    const SLaunch L = make_slaunch<R, TH::NF>(g, sx, TY * CW);
     dim3 grid(L.grid), block(kThreads);
     switch (pass) {
-      case 0: hipLaunchKernelGGL((stream_pass<TH, R, 0>), grid, block, 0, st, b, tp, g, L); break;
-      case 1: hipLaunchKernelGGL((stream_pass<TH, R, 1>), grid, block, 0, st, b, tp, g, L); break;
-      case 2: hipLaunchKernelGGL((stream_pass<TH, R, 2>), grid, block, 0, st, b, tp, g, L); break;
-      case 3: hipLaunchKernelGGL((stream_pass<TH, R, 3>), grid, block, 0, st, b, tp, g, L); break;
-      case 4: hipLaunchKernelGGL((stream_pass<TH, R, 4>), grid, block, 0, st, b, tp, g, L); break;
+      case 0: hipLaunchKernelGGL((stream_pass<TH, R, 0, CW>), grid, block, 0, st, b, tp, g, L); break;
+      case 1: hipLaunchKernelGGL((stream_pass<TH, R, 1, CW>), grid, block, 0, st, b, tp, g, L); break;
+      case 2: hipLaunchKernelGGL((stream_pass<TH, R, 2, CW>), grid, block, 0, st, b, tp, g, L); break;
+      case 3: hipLaunchKernelGGL((stream_pass<TH, R, 3, CW>), grid, block, 0, st, b, tp, g, L); break;
+      case 4: hipLaunchKernelGGL((stream_pass<TH, R, 4, CW>), grid, block, 0, st, b, tp, g, L); break;
       case 5:
-        if constexpr (StreamShape<R, 2 * TH::NF>::kFits) {
-          hipLaunchKernelGGL((stream_pass<TH, R, 23>), grid, block, 0, st, b, tp, g, L);
+        if constexpr (StreamRing<TH, R, 23, CW>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 23, CW>), grid, block, 0, st, b, tp, g, L);
           break;
         } else {
           return hipErrorInvalidValue;
         }
       case 6:
-        if constexpr (StreamShape<R, 2 * TH::NF, StreamRing<TH, R, 41>::RPM>::kFits) {
-          hipLaunchKernelGGL((stream_pass<TH, R, 41>), grid, block, 0, st, b, tp, g, L);
+        if constexpr (StreamRing<TH, R, 41, CW>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 41, CW>), grid, block, 0, st, b, tp, g, L);
           break;
         } else {
           return hipErrorInvalidValue;
@@ -698,18 +712,24 @@ struct TheoryLaunchers {
     }
     return hipGetLastError();
   }
+  template <typename R>
+  static hipError_t stream_v(int pass, int v, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
+    if constexpr (wide_fits<R>()) {
+      if (strip_cw(v) == 2) return stream_r<R, 2>(pass, strip_rows(v), b, tp, g, st);
+    }
+    return stream_r<R, 1>(pass, strip_rows(v), b, tp, g, st);
+  }
 
   // Passes 2 and 3 fused into one streaming sweep (pass id 5), and pass 4
   // with pass 1 of the next step (pass id 6), when both halves fit one ring
   // and the field uses 32-bit offsets.
   static int can_fuse23(int prec, const Geom& g) {
-    const bool fits = prec ? StreamShape<float, 2 * TH::NF>::kFits : StreamShape<double, 2 * TH::NF>::kFits;
-    return fits && effective_shape(3, g) == 3;
+    const bool fits = prec ? StreamRing<TH, float, 23>::shape::kFits : StreamRing<TH, double, 23>::shape::kFits;
+    return fits && (prec ? effective_shape<float>(3, g) : effective_shape<double>(3, g)) == 3;
   }
   static int can_fuse41(int prec, const Geom& g) {
-    constexpr int rf = StreamRing<TH, float, 41>::RPM, rd = StreamRing<TH, double, 41>::RPM;
-    const bool fits = prec ? StreamShape<float, 2 * TH::NF, rf>::kFits : StreamShape<double, 2 * TH::NF, rd>::kFits;
-    return fits && effective_shape(3, g) == 3 && !TH::kL2Normalize;
+    const bool fits = prec ? StreamRing<TH, float, 41>::shape::kFits : StreamRing<TH, double, 41>::shape::kFits;
+    return fits && (prec ? effective_shape<float>(3, g) : effective_shape<double>(3, g)) == 3 && !TH::kL2Normalize;
   }
 
   static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
@@ -733,9 +753,9 @@ struct TheoryLaunchers {
   static hipError_t pass_r(int pass, int v, const PassBufs& b, const TP& tp, const Geom& g, hipStream_t st) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
-    v = effective_shape(v, g);
-    if (pass == 5 || pass == 6) return v >= 3 ? stream_r<R>(pass, strip_rows(v), b, tp, g, st) : hipErrorInvalidValue;
-    if (v >= 3) return stream_r<R>(pass, strip_rows(v), b, tp, g, st);
+    v = effective_shape<R>(v, g);
+    if (pass == 5 || pass == 6) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
+    if (v >= 3) return stream_v<R>(pass, v, b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
     return pass_rv<R, r0>(pass, b, tp, g, st);

@@ -1011,7 +1011,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         break;
       }
       t /= reps;
-      if (ms) ms[4 * c->ops->num_shapes + 3 + (v - 3)] = t;
+      if (ms) ms[4 * c->ops->num_shapes + (c->ops->num_shapes - 3) + (v - 3)] = t;
       if (pick41 < 0 || t < best41) {
         best41 = t;
         pick41 = v;

@@ -39,23 +39,42 @@ struct PassBufs {
 
 // Ring of NFR components (NF for one stage field, 2 NF when two stage fields
 // share a pass).
-// RPM: most rows per wave and iteration (1 where the register prefetch of two
-// rows would not fit the register file).
-template <typename R, int NFR, int RPM = 2>
+// Shape code SC: bits 0-1 = most rows per wave and iteration (1 where the
+// register prefetch of two rows would not fit the register file); bit 2 = the
+// strip is two wavefronts (128 columns) wide, the workgroup's 4 waves split as
+// 2 across the columns x 2 down the rows (the y-halo columns are then shared
+// by twice as many sites), else one wavefront (64 columns) with all 4 waves
+// down the rows.
+template <typename R, int NFR, int SC = 2>
 struct StreamShape {
-  static constexpr int RY = TY + 4;
+  static constexpr int RPM = SC & 3;
+  static constexpr int CW = (SC & 4) ? 2 : 1;  // wavefronts across the columns
+  static constexpr int TYW = TY * CW;          // strip width in columns
+  static constexpr int RW = kWaves / CW;       // wavefronts down the rows
+  static constexpr int RY = TYW + 4;
   static constexpr int kBytesRow = NFR * RY * (int)sizeof(R);
-  // Two rows per wave and iteration when that 16-row ring fits 40 KiB, else one.
-  static constexpr int RPW = (RPM >= 2 && 16 * kBytesRow <= 40960) ? 2 : 1;
-  static constexpr int RB = kWaves * RPW;  // rows computed per iteration
-  static constexpr int NR = (RB + 4 <= 8) ? 8 : 16;  // ring rows: a power of two >= RB + 4
+  static constexpr int nr_for(int rb) { return (rb + 4 <= 8) ? 8 : 16; }
+  // Two rows per wave and iteration when that ring fits 40 KiB, else one.
+  static constexpr int RPW = (RPM >= 2 && nr_for(2 * RW) * kBytesRow <= 40960) ? 2 : 1;
+  static constexpr int RB = RW * RPW;  // rows computed per iteration
+  static constexpr int NR = nr_for(RB);  // ring rows: a power of two >= RB + 4
   static constexpr bool kFits = NR * kBytesRow <= 65536;
   static_assert(RB + 4 <= NR, "ring too small");
 };
 
-template <typename R, int NFR, int RPM = 2>
+// Column wave (0 .. CW-1) and row wave (0 .. RW-1) of the calling thread.
+template <class S>
+__device__ __forceinline__ int col_wave() {
+  return (threadIdx.x >> 6) % S::CW;
+}
+template <class S>
+__device__ __forceinline__ int row_wave() {
+  return (threadIdx.x >> 6) / S::CW;
+}
+
+template <typename R, int NFR, int SC = 2>
 struct Ring {
-  using S = StreamShape<R, NFR, RPM>;
+  using S = StreamShape<R, NFR, SC>;
   static_assert(S::kFits, "ring does not fit in 64 KiB of LDS");
   R v[NFR][S::NR][S::RY];
 };
@@ -68,10 +87,10 @@ struct SLaunch {
 };
 
 template <typename R, int NF>
-inline SLaunch make_slaunch(const Geom& g, int sx) {
+inline SLaunch make_slaunch(const Geom& g, int sx, int tyw) {
   SLaunch L;
   L.sx = sx;
-  L.nsy = (g.Y + TY - 1) / TY;
+  L.nsy = (g.Y + tyw - 1) / tyw;
   L.nblocks = ((g.X + sx - 1) / sx) * L.nsy;
   L.per_xcd = (L.nblocks + 7) / 8;
   L.grid = 8 * L.per_xcd;
@@ -107,8 +126,8 @@ __device__ __forceinline__ void st(void* base, int a, unsigned N, unsigned boff,
 // Stencils of component a at ring column cy; sl[0..4] are the ring slots of
 // rows x-2 .. x+2.  Same formulas and masks as window_derivs; INTERIOR (every
 // mask true) drops the selects.
-template <bool INTERIOR, typename T, typename R, int NFR, int RPM>
-__device__ __forceinline__ void ring_derivs(const Ring<R, NFR, RPM>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
+template <bool INTERIOR, typename T, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_derivs(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                             const SiteInfo& s, T& f, T d1[2], T d2[3]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23), c43 = T(SS_C43);
   const T c = T(q.v[a][sl[2]][cy]);
@@ -135,8 +154,8 @@ __device__ __forceinline__ void ring_derivs(const Ring<R, NFR, RPM>& q, int a, c
   }
 }
 
-template <bool INTERIOR, typename T, typename R, int NFR, int RPM>
-__device__ __forceinline__ void ring_d1(const Ring<R, NFR, RPM>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
+template <bool INTERIOR, typename T, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                         const SiteInfo& s, T& f, T d1[2]) {
   const T c12 = T(SS_C12), c23 = T(SS_C23);
   const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
@@ -163,17 +182,17 @@ struct StreamPrefetch {
   R la[RPW + 1][NF], lc[RPW + 1][NF];  // fused 4+1 only: l1 and l3
 };
 
-template <typename R, int NFR, int RPM>
+template <typename R, int NFR, int SC>
 __device__ __forceinline__ void group_site(int k, int& r, int& c) {
-  using S = StreamShape<R, NFR, RPM>;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  using S = StreamShape<R, NFR, SC>;
+  const int lane = threadIdx.x & 63, rw = row_wave<S>();
   if (k < S::RPW) {
-    r = (w * S::RPW + k + S::RB - 2) & (S::RB - 1);
-    c = lane + 2;
+    r = (rw * S::RPW + k + S::RB - 2) & (S::RB - 1);
+    c = col_wave<S>() * TY + lane + 2;
   } else {
     r = threadIdx.x >> 2;
     const int qd = threadIdx.x & 3;
-    c = qd < 2 ? qd : TY + qd;
+    c = qd < 2 ? qd : S::TYW + qd;
   }
 }
 
@@ -182,17 +201,17 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
 // element 0 and is replaced by 0, so no load sits behind a branch.  PASS
 // selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 41 F, V,
 // l2, l1, l3.
-template <int PASS, typename R, int NF, int NFR, int RPM>
-__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, RPM>::RPW>& p, const PassBufs& b,
+template <int PASS, typename R, int NF, int NFR, int SC>
+__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
-  using S = StreamShape<R, NFR, RPM>;
+  using S = StreamShape<R, NFR, SC>;
   const void* L = (PASS == 4 || PASS == 41) ? b.l2 : b.l1;
   const unsigned NP = (unsigned)g.N;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
     if (PASS == 41 && k == S::RPW) break;  // the 4+1 sweep loads its halo-column sites when it stores them
     int r, c;
-    group_site<R, NFR, RPM>(k, r, c);
+    group_site<R, NFR, SC>(k, r, c);
     const int x = rb + r, y = y0 - 2 + c;
     const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
     const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
@@ -250,16 +269,16 @@ __device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
 
 // Write a group into the ring; slot0 is the ring slot of its first row.  PASS
 // 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to NF..2NF-1.
-template <int PASS, typename R, int NF, int NFR, int RPM>
-__device__ __forceinline__ void group_store(Ring<R, NFR, RPM>& q,
-                                            const StreamPrefetch<R, NF, StreamShape<R, NFR, RPM>::RPW>& p, int slot0,
+template <int PASS, typename R, int NF, int NFR, int SC>
+__device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
+                                            const StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, int slot0,
                                             R dt) {
-  using S = StreamShape<R, NFR, RPM>;
+  using S = StreamShape<R, NFR, SC>;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
     if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
     int r, c;
-    group_site<R, NFR, RPM>(k, r, c);
+    group_site<R, NFR, SC>(k, r, c);
     const int slot = (slot0 + r) & (S::NR - 1);
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
@@ -277,19 +296,19 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, RPM>& q,
 // NF..2NF-1, both from the group's loads (rows from rb, columns from y0 - 2);
 // the halo-column sites (threads < 4 RB) are loaded here, not prefetched,
 // which keeps five arrays of one site out of the prefetch registers.
-template <class TH, typename R, int NFR, int RPM>
-__device__ __forceinline__ void group_store41(Ring<R, NFR, RPM>& q,
-                                              const StreamPrefetch<R, TH::NF, StreamShape<R, NFR, RPM>::RPW>& p,
+template <class TH, typename R, int NFR, int SC>
+__device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q,
+                                              const StreamPrefetch<R, TH::NF, StreamShape<R, NFR, SC>::RPW>& p,
                                               int slot0, R dt, const typename TH::TP& tp, const PassBufs& b,
                                               const Geom& g, int rb, int y0, bool kill) {
-  using S = StreamShape<R, NFR, RPM>;
+  using S = StreamShape<R, NFR, SC>;
   constexpr int NF = TH::NF;
   const unsigned NP = (unsigned)g.N;
 #pragma unroll
   for (int k = 0; k <= S::RPW; ++k) {
     if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
     int r, c;
-    group_site<R, NFR, RPM>(k, r, c);
+    group_site<R, NFR, SC>(k, r, c);
     const int slot = (slot0 + r) & (S::NR - 1);
     const int x = rb + r, y = y0 - 2 + c;
     const bool in = x >= 0 && x < g.X && y >= 0 && y < g.Y;
@@ -320,22 +339,23 @@ __device__ __forceinline__ void group_store41(Ring<R, NFR, RPM>& q,
   }
 }
 
-// The RPW sites of this thread in one iteration (rows c0 + w*RPW + k), from
+// The RPW sites of this thread in one iteration (rows c0 + rw*RPW + k, rw its
+// row wave; column y), from
 // the stage field in ring components AOFF .. AOFF+NF-1.
 // Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied), and
 // in the 4+1 sweep also l1 and l3 from kA / kC;
 // with FOFF >= 0 it takes the new field F' from ring components FOFF ..
 // FOFF+NF-1 (the 4+1 sweep, which builds F' there) instead of forming it.
-template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int RPM>
-__device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const PassBufs& b, const typename TH::TP& tp,
+template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int SC>
+__device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
                                             double& acc1, const R* kA = nullptr, const R* kC = nullptr) {
-  using S = StreamShape<R, NFR, RPM>;
+  using S = StreamShape<R, NFR, SC>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
   const unsigned NP = (unsigned)g.N;
-  const int w = threadIdx.x >> 6;
+  const int w = row_wave<S>();
   // Pass 4 in FP32: l1 and l3 of this thread's sites, issued before the
   // stencils so their latency overlaps the variation (the stores of the loop
   // below would otherwise keep the compiler from hoisting them).  In FP64 the
@@ -434,25 +454,26 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, RPM>& q, const Pa
 // ring, pass 4 runs on the first half (F', V' -> Fout, Vout) and pass 1 on the
 // second (l1 of the next step -> lout, its energy partial -> part2)).
 // Partials: one per strip.
-template <class TH, typename R, int PASS>
+template <class TH, typename R, int PASS, int CW = 1>
 struct StreamRing {
   static constexpr int NFR = (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
   // the 4+1 sweep prefetches five arrays: one row per wave and iteration
-  static constexpr int RPM = PASS == 41 ? 1 : 2;
-  using type = Ring<R, NFR, RPM>;
+  static constexpr int SC = (PASS == 41 ? 1 : 2) | (CW == 2 ? 4 : 0);
+  using shape = StreamShape<R, NFR, SC>;
+  using type = Ring<R, NFR, SC>;
 };
 
-template <class TH, typename R, int PASS>
-__device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::type& q, double* red,
+template <class TH, typename R, int PASS, int CW>
+__device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW>::type& q, double* red,
                                              const PassBufs& b, const typename TH::TP& tp, const Geom& g, int xs,
                                              int sx, int y0, bool kill, int t) {
-  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS>::NFR, RPM = StreamRing<TH, R, PASS>::RPM;
-  using S = StreamShape<R, NFR, RPM>;
+  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS, CW>::NFR, SC = StreamRing<TH, R, PASS, CW>::SC;
+  using S = StreamShape<R, NFR, SC>;
   constexpr int RB = S::RB, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
   const int lane = threadIdx.x & 63;
-  const int y = y0 + lane;
-  const int cy = lane + 2;
+  const int y = y0 + col_wave<S>() * TY + lane;
+  const int cy = col_wave<S>() * TY + lane + 2;
   const int nit = sx / RB;
   (void)kill;
   const StencilK<R> K(g);
@@ -505,7 +526,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
   }
   // Pass 4: V and l2 of the sites this thread computes in the current
   // iteration (keep) and, for the first two compute rows, in the next (hold).
-  const int w = threadIdx.x >> 6;
+  const int w = row_wave<S>();
   R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
   // 4+1 sweep: l1 and l3 of the same sites (its front loads carry them already)
   R keepA[RPW][NF], keepC[RPW][NF], holdA[RPW][NF], holdC[RPW][NF];
@@ -536,7 +557,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     }
   }
   StreamPrefetch<R, NF, RPW> pf;
-  if (nit > 1) group_load<PASS, R, NF, NFR, RPM>(pf, b, g, xs + RB + 2, y0, kill);
+  if (nit > 1) group_load<PASS, R, NF, NFR, SC>(pf, b, g, xs + RB + 2, y0, kill);
 
   PassBufs b2 = b, b3 = b;
   b3.lout = b.lout2;
@@ -546,7 +567,8 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
     __syncthreads();  // ring holds rows xs + it*RB - 2 .. xs + it*RB + RB + 1
     const int c0 = xs + it * RB;
     // every mask true for all rows and columns of this iteration
-    const bool interior = (c0 > g.halo) && (c0 + RB - 1 < g.X - g.halo) && (y0 > g.halo) && (y0 + TY - 1 < g.Y - g.halo);
+    const bool interior =
+        (c0 > g.halo) && (c0 + RB - 1 < g.X - g.halo) && (y0 > g.halo) && (y0 + S::TYW - 1 < g.Y - g.halo);
     if (PASS == 23) {
       if (interior) {
         stream_rows<TH, R, 2, true, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
@@ -578,9 +600,9 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
       // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
       // no row of iteration it+1 other than themselves maps to
       if constexpr (PASS == 41)
-        group_store41<TH, R, NFR, RPM>(q, pf, (it * RB + RB + 4) & (NR - 1), dt, tp, b, g, c0 + RB + 2, y0, kill);
+        group_store41<TH, R, NFR, SC>(q, pf, (it * RB + RB + 4) & (NR - 1), dt, tp, b, g, c0 + RB + 2, y0, kill);
       else
-        group_store<PASS, R, NF, NFR, RPM>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
+        group_store<PASS, R, NF, NFR, SC>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
       if (PASS == 4 || PASS == 41) {
         // next iteration's sites: the group just stored, or (first two compute
         // rows) the hold from the group before; the hold takes the new group.
@@ -602,7 +624,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
           }
         }
       }
-      if (it + 2 < nit) group_load<PASS, R, NF, NFR, RPM>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
+      if (it + 2 < nit) group_load<PASS, R, NF, NFR, SC>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
     }
   }
   if (PASS == 1) {
@@ -620,15 +642,16 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS>::t
   }
 }
 
-template <class TH, typename R, int PASS>
+template <class TH, typename R, int PASS, int CW>
 __global__ void __launch_bounds__(kThreads) stream_pass(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
-  __shared__ typename StreamRing<TH, R, PASS>::type q;
+  __shared__ typename StreamRing<TH, R, PASS, CW>::type q;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, L.nblocks, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
   const int sxi = t / L.nsy, syi = t - sxi * L.nsy;
   const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
-  stream_strip<TH, R, PASS>(q, red, b, tp, g, sxi * L.sx, L.sx, syi * TY, kill, t);
+  stream_strip<TH, R, PASS, CW>(q, red, b, tp, g, sxi * L.sx, L.sx, syi * StreamRing<TH, R, PASS, CW>::shape::TYW,
+                                kill, t);
 }
 
 }  // namespace ss
