@@ -115,8 +115,10 @@ def precision_workload(args):
     chunk the state is converted to FP64 in place and polished for at most
     ``--polish`` steps: if max |EG| < tol it is done, otherwise the state goes
     back to FP32 and the next chunk runs, sized from the decay rate of the
-    polished max |EG| over the last two checks.  The last polish is the final
-    FP64 phase of at most ``--polish`` steps."""
+    polished max |EG| over the last two checks.  When a chunk improves the
+    polished max |EG| by less than 30 %, FP32 has stalled (its steps no longer
+    resolve the remaining increments) and the run finishes in FP64 (the tail,
+    counted with the polish time)."""
     import soliton_solver_amd as ssa
     import torch
     n = args.n
@@ -144,6 +146,7 @@ def precision_workload(args):
     t32 = tpol = 0.0
     eap = np.inf
     chunk, checks = args.chunk, []
+    fp64_tail = 0
     while True:
         s32.use_graphs(args.graph_steps)
         E, _, ea32, dn, dt_ = _chunk(s32, E, min(chunk, args.max_steps - n32))
@@ -161,8 +164,18 @@ def precision_workload(args):
               f"max|EG|={eap:.2e}", file=sys.stderr, flush=True)
         if eap < tol or n32 >= args.max_steps:
             break
-        # next chunk: the steps the last two checks' geometric decay predicts, within [chunk/10, chunk]
         checks.append((n32 + npol_total, eap))
+        if len(checks) >= 2 and checks[-1][1] > 0.7 * checks[-2][1]:
+            # FP32 has stalled (its steps no longer resolve the remaining
+            # increments): finish in FP64
+            t0 = time.perf_counter()
+            Ep, eap, ntail, _ = relax_timed(s32, Ep, tol, max(0, args.max_steps - n32), args.check_every)
+            tpol += time.perf_counter() - t0
+            npol_total += ntail
+            fp64_tail = ntail
+            print(f"[bench] fp64 tail: {ntail} steps: E={Ep:.12f}, max|EG|={eap:.2e}", file=sys.stderr, flush=True)
+            break
+        # next chunk: the steps the last two checks' geometric decay predicts, within [chunk/10, chunk]
         if len(checks) >= 2 and checks[-1][1] < checks[-2][1]:
             (n0, e0), (n1, e1) = checks[-2], checks[-1]
             need = np.log(tol / e1) / (np.log(e1 / e0) / (n1 - n0))
@@ -195,8 +208,8 @@ def precision_workload(args):
         "converged": bool(ea64 < tol and eap < tol),
         "fp64_form": {"steps": n64, "seconds": t64, "energy": E64, "err_absmax": ea64},
         "fp32_phase": {"steps": n32, "seconds": t32, "polish_checks": nchecks},
-        "fp64_polish": {"steps_total": npol_total, "steps_final": npol, "seconds": tpol, "energy": Ep,
-                        "err_absmax": eap},
+        "fp64_polish": {"steps_total": npol_total, "steps_final": npol, "tail_steps": fp64_tail, "seconds": tpol,
+                        "energy": Ep, "err_absmax": eap},
         "skyrmion_number": {"fp64_form": Q64, "fp32_polished": Qp, "identical_rounded": round(Q64) == round(Qp)},
         "relE_vs_fp64_form": dE,
         "max_dm_vs_fp64_form": dm,
