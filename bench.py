@@ -218,6 +218,97 @@ def precision_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def numpy_oracle_rates(gl_steps, cm_steps):
+    """The NumPy oracle (single thread), first steps of the two small reference
+    workloads: Abelian Higgs 128^2 (spacing 0.2, dt 0.1, tanh(r) e^{i theta}
+    vortex + U(-1e-3, 1e-3)) and chiral magnet 320^2 (xsize 10, Heusler,
+    demag, one anti-skyrmion of radius 1 + N(0, 1e-6))."""
+    import soliton_solver_amd as ssa
+    from oracle import core as ocore
+    from oracle.theories import chiral_magnet as o_cm
+    from oracle.theories import ginzburg_landau as o_gl
+    out = {}
+    cases = [
+        ("abelian_higgs_128", "Abelian Higgs", o_gl, gl_steps,
+         dict(xlen=128, ylen=128, xsize=0.2 * 127, ysize=0.2 * 127, q=1.0, Lambda=1.0, u1=1.0, vortex_number=1.0,
+              time_step=0.1),
+         {"mode": "vortex", "noise": 1e-3, "seed": 0}),
+        ("chiral_magnet_320", "Chiral magnet", o_cm, cm_steps,
+         dict(xlen=320, ylen=320, xsize=10.0, ysize=10.0, J=40e-12, K=0.8e6, D=4e-3, M=580e3, B=0.0,
+              dmi_term="Heusler", demag=True, unit_magnetization=True),
+         {"mode": "skyrmions", "ansatz": "anti", "skyrmions": [(5.0, 5.0, 1.0)], "noise": 1e-6, "seed": 1}),
+    ]
+    for key, name, o, steps, kw, cfg in cases:
+        if steps <= 0:
+            continue
+        th = ssa.load_theory(name)
+        p_i, p_f = th.params.pack_device_params(th.params.default_params(**kw).resolved())
+        p_i = np.ascontiguousarray(p_i, dtype=np.int32)
+        p_f = np.ascontiguousarray(p_f, dtype=np.float64)
+        X, Y, nf = int(p_i[0]), int(p_i[1]), int(p_i[4])
+        g = np.empty((2, X, Y))
+        g[0] = p_f[2] * np.arange(X)[:, None]
+        g[1] = p_f[3] * np.arange(Y)[None, :]
+        F = np.zeros((nf, X, Y))
+        V = np.zeros_like(F)
+        th.initial_config.initialize(Field=F, Velocity=V, grid=g, p_i_h=p_i, p_f_h=p_f, config=cfg)
+        E0 = ocore.theory_energy(o, F, p_i, p_f)
+        t0 = time.perf_counter()
+        ocore.relax(o, F, V, p_i, p_f, steps, prev_energy=E0)
+        dt = time.perf_counter() - t0
+        out[key] = {"steps": steps, "seconds": dt, "updates_per_s": X * Y * steps / dt}
+    return out
+
+
+# rocprofv3 byte counters against a plain 1 GiB device copy on MI355X
+# (bench.py --copy-calibration): FETCH_SIZE reports 0.5 KiB per KiB read,
+# WRITE_SIZE 1.0 KiB per KiB written.
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
+
+
+def measure_traffic(child_args, kernel):
+    """HBM bytes per launch of ``kernel`` (a kernel name as rocprof prints it,
+    without namespace and arguments): two ``rocprofv3 --pmc`` runs (FETCH_SIZE,
+    then WRITE_SIZE) of a 3-step child ``bench.py --profile-only`` with the
+    same shapes, each scaled by the copy calibration.  None when rocprofv3 is
+    absent or a run fails."""
+    import glob
+    import re
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    kib = {}
+    with tempfile.TemporaryDirectory() as d:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(d, ctr.lower())
+            cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", ctr, "-d", out, "-o", "t", "--", sys.executable,
+                   os.path.abspath(__file__), "--profile-only", "--no-traffic", *child_args, "--steps", "3",
+                   "--warmup", "1", "--graph-steps", "0"]
+            print(f"[bench] traffic: rocprofv3 --pmc {ctr} (3 steps)", file=sys.stderr, flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd=ROOT)
+            dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
+            if r.returncode != 0 or not dbs:
+                print(f"[bench] traffic: rocprofv3 run failed (rc {r.returncode})", file=sys.stderr, flush=True)
+                return None
+            con = sqlite3.connect(dbs[0])
+            rows = con.execute("select kernel_name, avg(value) from counters_collection where counter_name = ? "
+                               "group by kernel_name", (ctr,)).fetchall()
+            con.close()
+            for name, v in rows:
+                short = re.sub(r"\(.*$", "", re.sub(r"^void ", "", name)).replace("ss::", "")
+                if short == kernel:
+                    kib[ctr] = float(v)
+    if len(kib) != 2:
+        return None
+    return {"bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024 + kib["WRITE_SIZE"] * WRITE_SCALE * 1024,
+            "read_bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024, "write_bytes": kib["WRITE_SIZE"] * WRITE_SCALE * 1024}
+
+
 def _chunk(sim, E, nsteps):
     """``nsteps`` steps in one timed call; returns (E, err, err_absmax, steps, seconds)."""
     import torch
@@ -298,6 +389,9 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=1024, help="side of the CPU-baseline sample lattice")
     ap.add_argument("--cpu-steps", type=int, default=150, help="steps of the CPU-baseline sample (~10 s on 16 cores)")
+    ap.add_argument("--numpy-steps", default="200,40",
+                    help="first steps of the single-thread NumPy oracle on the Abelian-Higgs 128^2 and chiral 320^2 "
+                         "workloads (0 skips one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape of every pass")
     ap.add_argument("--tiles", default="", help="tile variants of passes 1-4 for the FP32 form, e.g. 1,1,1,2 "
@@ -318,6 +412,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="precision workload: max |EG| to reach")
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--check-every", type=int, default=500)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM bytes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -438,6 +534,23 @@ def main():
                "sample": f"C++/OpenMP FP64 restatement of the reference path (unfused, derivative scratch), "
                          f"chiral magnet {m}x{m} same parameters, {args.cpu_steps} steps, {tc:.1f} s; "
                          f"host {model or platform.processor()}, os.cpu_count()={os.cpu_count()}"}
+        gl_steps, cm_steps = (int(v) for v in args.numpy_steps.split(","))
+        cpu["numpy_oracle_single_thread"] = numpy_oracle_rates(gl_steps, cm_steps)
+        print(f"[bench] numpy oracle: {cpu['numpy_oracle_single_thread']}", file=sys.stderr, flush=True)
+
+    v4 = tiles32["variants"][3]
+    if f41 and v4 < 3:
+        v4 = 3
+    kernel = (f"stream_pass<ChiralMagnet, float, {41 if f41 else 4}, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
+              f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>")
+    traffic = None
+    if not args.no_traffic and not args.profile_only:
+        child = ["--tiles", ",".join(str(v) for v in tiles32["variants"]),
+                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)), "--n", str(n)]
+        traffic = measure_traffic(child, kernel)
+        if traffic:
+            print(f"[bench] traffic of {kernel}: {traffic['bytes'] / 1e9:.3f} GB per launch "
+                  f"({traffic['bytes'] / alg_pass4:.2f}x algorithmic)", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -464,11 +577,9 @@ def main():
                    "parallelism": "single GPU"},
         "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": (f"stream_pass<ChiralMagnet, float, {41 if f41 else 4}> "
-                                f"(strips of {tiles32['rows'][3]} rows)"
-                                if tiles32["variants"][3] >= 3 else
-                                f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>"),
+                     "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_read_write": [traffic["read_bytes"], traffic["write_bytes"]] if traffic else None,
+                     "kernel": kernel + (f" (strips of {tiles32['rows'][3]} rows)" if v4 >= 3 else ""),
                      "alg_bytes_per_launch": alg_pass4, "avg_launch_ms": passes[4],
                      "step_kernels_gbs": step_gbs, "pass_ms": [float(x) for x in passes]},
         "fp64_form": result64,
