@@ -80,13 +80,20 @@ class BatchSimulation:
         return out.reshape(self.field_shape(i))
 
     def initialize(self, configs):
-        """One theory ``initial_config`` dict per instance (host-side generators)."""
+        """One theory ``initial_config`` dict per instance (host-side generators);
+        ``{"mode": "array", "field": F, "velocity": V}`` injects arrays, as for
+        ``Simulation.initialize``."""
         for i, cfg in enumerate(configs):
             shape = self.field_shape(i)
             F = np.zeros(shape)
             V = np.zeros(shape)
-            self.theory.initial_config.initialize(Field=F, Velocity=V, grid=self.grid(i), p_i_h=self.p_i[i],
-                                                  p_f_h=self.p_f[i], config=cfg)
+            if str(cfg.get("mode", "initial")).lower() == "array":
+                F[...] = np.asarray(cfg["field"], dtype=np.float64).reshape(shape)
+                if cfg.get("velocity") is not None:
+                    V[...] = np.asarray(cfg["velocity"], dtype=np.float64).reshape(shape)
+            else:
+                self.theory.initial_config.initialize(Field=F, Velocity=V, grid=self.grid(i), p_i_h=self.p_i[i],
+                                                      p_f_h=self.p_f[i], config=cfg)
             self.set_field(i, F, V)
         self.begin()
 

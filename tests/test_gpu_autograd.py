@@ -36,6 +36,12 @@ CASES = [
      lambda th, pi, pf: H.bs_field(th, pi, pf, seed=3, k=2, amp=1e-2)),
     ("Anisotropic superconductor", lambda th: H.as_params(th, 48, 40, seed=4),
      lambda th, pi, pf: H.as_field(th, pi, pf, seed=4, amp=5e-2)),
+    ("Anyon superconductor", lambda th: H.mcsh_params(th, 42, 36), lambda th, pi, pf: H.mcsh_field(th, pi, pf, seed=5)),
+    ("Ferromagnetic superconductor", lambda th: H.fs_params(th, 40, 34, eta2=0.05),
+     lambda th, pi, pf: H.fs_field(th, pi, pf, seed=6)),
+    ("Spin triplet superconducting ferromagnet", lambda th: H.st_params(th, 40, 34),
+     lambda th, pi, pf: H.st_field(th, pi, pf, seed=6)),
+    ("Liquid crystal", lambda th: H.lc_params(th, 40, 34), lambda th, pi, pf: H.lc_field(th, pi, pf, seed=6)),
 ]
 
 

@@ -94,3 +94,50 @@ def test_batch_graphs_replay_like_eager_and_instances_retire():
         assert a.state()["steps"][i] == sa["steps"][i]
     a.close()
     b.close()
+
+
+_THEORY_BATCHES = {
+    "Chiral magnet": ((lambda th, n, m: H.chiral_params(th, n, m)),
+                      (lambda th, pi, pf, k: H.chiral_field(th, pi, pf, seed=10 + k, rough=True))),
+    "Ginzburg-Landau superconductor": ((lambda th, n, m: H.gl_params(th, n, m)),
+                                       (lambda th, pi, pf, k: H.gl_field(th, pi, pf, seed=10 + k))),
+    "Anisotropic superconductor": ((lambda th, n, m: H.as_params(th, n, m, seed=3)),
+                                   (lambda th, pi, pf, k: H.as_field(th, pi, pf, seed=10 + k))),
+    "Anyon superconductor": ((lambda th, n, m: H.mcsh_params(th, n, m)),
+                             (lambda th, pi, pf, k: H.mcsh_field(th, pi, pf, seed=10 + k))),
+    "Ferromagnetic superconductor": ((lambda th, n, m: H.fs_params(th, n, m, eta2=0.05)),
+                                     (lambda th, pi, pf, k: H.fs_field(th, pi, pf, seed=10 + k))),
+    "Spin triplet superconducting ferromagnet": ((lambda th, n, m: H.st_params(th, n, m)),
+                                                 (lambda th, pi, pf, k: H.st_field(th, pi, pf, seed=10 + k))),
+    "Liquid crystal": ((lambda th, n, m: H.lc_params(th, n, m)),
+                       (lambda th, pi, pf, k: H.lc_field(th, pi, pf, seed=10 + k))),
+}
+
+
+@pytest.mark.parametrize("name", list(_THEORY_BATCHES))
+def test_batch_of_each_theory_matches_oracle(name):
+    """Two instances of different shapes of every batchable theory in one
+    batch (FP64): each follows the oracle's trajectory (energy 1e-10
+    relative, max |dF| 1e-9) over 8 steps."""
+    th = ssa.load_theory(name)
+    mkp, mkf = _THEORY_BATCHES[name]
+    params = [mkp(th, 40, 52), mkp(th, 57, 36)]
+    fields = []
+    for k, p in enumerate(params):
+        p_i, p_f = H.packed(th, p)
+        fields.append(mkf(th, p_i, p_f, k))
+    batch = BatchSimulation(th, params, precision="fp64")
+    batch.initialize([{"mode": "array", "field": F} for F in fields])
+    batch.run(8, tol=0.0)
+    st = batch.state()
+    o = H.ORACLES[name]
+    for i, (p, F0) in enumerate(zip(params, fields)):
+        p_i, p_f = H.packed(th, p)
+        F = F0.copy()
+        V = np.zeros_like(F)
+        Es, _ = ocore.relax(o, F, V, p_i, p_f, 8)
+        dF = np.abs(batch.get_field(i) - F).max()
+        print(f"[batch {name} inst {i}] relE={abs(st['energy'][i] - Es[-1]) / abs(Es[-1]):.2e} max|dF|={dF:.2e}")
+        assert abs(st["energy"][i] - Es[-1]) <= 1e-10 * abs(Es[-1])
+        assert dF <= 1e-9
+    batch.close()
