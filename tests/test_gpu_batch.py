@@ -101,7 +101,8 @@ _THEORY_BATCHES = {
                       (lambda th, pi, pf, k: H.chiral_field(th, pi, pf, seed=10 + k, rough=True))),
     "Ginzburg-Landau superconductor": ((lambda th, n, m: H.gl_params(th, n, m)),
                                        (lambda th, pi, pf, k: H.gl_field(th, pi, pf, seed=10 + k))),
-    "Anisotropic superconductor": ((lambda th, n, m: H.as_params(th, n, m, seed=3)),
+    # xsize 16: at the default 25 the coarser 40-row lattice (h = 0.64) blows up within 8 steps, oracle included
+    "Anisotropic superconductor": ((lambda th, n, m: H.as_params(th, n, m, seed=3, xsize=16.0)),
                                    (lambda th, pi, pf, k: H.as_field(th, pi, pf, seed=10 + k))),
     "Anyon superconductor": ((lambda th, n, m: H.mcsh_params(th, n, m)),
                              (lambda th, pi, pf, k: H.mcsh_field(th, pi, pf, seed=10 + k))),
