@@ -344,6 +344,9 @@ def batch_workload(args):
     bs = BatchSimulation(th, params, precision="fp32")
     bs.initialize([{"mode": "initial", "ansatz": "neel", "skyrmion_number": int(q), "noise": 1e-3, "seed": i}
                    for i, q in enumerate(charge)])
+    tinfo = bs.autotune_tiles(reps=5)
+    print(f"[bench] batch tile variant {tinfo['variant']} ({tinfo['rows']} rows), ms per variant "
+          + " ".join(f"{x:.3f}" for x in tinfo["ms"]), file=sys.stderr, flush=True)
     bs.use_graphs(20)
     sites = np.array([int(n) * int(n) for n in sizes], dtype=np.float64)
     torch.cuda.synchronize()
@@ -370,7 +373,7 @@ def batch_workload(args):
         "data": "synthetic",
         "config": {"workload": f"{ninst} baby Skyrme relaxations, sizes U{{64,128,256,512}}^2, seed {args.seed}, "
                                f"until max|EG| < {args.tol:g}", "instances": ninst,
-                   "total_sites": int(sites.sum()), "tiles": bs.num_tiles},
+                   "total_sites": int(sites.sum()), "tiles": bs.num_tiles, "tile_rows": tinfo["rows"]},
         "time_to_converge_s": wall,
         "converged": int((st["converged_step"] >= 0).sum()),
         "steps_max": int(st["steps"].max()),
@@ -409,7 +412,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--tol", type=float, default=1e-6, help="precision workload: max |EG| to reach")
+    ap.add_argument("--tol", type=float, default=None,
+                    help="max |EG| to reach (precision workload: 1e-6; batch workload: 1e-3, the FP32 floor)")
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--check-every", type=int, default=500)
     ap.add_argument("--no-traffic", action="store_true",
@@ -444,9 +448,11 @@ def main():
     torch.cuda.init()
 
     if args.workload == "precision":
+        args.tol = 1e-6 if args.tol is None else args.tol
         precision_workload(args)
         return
     if args.workload == "batch":
+        args.tol = 1e-3 if args.tol is None else args.tol
         batch_workload(args)
         return
 

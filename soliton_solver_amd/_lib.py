@@ -102,6 +102,9 @@ SIGNATURES = {
     "ss_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_batch_state": (C.c_int, [C.c_void_p, _f64p, _f64p, _i32p, _i32p]),
     "ss_batch_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "ss_batch_autotune_tiles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _f64p]),
+    "ss_batch_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_batch_tile_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -100,6 +100,25 @@ class BatchSimulation:
     def begin(self):
         self._check(_lib.lib().ss_batch_begin(self._b))
 
+    def autotune_tiles(self, reps: int = 10) -> dict:
+        """Time the four RK4 passes in each tile variant (LDS tiles of the default
+        rows, halved, quartered) on the live buffers and keep the fastest; the
+        relaxation state is unchanged."""
+        chosen = C.c_int()
+        ms = np.zeros(3)
+        self._check(_lib.lib().ss_batch_autotune_tiles(self._b, int(reps), C.byref(chosen), ms))
+        info = self.tile_variant()
+        info["ms"] = ms
+        return info
+
+    def set_tile_variant(self, variant: int) -> None:
+        self._check(_lib.lib().ss_batch_set_tile_variant(self._b, int(variant)))
+
+    def tile_variant(self) -> dict:
+        v, rows = C.c_int(), C.c_int()
+        self._check(_lib.lib().ss_batch_tile_variant(self._b, C.byref(v), C.byref(rows)))
+        return {"variant": v.value, "rows": rows.value}
+
     def use_graphs(self, steps_per_graph: int):
         self._check(_lib.lib().ss_batch_use_graphs(self._b, int(steps_per_graph)))
 

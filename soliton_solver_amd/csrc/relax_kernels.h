@@ -494,10 +494,10 @@ struct BatchBufs {
   int ntiles, per_xcd;
 };
 
-template <class TH, typename R, int PASS>
+template <class TH, typename R, int PASS, int TXR = 0>
 __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
-  using S = TileShape<R, TH::NF>;
-  __shared__ Tile<R, TH::NF> tile;
+  using S = TileShape<R, TH::NF, TXR>;
+  __shared__ Tile<R, TH::NF, TXR> tile;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
   if (t < 0) return;
@@ -521,15 +521,15 @@ __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
   b.part0 = B.part0;
   b.part1 = B.part1;
   const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
-  stage_tile<TH, R, PASS>(tile, red, b, I.tp, I.g, tx * S::TX, ty * TY, kill, t);
+  stage_tile<TH, R, PASS, TXR>(tile, red, b, I.tp, I.g, tx * S::TX, ty * TY, kill, t);
 }
 
 // Energy partials of every instance (active or not) from its current field.
-template <class TH, typename R>
+template <class TH, typename R, int TXR = 0>
 __global__ void __launch_bounds__(kThreads) batch_energy_pass(BatchBufs B) {
-  using S = TileShape<R, TH::NF>;
+  using S = TileShape<R, TH::NF, TXR>;
   constexpr int NF = TH::NF;
-  __shared__ Tile<R, NF> tile;
+  __shared__ Tile<R, NF, TXR> tile;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
   if (t < 0) return;
@@ -852,23 +852,31 @@ struct TheoryLaunchers {
     memcpy(dst, &I, sizeof(I));
   }
 
-  template <typename R>
-  static hipError_t batch_pass_r(int pass, const BatchBufs& B, hipStream_t st) {
+  template <typename R, int TXR>
+  static hipError_t batch_pass_rv(int pass, const BatchBufs& B, hipStream_t st) {
     const dim3 grid(8 * B.per_xcd), block(kThreads);
     switch (pass) {
-      case 1: hipLaunchKernelGGL((batch_stage_pass<TH, R, 1>), grid, block, 0, st, B); break;
-      case 2: hipLaunchKernelGGL((batch_stage_pass<TH, R, 2>), grid, block, 0, st, B); break;
-      case 3: hipLaunchKernelGGL((batch_stage_pass<TH, R, 3>), grid, block, 0, st, B); break;
-      case 4: hipLaunchKernelGGL((batch_stage_pass<TH, R, 4>), grid, block, 0, st, B); break;
-      case 5: hipLaunchKernelGGL((batch_energy_pass<TH, R>), grid, block, 0, st, B); break;
+      case 1: hipLaunchKernelGGL((batch_stage_pass<TH, R, 1, TXR>), grid, block, 0, st, B); break;
+      case 2: hipLaunchKernelGGL((batch_stage_pass<TH, R, 2, TXR>), grid, block, 0, st, B); break;
+      case 3: hipLaunchKernelGGL((batch_stage_pass<TH, R, 3, TXR>), grid, block, 0, st, B); break;
+      case 4: hipLaunchKernelGGL((batch_stage_pass<TH, R, 4, TXR>), grid, block, 0, st, B); break;
+      case 5: hipLaunchKernelGGL((batch_energy_pass<TH, R, TXR>), grid, block, 0, st, B); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
   }
+  template <typename R>
+  static hipError_t batch_pass_r(int pass, int v, const BatchBufs& B, hipStream_t st) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (v == 2 && r2 != r1) return batch_pass_rv<R, r2>(pass, B, st);
+    if (v >= 1 && r1 != r0) return batch_pass_rv<R, r1>(pass, B, st);
+    return batch_pass_rv<R, r0>(pass, B, st);
+  }
 
-  // pass 1..4: fused RK4 passes; 5: energy partials of every instance.
-  static hipError_t batch_pass(int prec, int pass, const BatchBufs& B, hipStream_t st) {
-    return prec ? batch_pass_r<float>(pass, B, st) : batch_pass_r<double>(pass, B, st);
+  // pass 1..4: fused RK4 passes; 5: energy partials of every instance.  ``v``:
+  // tile variant 0..2 (the tile table of the batch must be built for it).
+  static hipError_t batch_pass(int prec, int pass, int v, const BatchBufs& B, hipStream_t st) {
+    return prec ? batch_pass_r<float>(pass, v, B, st) : batch_pass_r<double>(pass, v, B, st);
   }
 
   // which 0: energy finalize with ``mode`` (0 arrest, 1 report, 2 begin); 1: error finalize with ``tol``.
@@ -903,7 +911,7 @@ struct TheoryOps {
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
-  hipError_t (*batch_pass)(int, int, const BatchBufs&, hipStream_t);
+  hipError_t (*batch_pass)(int, int, int, const BatchBufs&, hipStream_t);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
   int l2_normalize;  // the flow L2-normalises stage fields (unfused normalised path)
 };

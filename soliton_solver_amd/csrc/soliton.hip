@@ -1259,8 +1259,11 @@ struct ss_batch {
   std::vector<Geom> geoms;
   std::vector<long long> offs;
   std::vector<int> tile_begin, ntiles_i;
+  std::vector<unsigned char> tps;  // theory parameters of every instance, 512 bytes each
   long long total = 0;  // elements per field buffer
   int ntiles = 0;
+  int ntcap = 0;        // capacity of the tile table and of each partials row (the most tiles over the variants)
+  int tv = 0;           // tile variant (0..2: default rows, halved, quartered)
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
   void* l[3] = {nullptr, nullptr, nullptr};
@@ -1291,6 +1294,37 @@ static int bfail(ss_batch* b, int code, const char* fmt, ...) {
     if (_e != hipSuccess) return bfail((b), -2, "%s failed: %s", #expr, hipGetErrorString(_e));   \
   } while (0)
 
+// Tile table and instance descriptors of the batch for tile variant ``v``
+// (rows per tile: the variant's), uploaded; the graph is dropped.
+static int batch_tiles(ss_batch* b, int v) {
+  const int isz = b->ops->batch_inst_size();
+  std::vector<unsigned char> host_insts((size_t)isz * b->ninst);
+  std::vector<int> tinst;
+  const int tx = b->ops->shape_rows(b->prec, v);
+  b->tile_begin.clear();
+  b->ntiles_i.clear();
+  b->ntiles = 0;
+  for (int i = 0; i < b->ninst; ++i) {
+    const Geom& g = b->geoms[i];
+    const int nty = (g.Y + TY - 1) / TY;
+    const int nt = ((g.X + tx - 1) / tx) * nty;
+    b->tile_begin.push_back(b->ntiles);
+    b->ntiles_i.push_back(nt);
+    b->ops->batch_fill(host_insts.data() + (size_t)isz * i, g, b->tps.data() + (size_t)512 * i, b->offs[i],
+                       b->ntiles, nt, nty);
+    for (int k = 0; k < nt; ++k) tinst.push_back(i);
+    b->ntiles += nt;
+  }
+  if (b->ntiles > b->ntcap) return bfail(b, -1, "tile table of variant %d exceeds its capacity", v);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  BHIP(b, hipMemcpyAsync(b->tile_inst, tinst.data(), sizeof(int) * tinst.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->insts, host_insts.data(), host_insts.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->tv = v;
+  return 0;
+}
+
 static BatchBufs batch_bufs(ss_batch* b) {
   BatchBufs B{};
   for (int k = 0; k < 2; ++k) {
@@ -1299,7 +1333,7 @@ static BatchBufs batch_bufs(ss_batch* b) {
   }
   for (int k = 0; k < 3; ++k) B.l[k] = b->l[k];
   B.part0 = b->part;
-  B.part1 = b->part + b->ntiles;
+  B.part1 = b->part + b->ntcap;
   B.tile_inst = b->tile_inst;
   B.insts = b->insts;
   B.st = b->st;
@@ -1310,11 +1344,11 @@ static BatchBufs batch_bufs(ss_batch* b) {
 
 static int batch_launch_step(ss_batch* b, double tol) {
   const BatchBufs B = batch_bufs(b);
-  BHIP(b, b->ops->batch_pass(b->prec, 1, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 1, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(0, b->ninst, B, 0, 0.0, b->stream));
-  BHIP(b, b->ops->batch_pass(b->prec, 2, B, b->stream));
-  BHIP(b, b->ops->batch_pass(b->prec, 3, B, b->stream));
-  BHIP(b, b->ops->batch_pass(b->prec, 4, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 2, b->tv, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 3, b->tv, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 4, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(1, b->ninst, B, 0, tol, b->stream));
   return 0;
 }
@@ -1351,10 +1385,7 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
     ss_batch_destroy(b);
     return rc;
   };
-  const int isz = ops->batch_inst_size();
-  std::vector<unsigned char> host_insts((size_t)isz * ninst);
-  std::vector<int> tinst;
-  const int tx = ops->tile_rows(precision);
+  b->tps.assign((size_t)512 * ninst, 0);
   for (int i = 0; i < ninst; ++i) {
     ss_ctx tmp;
     Geom g{};
@@ -1368,22 +1399,20 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
       bfail(b, -1, "instance %d: theory %s is built for %d fields", i, ops->name, ops->nf);
       return bail(-1);
     }
-    alignas(16) unsigned char tp[512];
     char msg[256] = {0};
-    if (ops->make_tp(pi, n_i, pf, n_f, tp, msg, sizeof(msg))) {
+    if (ops->make_tp(pi, n_i, pf, n_f, b->tps.data() + (size_t)512 * i, msg, sizeof(msg))) {
       bfail(b, -1, "instance %d: %s", i, msg);
       return bail(-1);
     }
-    const int nty = (g.Y + TY - 1) / TY;
-    const int nt = ((g.X + tx - 1) / tx) * nty;
     b->geoms.push_back(g);
     b->offs.push_back(b->total);
-    b->tile_begin.push_back(b->ntiles);
-    b->ntiles_i.push_back(nt);
-    ops->batch_fill(host_insts.data() + (size_t)isz * i, g, tp, b->total, b->ntiles, nt, nty);
-    for (int k = 0; k < nt; ++k) tinst.push_back(i);
     b->total += (long long)g.nf * g.N;
-    b->ntiles += nt;
+  }
+  for (int v = 0; v < 3; ++v) {
+    const int tx = ops->shape_rows(precision, v);
+    int nt = 0;
+    for (const Geom& g : b->geoms) nt += ((g.X + tx - 1) / tx) * ((g.Y + TY - 1) / TY);
+    b->ntcap = nt > b->ntcap ? nt : b->ntcap;
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     bfail(b, -2, "hipStreamCreate failed");
@@ -1394,9 +1423,9 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
   for (int k = 0; k < 2 && ok; ++k)
     ok = hipMalloc(&b->F[k], bytes) == hipSuccess && hipMalloc(&b->V[k], bytes) == hipSuccess;
   for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&b->l[k], bytes) == hipSuccess;
-  ok = ok && hipMalloc(&b->part, sizeof(double) * 2 * (size_t)b->ntiles) == hipSuccess;
-  ok = ok && hipMalloc(&b->tile_inst, sizeof(int) * (size_t)b->ntiles) == hipSuccess;
-  ok = ok && hipMalloc(&b->insts, host_insts.size()) == hipSuccess;
+  ok = ok && hipMalloc(&b->part, sizeof(double) * 2 * (size_t)b->ntcap) == hipSuccess;
+  ok = ok && hipMalloc(&b->tile_inst, sizeof(int) * (size_t)b->ntcap) == hipSuccess;
+  ok = ok && hipMalloc(&b->insts, (size_t)ops->batch_inst_size() * ninst) == hipSuccess;
   ok = ok && hipMalloc(&b->st, sizeof(BatchState) * ninst) == hipSuccess;
   ok = ok && hipHostMalloc(&b->host_st, sizeof(BatchState) * ninst, hipHostMallocDefault) == hipSuccess;
   if (!ok) {
@@ -1408,15 +1437,12 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
          hipMemsetAsync(b->V[k], 0, bytes, b->stream) == hipSuccess;
   for (int k = 0; k < 3 && ok; ++k) ok = hipMemsetAsync(b->l[k], 0, bytes, b->stream) == hipSuccess;
   ok = ok && hipMemsetAsync(b->st, 0, sizeof(BatchState) * ninst, b->stream) == hipSuccess;
-  ok = ok && hipMemcpyAsync(b->tile_inst, tinst.data(), sizeof(int) * tinst.size(), hipMemcpyHostToDevice,
-                            b->stream) == hipSuccess;
-  ok = ok && hipMemcpyAsync(b->insts, host_insts.data(), host_insts.size(), hipMemcpyHostToDevice, b->stream) ==
-                 hipSuccess;
   ok = ok && hipStreamSynchronize(b->stream) == hipSuccess;
   if (!ok) {
     bfail(b, -2, "initialisation of the batch failed");
     return bail(-2);
   }
+  if (batch_tiles(b, 0)) return bail(-2);
   *out = b;
   return 0;
 }
@@ -1489,7 +1515,7 @@ int ss_batch_download(ss_batch* b, int inst, int buf, double* host, size_t n) {
 int ss_batch_begin(ss_batch* b) {
   if (!b) return bfail(nullptr, -1, "null batch");
   const BatchBufs B = batch_bufs(b);
-  BHIP(b, b->ops->batch_pass(b->prec, 5, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 5, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(0, b->ninst, B, 2, 0.0, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   return 0;
@@ -1553,7 +1579,7 @@ int ss_batch_step(ss_batch* b, int nsteps, double tol, int check_every, int* n_a
 int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps, int* converged_step) {
   if (!b) return bfail(nullptr, -1, "null batch");
   const BatchBufs B = batch_bufs(b);
-  BHIP(b, b->ops->batch_pass(b->prec, 5, B, b->stream));
+  BHIP(b, b->ops->batch_pass(b->prec, 5, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(0, b->ninst, B, 1, 0.0, b->stream));
   if (batch_read_state(b)) return -2;
   for (int i = 0; i < b->ninst; ++i) {
@@ -1567,7 +1593,65 @@ int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps
 
 int ss_batch_memory_bytes(ss_batch* b, size_t* bytes) {
   if (!b || !bytes) return bfail(b, -1, "null argument");
-  *bytes = 7 * (size_t)b->total * elem_size(b->prec) + sizeof(double) * 2 * (size_t)b->ntiles;
+  *bytes = 7 * (size_t)b->total * elem_size(b->prec) + sizeof(double) * 2 * (size_t)b->ntcap;
+  return 0;
+}
+
+// Times the four RK4 passes of the batch in each tile variant on the live
+// buffers (inactive instances are skipped, as in a step) and keeps the
+// fastest.  The passes write only l_k, the partials and the next-parity F, V,
+// so the relaxation state is unchanged.
+int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (reps < 1) return bfail(b, -1, "reps must be >= 1");
+  hipEvent_t e0, e1;
+  BHIP(b, hipEventCreate(&e0));
+  BHIP(b, hipEventCreate(&e1));
+  int best_v = 0;
+  float best = 0.f;
+  int rc = 0;
+  for (int v = 0; v < 3 && rc == 0; ++v) {
+    rc = batch_tiles(b, v);
+    if (rc) break;
+    const BatchBufs B = batch_bufs(b);
+    hipError_t e = hipSuccess;
+    for (int p = 1; p <= 4 && e == hipSuccess; ++p) e = b->ops->batch_pass(b->prec, p, v, B, b->stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+      for (int p = 1; p <= 4 && e == hipSuccess; ++p) e = b->ops->batch_pass(b->prec, p, v, B, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e != hipSuccess) {
+      rc = bfail(b, -2, "batch tile autotuning of variant %d failed: %s", v, hipGetErrorString(e));
+      break;
+    }
+    t /= reps;
+    if (ms) ms[v] = t;
+    if (v == 0 || t < best) {
+      best = t;
+      best_v = v;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if (batch_tiles(b, best_v)) return -2;
+  if (chosen) *chosen = best_v;
+  return 0;
+}
+
+int ss_batch_set_tile_variant(ss_batch* b, int variant) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (variant < 0 || variant > 2) return bfail(b, -1, "batch tile variant is %d (valid: 0 .. 2)", variant);
+  return batch_tiles(b, variant);
+}
+
+int ss_batch_tile_variant(ss_batch* b, int* variant, int* rows) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (variant) *variant = b->tv;
+  if (rows) *rows = b->ops->shape_rows(b->prec, b->tv);
   return 0;
 }
 

@@ -142,3 +142,35 @@ def test_batch_of_each_theory_matches_oracle(name):
         assert abs(st["energy"][i] - Es[-1]) <= 1e-10 * abs(Es[-1])
         assert dF <= 1e-9
     batch.close()
+
+
+def test_batch_autotune_keeps_state_and_trajectory():
+    """Autotuning the batch's tile shape leaves the state alone, and a run in
+    the chosen shape follows the default-shape run (FP64: 1e-12 relative
+    energy, fields to 1e-12; only partial-sum order differs)."""
+    th, params, cfgs = _bs_instances([48, 100, 64, 130], seed=7)
+    a = BatchSimulation(th, params, precision="fp64")
+    b = BatchSimulation(th, params, precision="fp64")
+    a.initialize(cfgs)
+    b.initialize(cfgs)
+    a.run(5, tol=0.0)
+    b.run(5, tol=0.0)
+    F0 = [a.get_field(i).copy() for i in range(len(params))]
+    info = a.autotune_tiles(reps=3)
+    assert info["variant"] in (0, 1, 2) and len(info["ms"]) == 3 and np.all(info["ms"] > 0)
+    for i in range(len(params)):
+        assert np.array_equal(a.get_field(i), F0[i])
+    a.run(4, tol=0.0)
+    b.run(4, tol=0.0)
+    for v in (2, 1, 0):
+        a.set_tile_variant(v)
+        assert a.tile_variant()["variant"] == v
+        a.run(4, tol=0.0)
+        b.run(4, tol=0.0)
+    sa, sb = a.state(), b.state()
+    for i in range(len(params)):
+        assert abs(sa["energy"][i] - sb["energy"][i]) <= 1e-12 * abs(sb["energy"][i])
+        assert np.abs(a.get_field(i) - b.get_field(i)).max() <= 1e-12
+        assert sa["steps"][i] == sb["steps"][i] == 21
+    a.close()
+    b.close()
