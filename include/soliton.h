@@ -258,11 +258,12 @@ int ss_batch_use_graphs(ss_batch* b, int steps_per_graph);
 int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps, int* converged_step);
 int ss_batch_memory_bytes(ss_batch* b, size_t* bytes);
 
-/* Tile shape of the batch's RK4 passes (no counterpart in the reference):
- * variants 0-2 = LDS tiles of the default rows, halved, quartered, one shape
- * for every instance.  ss_batch_autotune_tiles times the four passes in each
- * variant ``reps`` times on the live buffers and keeps the fastest; ms[0..2]
- * = mean time of the four passes per variant.  The relaxation state is
+/* Launch shape of the batch's RK4 passes (no counterpart in the reference):
+ * the single-lattice variants (0-2 LDS tiles of the default rows, halved,
+ * quartered; 3-8 streaming strips of 64/128/256 rows, 64 or 128 columns
+ * wide), one shape for every instance.  ss_batch_autotune_tiles times the
+ * four passes in each variant ``reps`` times on the live buffers and keeps
+ * the fastest; ms[0..8] = mean time of the four passes per variant.  The relaxation state is
  * unchanged; results do not depend on the variant beyond FP64 summation
  * order. */
 int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms);

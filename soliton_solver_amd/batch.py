@@ -101,11 +101,12 @@ class BatchSimulation:
         self._check(_lib.lib().ss_batch_begin(self._b))
 
     def autotune_tiles(self, reps: int = 10) -> dict:
-        """Time the four RK4 passes in each tile variant (LDS tiles of the default
-        rows, halved, quartered) on the live buffers and keep the fastest; the
-        relaxation state is unchanged."""
+        """Time the four RK4 passes in each shape variant (0-2 LDS tiles of the
+        default rows, halved, quartered; 3-8 streaming strips, as for a single
+        lattice) on the live buffers and keep the fastest; the relaxation state is
+        unchanged."""
         chosen = C.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(9)
         self._check(_lib.lib().ss_batch_autotune_tiles(self._b, int(reps), C.byref(chosen), ms))
         info = self.tile_variant()
         info["ms"] = ms

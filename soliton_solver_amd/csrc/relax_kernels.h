@@ -492,6 +492,7 @@ struct BatchBufs {
   const void* insts;   // BatchInst<TH>[ninst]
   BatchState* st;
   int ntiles, per_xcd;
+  int sx;              // streaming shapes: rows per strip
 };
 
 template <class TH, typename R, int PASS, int TXR = 0>
@@ -522,6 +523,40 @@ __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
   b.part1 = B.part1;
   const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
   stage_tile<TH, R, PASS, TXR>(tile, red, b, I.tp, I.g, tx * S::TX, ty * TY, kill, t);
+}
+
+// Streaming form of the batched passes: the table entries are strips of
+// B.sx rows by the shape's width, numbered per instance row-band by
+// row-band.  ENERGY: pass 1 on every instance, active or not, for its energy
+// partials (its l1 output is scratch).
+template <class TH, typename R, int PASS, int CW, bool ENERGY = false>
+__global__ void __launch_bounds__(kThreads) batch_stream_pass(BatchBufs B) {
+  __shared__ typename StreamRing<TH, R, PASS, CW>::type q;
+  __shared__ double red[kWaves];
+  constexpr int TYW = StreamRing<TH, R, PASS, CW>::shape::TYW;
+  const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
+  if (t < 0) return;
+  const int i = B.tile_inst[t];
+  const BatchState& st = B.st[i];
+  if (!ENERGY && !st.active) return;  // uniform per block
+  const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
+  const int lt = t - I.tile_begin;
+  const int sxi = lt / I.ntiles_y, syi = lt - sxi * I.ntiles_y;
+  const int cur = st.cur;
+  const size_t es = sizeof(R);
+  PassBufs b{};
+  b.F = (const char*)B.F[cur] + I.off * es;
+  b.V = (const char*)B.V[cur] + I.off * es;
+  b.l1 = (const char*)B.l[0] + I.off * es;
+  b.l2 = (const char*)B.l[1] + I.off * es;
+  b.l3 = (const char*)B.l[2] + I.off * es;
+  b.lout = PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + I.off * es : nullptr;
+  b.Fout = (char*)B.F[cur ^ 1] + I.off * es;
+  b.Vout = (char*)B.V[cur ^ 1] + I.off * es;
+  b.part0 = B.part0;
+  b.part1 = B.part1;
+  const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
+  stream_strip<TH, R, PASS, CW>(q, red, b, I.tp, I.g, sxi * B.sx, B.sx, syi * TYW, kill, t);
 }
 
 // Energy partials of every instance (active or not) from its current field.
@@ -873,10 +908,46 @@ struct TheoryLaunchers {
     return batch_pass_rv<R, r0>(pass, B, st);
   }
 
+  template <typename R, int CW>
+  static hipError_t batch_stream_rv(int pass, const BatchBufs& B, hipStream_t st) {
+    const dim3 grid(8 * B.per_xcd), block(kThreads);
+    switch (pass) {
+      case 1: hipLaunchKernelGGL((batch_stream_pass<TH, R, 1, CW>), grid, block, 0, st, B); break;
+      case 2: hipLaunchKernelGGL((batch_stream_pass<TH, R, 2, CW>), grid, block, 0, st, B); break;
+      case 3: hipLaunchKernelGGL((batch_stream_pass<TH, R, 3, CW>), grid, block, 0, st, B); break;
+      case 4: hipLaunchKernelGGL((batch_stream_pass<TH, R, 4, CW>), grid, block, 0, st, B); break;
+      case 5: hipLaunchKernelGGL((batch_stream_pass<TH, R, 1, CW, true>), grid, block, 0, st, B); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+
+  // Strip rows and width (columns) of shape variant ``v`` in the batched
+  // form, 0 rows for the tile variants; the width is the effective one (a
+  // two-wavefront variant runs one wavefront wide where its rings do not fit).
+  static void batch_shape(int prec, int v, int* sx, int* tyw) {
+    if (v < 3) {
+      *sx = 0;
+      *tyw = TY;
+      return;
+    }
+    const bool wide = prec ? wide_fits<float>() : wide_fits<double>();
+    *sx = strip_rows(v);
+    *tyw = (strip_cw(v) == 2 && wide) ? 2 * TY : TY;
+  }
+
   // pass 1..4: fused RK4 passes; 5: energy partials of every instance.  ``v``:
-  // tile variant 0..2 (the tile table of the batch must be built for it).
+  // shape variant 0..8 (the tile table of the batch must be built for it).
   static hipError_t batch_pass(int prec, int pass, int v, const BatchBufs& B, hipStream_t st) {
+    if (v >= 3) return prec ? batch_stream_v<float>(pass, v, B, st) : batch_stream_v<double>(pass, v, B, st);
     return prec ? batch_pass_r<float>(pass, v, B, st) : batch_pass_r<double>(pass, v, B, st);
+  }
+  template <typename R>
+  static hipError_t batch_stream_v(int pass, int v, const BatchBufs& B, hipStream_t st) {
+    if constexpr (wide_fits<R>()) {
+      if (strip_cw(v) == 2) return batch_stream_rv<R, 2>(pass, B, st);
+    }
+    return batch_stream_rv<R, 1>(pass, B, st);
   }
 
   // which 0: energy finalize with ``mode`` (0 arrest, 1 report, 2 begin); 1: error finalize with ``tol``.
@@ -912,6 +983,7 @@ struct TheoryOps {
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
   void (*batch_fill)(void*, const Geom&, const void*, long long, int, int, int);
   hipError_t (*batch_pass)(int, int, int, const BatchBufs&, hipStream_t);
+  void (*batch_shape)(int, int, int*, int*);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
   int l2_normalize;  // the flow L2-normalises stage fields (unfused normalised path)
 };
@@ -936,6 +1008,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::init,                       \
                                       &ss::TheoryLaunchers<TH>::batch_fill,                 \
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
+                                      &ss::TheoryLaunchers<TH>::batch_shape,                \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \
                                       TH::kL2Normalize ? 1 : 0};                            \
     return &ops;                                                                            \

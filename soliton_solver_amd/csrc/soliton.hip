@@ -1263,7 +1263,8 @@ struct ss_batch {
   long long total = 0;  // elements per field buffer
   int ntiles = 0;
   int ntcap = 0;        // capacity of the tile table and of each partials row (the most tiles over the variants)
-  int tv = 0;           // tile variant (0..2: default rows, halved, quartered)
+  int tv = 0;           // shape variant (0..2 LDS tiles, 3..8 streaming strips, as for a single lattice)
+  int sx = 0;           // streaming variants: rows per strip
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
   void* l[3] = {nullptr, nullptr, nullptr};
@@ -1300,13 +1301,15 @@ static int batch_tiles(ss_batch* b, int v) {
   const int isz = b->ops->batch_inst_size();
   std::vector<unsigned char> host_insts((size_t)isz * b->ninst);
   std::vector<int> tinst;
-  const int tx = b->ops->shape_rows(b->prec, v);
+  int sx = 0, tyw = TY;
+  b->ops->batch_shape(b->prec, v, &sx, &tyw);
+  const int tx = sx ? sx : b->ops->shape_rows(b->prec, v);  // rows per strip or per tile
   b->tile_begin.clear();
   b->ntiles_i.clear();
   b->ntiles = 0;
   for (int i = 0; i < b->ninst; ++i) {
     const Geom& g = b->geoms[i];
-    const int nty = (g.Y + TY - 1) / TY;
+    const int nty = (g.Y + tyw - 1) / tyw;
     const int nt = ((g.X + tx - 1) / tx) * nty;
     b->tile_begin.push_back(b->ntiles);
     b->ntiles_i.push_back(nt);
@@ -1322,6 +1325,7 @@ static int batch_tiles(ss_batch* b, int v) {
   BHIP(b, hipMemcpyAsync(b->insts, host_insts.data(), host_insts.size(), hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   b->tv = v;
+  b->sx = sx;
   return 0;
 }
 
@@ -1339,6 +1343,7 @@ static BatchBufs batch_bufs(ss_batch* b) {
   B.st = b->st;
   B.ntiles = b->ntiles;
   B.per_xcd = (b->ntiles + 7) / 8;
+  B.sx = b->sx;
   return B;
 }
 
@@ -1408,10 +1413,12 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
     b->offs.push_back(b->total);
     b->total += (long long)g.nf * g.N;
   }
-  for (int v = 0; v < 3; ++v) {
-    const int tx = ops->shape_rows(precision, v);
+  for (int v = 0; v < ops->num_shapes; ++v) {
+    int sx = 0, tyw = TY;
+    ops->batch_shape(precision, v, &sx, &tyw);
+    const int tx = sx ? sx : ops->shape_rows(precision, v);
     int nt = 0;
-    for (const Geom& g : b->geoms) nt += ((g.X + tx - 1) / tx) * ((g.Y + TY - 1) / TY);
+    for (const Geom& g : b->geoms) nt += ((g.X + tx - 1) / tx) * ((g.Y + tyw - 1) / tyw);
     b->ntcap = nt > b->ntcap ? nt : b->ntcap;
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1610,7 +1617,7 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
   int best_v = 0;
   float best = 0.f;
   int rc = 0;
-  for (int v = 0; v < 3 && rc == 0; ++v) {
+  for (int v = 0; v < b->ops->num_shapes && rc == 0; ++v) {
     rc = batch_tiles(b, v);
     if (rc) break;
     const BatchBufs B = batch_bufs(b);
@@ -1644,14 +1651,15 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
 
 int ss_batch_set_tile_variant(ss_batch* b, int variant) {
   if (!b) return bfail(nullptr, -1, "null batch");
-  if (variant < 0 || variant > 2) return bfail(b, -1, "batch tile variant is %d (valid: 0 .. 2)", variant);
+  if (variant < 0 || variant >= b->ops->num_shapes)
+    return bfail(b, -1, "batch shape variant is %d (valid: 0 .. %d)", variant, b->ops->num_shapes - 1);
   return batch_tiles(b, variant);
 }
 
 int ss_batch_tile_variant(ss_batch* b, int* variant, int* rows) {
   if (!b) return bfail(nullptr, -1, "null batch");
   if (variant) *variant = b->tv;
-  if (rows) *rows = b->ops->shape_rows(b->prec, b->tv);
+  if (rows) *rows = b->sx ? b->sx : b->ops->shape_rows(b->prec, b->tv);
   return 0;
 }
 
