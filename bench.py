@@ -506,6 +506,8 @@ def main():
         result64 = {"updates_per_s": ups64, "ms_per_step": t64 / args.steps * 1e3,
                     "hbm_frac_alg": ups64 * 16 * nf * 8 / (HBM_PEAK_GBS * 1e9),
                     "energy": E64, "fp32_vs_fp64_relE_after_steps": relE,
+                    "relE_note": "both forms from the same field after the same timed steps, mid-relaxation; "
+                                 "relaxed energies agree within 4e-9 (bench.py --workload precision)",
                     "tile_rows_per_pass": tiles64["rows"]}
         print(f"[bench] fp64 form: {t64 / args.steps * 1e3:.3f} ms/step, {ups64:.3e} updates/s, "
               f"relE(fp32 vs fp64)={relE:.2e}", file=sys.stderr, flush=True)
