@@ -1,18 +1,20 @@
 // Streaming form of the fused RK4 passes.
 //
-// A workgroup owns a strip of TY = 64 columns (y) by ``sx`` rows (x) and
-// marches down it RB rows at a time.  The stage field of the rows it needs
-// (RB computed rows plus the 2-row halo either side) lives in an LDS ring of
-// NR >= RB + 4 rows (row r in slot (r - x0 + 2) mod NR); after each group of
-// RB rows is computed, the next RB rows are written into slots no longer read.  The
-// global loads for those rows are issued one iteration ahead into registers,
-// so they are in flight while the current rows are computed.  Compared with
-// the tile form, rows are fetched once per strip instead of (TX + 4)/TX times,
-// and the ring is small enough for many resident workgroups per CU.
+// A workgroup owns a strip of 64 or 128 columns (y; one or two wavefronts
+// across) by ``sx`` rows (x) and marches down it RB rows at a time.  The stage
+// field of the rows it needs (RB computed rows plus the 2-row halo either
+// side) lives in an LDS ring of NR >= RB + 4 rows (row r in slot
+// (r - x0 + 2) mod NR); after each group of RB rows is computed, the next RB
+// rows are written into slots no longer read.  The global loads for those
+// rows are issued one iteration ahead into registers, so they are in flight
+// while the current rows are computed.  Compared with the tile form, rows are
+// fetched once per strip instead of (TX + 4)/TX times, and the ring is small
+// enough for several resident workgroups per CU.
 //
 // Per site the arithmetic is the tile form's (same stencils, same order, same
-// theory calls), so the two forms give bitwise the same fields; only the order
-// of the FP64 partial sums differs.
+// theory calls), so the two forms agree to rounding (contraction of the same
+// expressions may differ between the two inlined contexts); the FP64 partial
+// sums differ in order.
 #pragma once
 
 #include "common.h"
@@ -170,10 +172,10 @@ __device__ __forceinline__ void ring_d1(const Ring<R, NFR, SC>& q, int a, const 
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
-// interior sites (column lane) and, for threads < 4*RB, one halo-column site
-// (row tid/4, column 0, 1, 66 or 67 of the ring).  A group starts 2 rows after
-// the first row computed in the next iteration, so the interior row of
-// (wave w, k) is (w*RPW + k - 2) mod RB: each thread loads the sites it
+// interior sites (its column) and, for threads < 4*RB, one halo-column site
+// (row tid/4, ring column 0, 1, TYW+2 or TYW+3).  A group starts 2 rows after
+// the first row computed in the next iteration, so the interior row of (row
+// wave w, k) is (w*RPW + k - 2) mod RB: each thread loads the sites it
 // computes next iteration, except the first two compute rows (w*RPW + k < 2),
 // whose loaders hold them one iteration longer.
 template <typename R, int NF, int RPW>
@@ -418,12 +420,21 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
           const R Vc = kV[k * NF + a];
-          const R a1 = FOFF >= 0 ? kA[k * NF + a] : kPre ? pl1[kPre ? k : 0][a] : ld<R>(plane<R>(b.l1, a, NP), boff);
           const R a2 = kL[k * NF + a];
-          const R a3 = FOFF >= 0 ? kC[k * NF + a] : kPre ? pl3[kPre ? k : 0][a] : ld<R>(plane<R>(b.l3, a, NP), boff);
+          R a1, a3;
+          if constexpr (FOFF >= 0) {  // 4+1 sweep: kept from its front loads
+            a1 = kA[k * NF + a];
+            a3 = kC[k * NF + a];
+          } else if constexpr (kPre) {  // issued before the stencils
+            a1 = pl1[k][a];
+            a3 = pl3[k][a];
+          } else {
+            a1 = ld<R>(plane<R>(b.l1, a, NP), boff);
+            a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
+          }
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          if (FOFF >= 0)
+          if constexpr (FOFF >= 0)
             Fn[a] = q.v[FOFF + a][sl[2]][cy];
           else
             Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
