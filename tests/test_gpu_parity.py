@@ -195,6 +195,47 @@ def test_fp32_form_tracks_fp64():
     b.close()
 
 
+_FP32_CASES = {
+    "Ginzburg-Landau superconductor": (lambda th: H.gl_params(th, 60, 52), lambda th, pi, pf: H.gl_field(th, pi, pf, seed=3)),
+    "Baby Skyrme model": (lambda th: H.bs_params(th, 60, 52), lambda th, pi, pf: H.bs_field(th, pi, pf, seed=3, k=2)),
+    "Anisotropic superconductor": (lambda th: H.as_params(th, 60, 52, seed=2, xsize=16.0),
+                                   lambda th, pi, pf: H.as_field(th, pi, pf, seed=3)),
+    "Anyon superconductor": (lambda th: H.mcsh_params(th, 60, 52), lambda th, pi, pf: H.mcsh_field(th, pi, pf, seed=3)),
+    "Ferromagnetic superconductor": (lambda th: H.fs_params(th, 60, 52, eta2=0.05),
+                                     lambda th, pi, pf: H.fs_field(th, pi, pf, seed=3)),
+    "Spin triplet superconducting ferromagnet": (lambda th: H.st_params(th, 60, 52),
+                                                 lambda th, pi, pf: H.st_field(th, pi, pf, seed=3)),
+    "Liquid crystal": (lambda th: H.lc_params(th, 60, 52), lambda th, pi, pf: H.lc_field(th, pi, pf, seed=3)),
+}
+
+
+@pytest.mark.parametrize("name", list(_FP32_CASES))
+def test_fp32_form_tracks_fp64_for_every_theory(name):
+    """FP32 fields and stencils with FP64 energies: the energy gradient agrees
+    with the FP64 form to FP32 rounding (1e-5 of max|EG|), the energy to 1e-6,
+    and 20 steps from the same field stay within 1e-5 relative energy."""
+    th = ssa.load_theory(name)
+    mkp, mkf = _FP32_CASES[name]
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    a = _sim(th, p, F)
+    b = _sim(th, p, F, precision="fp32")
+    ga, gb = a.energy_gradient(), b.energy_gradient()
+    scale = np.abs(ga).max()
+    E0 = a.compute_energy()
+    print(f"[fp32 {name}] max|dEG|/max|EG|={np.abs(ga - gb).max() / scale:.2e} "
+          f"relE0={H.rel(b.compute_energy(), E0):.2e}")
+    assert np.abs(ga - gb).max() <= 1e-5 * scale
+    assert H.rel(b.compute_energy(), E0) < 1e-6
+    Ea, _, _ = a.steps(E0, 20)
+    Eb, _, _ = b.steps(E0, 20)
+    print(f"[fp32 {name}] after 20 steps relE={H.rel(Eb, Ea):.2e}")
+    assert H.rel(Eb, Ea) < 1e-5
+    a.close()
+    b.close()
+
+
 def test_invalid_arguments_raise():
     th = ssa.load_theory("Chiral magnet")
     with pytest.raises(ValueError):
