@@ -218,6 +218,65 @@ def precision_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def converge_workload(args):
+    """Time-to-converge of the two small FP64 workloads on the GPU: the
+    Abelian-Higgs vortex, 128^2, 2000 steps (energy -> pi), and the chiral
+    magnet, 320^2, relaxed to max |EG| < 1e-8 (the demag saddle makes this
+    ~2e5 steps).  Autotuned shapes, graphs of 20 steps."""
+    import soliton_solver_amd as ssa
+    import torch
+    cases = [
+        ("abelian_higgs_128", "Abelian Higgs",
+         dict(xlen=128, ylen=128, xsize=0.2 * 127, ysize=0.2 * 127, q=1.0, Lambda=1.0, u1=1.0, vortex_number=1.0,
+              time_step=0.1),
+         {"mode": "vortex", "noise": 1e-3, "seed": 0}, 0.0, 2000),
+        ("chiral_magnet_320", "Chiral magnet",
+         dict(xlen=320, ylen=320, xsize=10.0, ysize=10.0, J=40e-12, K=0.8e6, D=4e-3, M=580e3, B=0.0,
+              dmi_term="Heusler", demag=True, unit_magnetization=True),
+         {"mode": "skyrmions", "ansatz": "anti", "skyrmions": [(5.0, 5.0, 1.0)], "noise": 1e-6, "seed": 1},
+         1e-8, 400000),
+    ]
+    res = {}
+    for key, name, kw, cfg, tol, max_steps in cases:
+        th = ssa.load_theory(name)
+        sim = ssa.Simulation(th.params.default_params(**kw), th, precision="fp64")
+        sim.initialize(cfg)
+        tiles = sim.autotune_tiles(reps=5)
+        sim.use_graphs(20)
+        E0 = sim.compute_energy()
+        sim.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if tol > 0.0:
+            E, ea, n = sim.minimize(tol=tol, max_steps=max_steps, check_every=5000, prev_energy=E0)
+        else:
+            E, _, ea = sim.steps(E0, max_steps)
+            n = max_steps
+        sim.synchronize()
+        dt = time.perf_counter() - t0
+        obs = sim.observables()
+        charge = {k: v for k, v in obs.items() if k != "energy"}
+        res[key] = {"seconds": dt, "steps": n, "us_per_step": dt / n * 1e6, "energy": E, "err_absmax": ea,
+                    "tol": tol, "charges": charge, "tile_variants": tiles["variants"], "fused23": tiles["fused23"],
+                    "fused41": tiles["fused41"]}
+        print(f"[bench] {key}: {n} steps in {dt:.2f} s ({dt / n * 1e6:.1f} us/step), E={E:.12f}, "
+              f"max|EG|={ea:.2e}, {charge}", file=sys.stderr, flush=True)
+        sim.close()
+    out = {
+        "metric": "time-to-converge, chiral magnet 320^2 FP64 to max|EG| < 1e-8",
+        "value": res["chiral_magnet_320"]["seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "chiral magnet 320x320, xsize 10, Heusler DMI, demag, one anti-skyrmion of radius 1 "
+                               "+ N(0, 1e-6); Abelian-Higgs vortex 128x128 spacing 0.2, 2000 steps beside it"},
+        "runs": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def numpy_oracle_rates(gl_steps, cm_steps):
     """The NumPy oracle (single thread), first steps of the two small reference
     workloads: Abelian Higgs 128^2 (spacing 0.2, dt 0.1, tanh(r) e^{i theta}
@@ -403,9 +462,10 @@ def main():
     ap.add_argument("--fused41", type=int, default=-1,
                     help="with --tiles: 1 runs pass 4 and the next step's pass 1 as one sweep")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
-    ap.add_argument("--workload", choices=["throughput", "precision", "batch"], default="throughput",
+    ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge"], default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
-                         "batch: many baby-Skyrme relaxations of uneven sizes")
+                         "batch: many baby-Skyrme relaxations of uneven sizes; converge: time-to-converge of the "
+                         "Abelian-Higgs 128^2 and chiral 320^2 FP64 workloads")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--nsk", type=int, default=1, help="precision workload: seeded anti-skyrmions")
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
@@ -454,6 +514,9 @@ def main():
     if args.workload == "batch":
         args.tol = 1e-3 if args.tol is None else args.tol
         batch_workload(args)
+        return
+    if args.workload == "converge":
+        converge_workload(args)
         return
 
     n = args.n
