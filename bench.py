@@ -107,7 +107,10 @@ def relax_timed(sim, E, tol, max_steps, check_every, floor=False):
 
 def precision_workload(args):
     """Time to max |EG| < tol: the plain FP64 form against the FP32 form with
-    FP64 polish, from the same seeded field.
+    FP64 polish, from the same seeded field.  ``--form fp32c`` (the default)
+    runs the FP32 phase with the compensated field update (FP32 hi + lo words,
+    increments added in FP64), which keeps the steps whose increments are
+    below an FP32 ulp of the field; ``--form fp32`` runs the plain FP32 form.
 
     FP32 storage puts a floor under max |EG| (field rounding ~6e-8 over h^2
     ~ 1e-3 gives ~1e-4), but the slow modes keep converging below it.  The
@@ -137,7 +140,8 @@ def precision_workload(args):
     Q64 = s64.observables()["skyrmion_number"]
     s64.close()
 
-    s32 = ssa.Simulation(p, th, precision="fp32")
+    form = args.form
+    s32 = ssa.Simulation(p, th, precision=form)
     s32.initialize({"mode": "array", "field": F0})
     s32.autotune_tiles(reps=3)
     tiles32 = s32.tile_variants()
@@ -160,7 +164,7 @@ def precision_workload(args):
         tpol += time.perf_counter() - t0
         npol_total += npol
         nchecks += 1
-        print(f"[bench] fp32 {n32} steps ({t32:.2f} s, max|EG| {ea32:.2e}) + polish {npol}: E={Ep:.12f}, "
+        print(f"[bench] {form} {n32} steps ({t32:.2f} s, max|EG| {ea32:.2e}) + polish {npol}: E={Ep:.12f}, "
               f"max|EG|={eap:.2e}", file=sys.stderr, flush=True)
         if eap < tol or n32 >= args.max_steps:
             break
@@ -183,7 +187,7 @@ def precision_workload(args):
         else:
             chunk = args.chunk
         t0 = time.perf_counter()
-        s32.set_precision("fp32")
+        s32.set_precision(form)
         s32.set_tile_variants(tiles32["variants"])
         tpol += time.perf_counter() - t0
         E = Ep
@@ -194,7 +198,7 @@ def precision_workload(args):
     dm = float(np.abs(Fp[:3] - F64[:3]).max())
     t_mixed = t32 + tpol
     out = {
-        "metric": "time-to-converge, FP32 form + FP64 polish vs FP64 form",
+        "metric": f"time-to-converge, {form} form + FP64 polish vs FP64 form",
         "value": t64 / t_mixed,
         "unit": "speedup (FP64 form time / FP32+polish time)",
         "n_gpus": 1,
@@ -207,7 +211,7 @@ def precision_workload(args):
                    "lattice": [n, n], "fp32_chunk_steps": args.chunk, "max_polish_steps": args.polish},
         "converged": bool(ea64 < tol and eap < tol),
         "fp64_form": {"steps": n64, "seconds": t64, "energy": E64, "err_absmax": ea64},
-        "fp32_phase": {"steps": n32, "seconds": t32, "polish_checks": nchecks},
+        "fp32_phase": {"form": form, "steps": n32, "seconds": t32, "polish_checks": nchecks, "tiles": tiles32},
         "fp64_polish": {"steps_total": npol_total, "steps_final": npol, "tail_steps": fp64_tail, "seconds": tpol,
                         "energy": Ep, "err_absmax": eap},
         "skyrmion_number": {"fp64_form": Q64, "fp32_polished": Qp, "identical_rounded": round(Q64) == round(Qp)},
@@ -471,6 +475,8 @@ def main():
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
+    ap.add_argument("--form", choices=["fp32c", "fp32"], default="fp32c",
+                    help="precision workload: FP32 phase with the compensated field update (fp32c) or plain (fp32)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--tol", type=float, default=None,
                     help="max |EG| to reach (precision workload: 1e-6; batch workload: 1e-3, the FP32 floor)")

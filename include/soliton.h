@@ -38,8 +38,16 @@ enum {
 };
 
 /* Storage / stencil precision.  Energy and charge sums and the arrest test are
- * FP64 in both. */
-enum { SS_PREC_FP64 = 0, SS_PREC_FP32 = 1 };
+ * FP64 in all three.
+ * FP32C: FP32 storage and stencils as FP32, with a second FP32 word per field
+ *   value (the field is hi + lo, nf*N more floats): the combine of each step
+ *   adds its increment to hi + lo in FP64 and splits the sum again, so steps
+ *   that move the field by less than an FP32 ulp accumulate instead of being
+ *   rounded away.  Stencils, energies and charges read hi.  Reference flow of
+ *   theories without the L2 normalisation; not in the fused 4+1 sweep; not
+ *   in batches.  ss_upload / ss_download of the field split / join hi + lo;
+ *   ss_init_field and ss_add_noise set lo to zero. */
+enum { SS_PREC_FP64 = 0, SS_PREC_FP32 = 1, SS_PREC_FP32C = 2 };
 
 /* Variation used by the flow and by ss_variation:
  * REFERENCE: the reference's Euler-Lagrange expression (theory
@@ -70,7 +78,9 @@ int ss_ctx_destroy(ss_ctx* ctx);
  * (xlen, ylen, number_total_fields) must not change. */
 int ss_set_params(ss_ctx* ctx, const int32_t* p_i, int n_i, const double* p_f, int n_f);
 
-/* Switch storage precision in place (FP32 relaxation -> FP64 polish). */
+/* Switch storage precision in place (FP32 relaxation -> FP64 polish).
+ * FP64 -> FP32C splits the field exactly into hi + lo; FP32C -> FP64 joins
+ * it; FP32 -> FP32C starts lo at zero; FP32C -> FP32 rounds hi + lo to FP32. */
 int ss_set_precision(ss_ctx* ctx, int precision);
 int ss_set_variation_mode(ss_ctx* ctx, int variation_mode);
 

@@ -25,6 +25,7 @@ THEORY_LIQUID_CRYSTAL = 9
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_FP32C = 2
 VAR_REFERENCE = 0
 VAR_EXACT_ADJOINT = 1
 AUX_HIGGS, AUX_HIGGS1, AUX_HIGGS2, AUX_PHASE_DIFFERENCE, AUX_SUPERCURRENT, AUX_MAGNETIC_FIELD = range(6)

@@ -11,8 +11,9 @@ four fused HIP passes with the arrest decision taken on the device.
 
 Extras beyond the reference (no counterpart there): ``steps`` (many steps per
 call), ``minimize`` (the headless loop the reference documents at
-core/simulation.py:8,39 but does not define), ``precision`` ("fp64" or the
-"fp32" stencil / FP64-reduction form), ``energy_gradient`` and array
+core/simulation.py:8,39 but does not define), ``precision`` ("fp64", the
+"fp32" stencil / FP64-reduction form, or "fp32c", the same with a compensated
+field update that keeps increments below an FP32 ulp), ``energy_gradient`` and array
 injection ``initialize({"mode": "array", ...})``.
 """
 from __future__ import annotations
@@ -24,7 +25,7 @@ import numpy as np
 from .. import _lib
 from .params import Params
 
-_PREC = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32}
+_PREC = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32, "fp32c": _lib.PREC_FP32C}
 _VAR = {"reference": _lib.VAR_REFERENCE, "exact_adjoint": _lib.VAR_EXACT_ADJOINT}
 
 
@@ -155,7 +156,9 @@ class Simulation:
             self._check(L.ss_set_params(self._ctx, self.p_i_h, self.p_i_h.size, self.p_f_h, self.p_f_h.size))
 
     def set_precision(self, precision: str) -> None:
-        """Switch storage precision in place (e.g. FP32 relaxation -> FP64 polish)."""
+        """Switch storage precision in place (e.g. FP32 relaxation -> FP64 polish).
+        "fp64" -> "fp32c" splits the field exactly into FP32 hi + lo words and
+        "fp32c" -> "fp64" joins them; "fp32" -> "fp32c" starts lo at zero."""
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {sorted(_PREC)}")
         self._check(_lib.lib().ss_set_precision(self._ctx, _PREC[precision]))

@@ -53,7 +53,7 @@ inline Launch make_launch(const Geom& g) {
 // outputs for the sites of the tile, and the block's partial (energy sum for
 // pass 1, signed max / max |EG| for pass 4) written to part0[t] / part1[t].
 // Shared by the single-lattice kernel and the batched kernel.
-template <class TH, typename R, int PASS, int TXR = 0>
+template <class TH, typename R, int PASS, int TXR = 0, bool COMP = false>
 __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* red, const PassBufs& b,
                                            const typename TH::TP& tp, const Geom& g, int x0, int y0, bool kill,
                                            int t) {
@@ -192,7 +192,10 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
         // RK4 combine.  With Temp4 = F + dt (V + l2/2) at the centre (f, from
         // LDS), F' = F + (k1 + 2 k2 + 2 k3 + k4)/6 = Temp4 + dt (l1 - 2 l2 + l3)/6,
         // so F is not re-read; V and l2 are the registers kept in phase A.
+        // COMP (FP32): the new field is formed by compensated_field from F and
+        // its low words instead.
         R Fn[NF], Vn[NF];
+        double Fd[COMP ? NF : 1];  // COMP: F + increment in FP64
         const R sixth = R(1.0 / 6.0);
         const R dt6 = dt * sixth;
 #pragma unroll
@@ -203,12 +206,22 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
           const R a1 = l1[o], a3 = l3[o];
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
+          if constexpr (COMP)
+            Fd[a] = compensated_sum(F[o], b.Flo[o], dt * (Vc + half * a2) + dt6 * (a1 - R(2) * a2 + a3));
+          else
+            Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
           const double e = (double)eg[a];
           acc0 = fmax(acc0, e);
           acc1 = fmax(acc1, fabs(e));
         }
-        if (TH::kUnitConstraint && g.unit_mag) {
+        if constexpr (COMP) {
+          static_assert(sizeof(R) == 4, "compensated combine: FP32 only");
+          float lon[NF];
+          compensated_split<TH>(tp, g, Fd, Fn, lon);
+#pragma unroll
+          for (int a = 0; a < NF; ++a) b.Flo[off + a * g.N] = lon[a];
+          if (TH::kUnitConstraint && g.unit_mag) TH::template project<R>(tp, Vn, Fn);
+        } else if (TH::kUnitConstraint && g.unit_mag) {
           TH::template normalize<R>(tp, Fn);
           TH::template project<R>(tp, Vn, Fn);
         }
@@ -238,7 +251,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
   }
 }
 
-template <class TH, typename R, int PASS, int TXR = 0>
+template <class TH, typename R, int PASS, int TXR = 0, bool COMP = false>
 __global__ void __launch_bounds__(kThreads)
 stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
   using S = TileShape<R, TH::NF, TXR>;
@@ -248,7 +261,7 @@ stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
   if (t < 0) return;  // whole block exits together, before any barrier
   const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
   const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
-  stage_tile<TH, R, PASS, TXR>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
+  stage_tile<TH, R, PASS, TXR, COMP>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
 }
 
 // Energy (kind < 0) or charge density ``kind`` of F, times gv: block partial
@@ -743,6 +756,20 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 7:
+        if constexpr (sizeof(R) == 4) {
+          hipLaunchKernelGGL((stream_pass_comp<TH, R, 4, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
+      case 8:
+        if constexpr (sizeof(R) == 4 && StreamRing<TH, R, 1, CW, true>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass_comp<TH, R, 1, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -757,7 +784,9 @@ struct TheoryLaunchers {
 
   // Passes 2 and 3 fused into one streaming sweep (pass id 5), and pass 4
   // with pass 1 of the next step (pass id 6), when both halves fit one ring
-  // and the field uses 32-bit offsets.
+  // and the field uses 32-bit offsets.  Pass ids 7 and 8: passes 4 (any shape
+  // variant) and 1 (streaming variants) of the compensated FP32 form, which
+  // needs the streaming pass 1 (can_comp).
   static int can_fuse23(int prec, const Geom& g) {
     const bool fits = prec ? StreamRing<TH, float, 23>::shape::kFits : StreamRing<TH, double, 23>::shape::kFits;
     return fits && (prec ? effective_shape<float>(3, g) : effective_shape<double>(3, g)) == 3;
@@ -765,6 +794,11 @@ struct TheoryLaunchers {
   static int can_fuse41(int prec, const Geom& g) {
     const bool fits = prec ? StreamRing<TH, float, 41>::shape::kFits : StreamRing<TH, double, 41>::shape::kFits;
     return fits && (prec ? effective_shape<float>(3, g) : effective_shape<double>(3, g)) == 3 && !TH::kL2Normalize;
+  }
+
+  static int can_comp(int prec, const Geom& g) {
+    return prec && StreamRing<TH, float, 1, 1, true>::shape::kFits && effective_shape<float>(3, g) == 3 &&
+           !TH::kL2Normalize;
   }
 
   static int ntiles(int prec, int v, const Geom& g) { return prec ? ntiles_r<float>(v, g) : ntiles_r<double>(v, g); }
@@ -779,6 +813,13 @@ struct TheoryLaunchers {
       case 2: hipLaunchKernelGGL((stage_pass<TH, R, 2, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 3: hipLaunchKernelGGL((stage_pass<TH, R, 3, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 4: hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 7:
+        if constexpr (sizeof(R) == 4) {
+          hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR, true>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -789,7 +830,7 @@ struct TheoryLaunchers {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
     v = effective_shape<R>(v, g);
-    if (pass == 5 || pass == 6) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
+    if (pass == 5 || pass == 6 || pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_v<R>(pass, v, b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -978,6 +1019,7 @@ struct TheoryOps {
   int num_shapes;
   int (*can_fuse23)(int, const Geom&);
   int (*can_fuse41)(int, const Geom&);
+  int (*can_comp)(int, const Geom&);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -1003,6 +1045,7 @@ struct TheoryOps {
                                       ss::TheoryLaunchers<TH>::kNumShapes,                  \
                                       &ss::TheoryLaunchers<TH>::can_fuse23,                 \
                                       &ss::TheoryLaunchers<TH>::can_fuse41,                 \
+                                      &ss::TheoryLaunchers<TH>::can_comp,                   \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

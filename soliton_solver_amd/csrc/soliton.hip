@@ -194,6 +194,25 @@ __global__ void k_div_scalar(double* G, long long n, const double* s) {
     G[i] = G[i] / d;
 }
 
+// Compensated FP32 form: split an FP64 field into hi = fl32(F), lo = fl32(F - hi);
+// join hi + lo back into FP64; fold lo into hi (to the plain FP32 form).
+__global__ void k_split(const double* src, float* hi, float* lo, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = src[i];
+    const float h = (float)v;
+    hi[i] = h;
+    lo[i] = (float)(v - (double)h);
+  }
+}
+__global__ void k_join(const float* hi, const float* lo, double* dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (double)hi[i] + (double)lo[i];
+}
+__global__ void k_fold(float* hi, const float* lo, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    hi[i] = (float)((double)hi[i] + (double)lo[i]);
+}
+
 __global__ void k_scale_into(double* dst, const double* src, long long n, double a) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     dst[i] = a * src[i];
@@ -223,6 +242,7 @@ struct ss_ctx {
   int fuse23 = 0;          // passes 2 and 3 as one streaming sweep (variant tv[2], at least 3)
   int fuse41 = 0;          // pass 4 and the next step's pass 1 as one sweep (variant tv[4], at least 3)
   void* l1alt = nullptr;   // with fuse41: l1 of the steps that start from parity 1
+  float* Flo = nullptr;    // SS_PREC_FP32C: low words of the field (nf*N, updated in place by pass 4)
   double* part = nullptr;  // 3 x ntiles
   DevState* st = nullptr;
   double* hist = nullptr;  // 2 x hist_cap
@@ -322,6 +342,8 @@ static void free_fields(ss_ctx* c) {
   }
   if (c->l1alt) hipFree(c->l1alt);
   c->l1alt = nullptr;
+  if (c->Flo) hipFree(c->Flo);
+  c->Flo = nullptr;
 }
 
 // The second l1 buffer of the fused 4+1 schedule: a step reads l1 of its
@@ -336,9 +358,47 @@ static int ensure_l1alt(ss_ctx* c) {
 
 static void* l1_buf(ss_ctx* c, int parity) { return (c->fuse41 && parity) ? c->l1alt : c->l[0]; }
 
+// The fused 4+1 sweep is available (not in the compensated form, whose pass 4
+// forms the new field from F and its low words).
+static bool can41(const ss_ctx* c) { return c->prec != SS_PREC_FP32C && c->ops->can_fuse41(c->prec, c->g); }
+
 // The fused 4+1 schedule is active (reference flow only).
 static bool use41(const ss_ctx* c) {
-  return c->fuse41 && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize;
+  return c->fuse41 && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec != SS_PREC_FP32C;
+}
+
+// Pass 1 of the step: in the compensated form the streaming pass 1 (id 8),
+// which also takes the energy of F + low words, in a strip variant.
+static int pass1_id(const ss_ctx* c) { return c->prec == SS_PREC_FP32C ? 8 : 1; }
+static int pass1_variant(const ss_ctx* c) {
+  return (c->prec == SS_PREC_FP32C && c->tv[1] < 3) ? 3 : c->tv[1];
+}
+
+// Energy partials of the field F (of F + its low words in the compensated
+// form) into c->part; *nparts = their number.
+static int energy_partials(ss_ctx* c, const void* F, int* nparts) {
+  if (c->prec == SS_PREC_FP32C) {
+    PassBufs b{};
+    b.F = F;
+    b.Flo = c->Flo;
+    b.lout = c->l[0];  // l1 of this field: scratch (every step recomputes it)
+    b.part0 = c->part;
+    b.st = c->st;
+    const int v = pass1_variant(c);
+    HIPCHK(c, c->ops->pass(c->prec, 8, v, b, c->tp, c->g, c->stream));
+    *nparts = c->ops->ntiles(c->prec, v, c->g);
+  } else {
+    HIPCHK(c, c->ops->density(c->prec, -1, F, nullptr, c->part, c->tp, c->g, c->stream));
+    *nparts = c->nt0;
+  }
+  return 0;
+}
+
+static int alloc_low_words(ss_ctx* c) {
+  const size_t bytes = (size_t)c->g.nf * (size_t)c->g.N * sizeof(float);
+  if (!c->Flo) HIPCHK(c, hipMalloc(&c->Flo, bytes));
+  HIPCHK(c, hipMemsetAsync(c->Flo, 0, bytes, c->stream));
+  return 0;
 }
 
 static int alloc_fields(ss_ctx* c) {
@@ -354,6 +414,7 @@ static int alloc_fields(ss_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->l[i], 0, bytes, c->stream));
   }
   if (c->fuse41 && ensure_l1alt(c)) return -1;
+  if (c->prec == SS_PREC_FP32C && alloc_low_words(c)) return -1;
   c->cur = 0;
   return 0;
 }
@@ -396,6 +457,7 @@ static PassBufs bufs_for(ss_ctx* c, int parity) {
   b.Fout = c->F[parity ^ 1];
   b.Vout = c->V[parity ^ 1];
   b.st = c->st;
+  b.Flo = c->Flo;
   return b;
 }
 
@@ -428,10 +490,11 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (!f41 || first) {
     b.lout = l1_buf(c, parity);
     b.part0 = p0;
-    HIPCHK(c, c->ops->pass(c->prec, 1, c->tv[1], b, c->tp, c->g, c->stream));
+    const int v1 = pass1_variant(c);
+    HIPCHK(c, c->ops->pass(c->prec, pass1_id(c), v1, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0,
-                       c->ops->ntiles(c->prec, c->tv[1], c->g), c->st, histE, c->hist_cap, c->g.killkinen);
+                       c->ops->ntiles(c->prec, v1, c->g), c->st, histE, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   } else if (ev) {
@@ -472,7 +535,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     return 0;
   }
   b.lout = nullptr;
-  HIPCHK(c, c->ops->pass(c->prec, 4, c->tv[4], b, c->tp, c->g, c->stream));
+  HIPCHK(c, c->ops->pass(c->prec, c->prec == SS_PREC_FP32C ? 7 : 4, c->tv[4], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
                      c->ops->ntiles(c->prec, c->tv[4], c->g), c->st, histErr, c->hist_cap);
@@ -595,8 +658,9 @@ static int launch_step_l2(ss_ctx* c, int parity) {
 // already given E(F_final) and the decision.
 static int launch_finish(ss_ctx* c, int nsteps) {
   if (!(use41(c) && nsteps > 0)) {
-    HIPCHK(c, c->ops->density(c->prec, -1, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
-    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
+    int np = 0;
+    if (energy_partials(c, c->F[c->cur], &np)) return -1;
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, np, c->st,
                        c->hist, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
   }
@@ -646,7 +710,8 @@ int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f,
   *out = nullptr;
   const TheoryOps* ops = theory_ops(theory_id);
   if (!ops) return fail(nullptr, "unknown or unbuilt theory id %d", theory_id);
-  if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return fail(nullptr, "bad precision %d", precision);
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32 && precision != SS_PREC_FP32C)
+    return fail(nullptr, "bad precision %d", precision);
   if (variation_mode != SS_VAR_REFERENCE && variation_mode != SS_VAR_EXACT_ADJOINT)
     return fail(nullptr, "bad variation mode %d", variation_mode);
   if (variation_mode == SS_VAR_EXACT_ADJOINT && precision != SS_PREC_FP64)
@@ -666,6 +731,10 @@ int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f,
   if (resolve_geom(c, p_i, n_i, p_f, n_f, &c->g)) return bail(-1);
   if (c->g.nf != ops->nf) {
     fail(c, "theory %s is built for %d fields (number_total_fields=%d)", ops->name, ops->nf, c->g.nf);
+    return bail(-1);
+  }
+  if (precision == SS_PREC_FP32C && !ops->can_comp(precision, c->g)) {
+    fail(c, "the compensated FP32 form is not available for theory %s at this size", ops->name);
     return bail(-1);
   }
   char msg[256] = {0};
@@ -756,21 +825,52 @@ int ss_set_variation_mode(ss_ctx* c, int mode) {
 
 int ss_set_precision(ss_ctx* c, int prec) {
   if (!c) return fail(nullptr, "null context");
-  if (prec != SS_PREC_FP64 && prec != SS_PREC_FP32) return fail(c, "bad precision %d", prec);
+  if (prec != SS_PREC_FP64 && prec != SS_PREC_FP32 && prec != SS_PREC_FP32C) return fail(c, "bad precision %d", prec);
   if (prec == c->prec) return 0;
-  if (prec == SS_PREC_FP32 && c->var_mode == SS_VAR_EXACT_ADJOINT)
+  if (prec != SS_PREC_FP64 && c->var_mode == SS_VAR_EXACT_ADJOINT)
     return fail(c, "the exact-adjoint variation needs FP64 storage");
+  if (prec == SS_PREC_FP32C && !c->ops->can_comp(prec, c->g))
+    return fail(c, "the compensated FP32 form is not available for theory %s at this size", c->ops->name);
   const long long n = (long long)c->g.nf * c->g.N;
+  if (prec != SS_PREC_FP64 && c->prec != SS_PREC_FP64) {
+    // FP32 <-> compensated FP32: the same storage; the low words start at
+    // zero, or are folded into the field (rounded to FP32) and dropped.
+    free_graphs(c);
+    if (prec == SS_PREC_FP32C) {
+      c->prec = prec;
+      if (alloc_low_words(c)) return -1;
+    } else {
+      hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, c->stream, (float*)c->F[c->cur], (const float*)c->Flo, n);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      hipFree(c->Flo);
+      c->Flo = nullptr;
+      c->prec = prec;
+    }
+    if (c->fuse41 && !can41(c)) c->fuse41 = 0;
+    return 0;
+  }
   const size_t nb = (size_t)n * elem_size(prec);
   void* nF = nullptr;
   void* nV = nullptr;
+  float* nLo = nullptr;
   HIPCHK(c, hipMalloc(&nF, nb));
   HIPCHK(c, hipMalloc(&nV, nb));
+  if (prec == SS_PREC_FP32C) HIPCHK(c, hipMalloc(&nLo, nb));
   if (prec == SS_PREC_FP64) {
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, c->stream, (const float*)c->F[c->cur], (double*)nF, n);
+    if (c->prec == SS_PREC_FP32C)
+      hipLaunchKernelGGL(k_join, dim3(2048), dim3(256), 0, c->stream, (const float*)c->F[c->cur], (const float*)c->Flo,
+                         (double*)nF, n);
+    else
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, c->stream, (const float*)c->F[c->cur],
+                         (double*)nF, n);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, c->stream, (const float*)c->V[c->cur], (double*)nV, n);
   } else {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, c->stream, (const double*)c->F[c->cur], (float*)nF, n);
+    if (prec == SS_PREC_FP32C)
+      hipLaunchKernelGGL(k_split, dim3(2048), dim3(256), 0, c->stream, (const double*)c->F[c->cur], (float*)nF, nLo, n);
+    else
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, c->stream,
+                         (const double*)c->F[c->cur], (float*)nF, n);
     hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, c->stream, (const double*)c->V[c->cur], (float*)nV, n);
   }
   HIPCHK(c, hipGetLastError());
@@ -779,13 +879,15 @@ int ss_set_precision(ss_ctx* c, int prec) {
   free_fields(c);
   c->prec = prec;
   if (c->fuse23 && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
-  if (c->fuse41 && !c->ops->can_fuse41(prec, c->g)) c->fuse41 = 0;
+  if (c->fuse41 && !can41(c)) c->fuse41 = 0;
   if (alloc_fields(c)) return -1;
   HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->V[0], nV, nb, hipMemcpyDeviceToDevice, c->stream));
+  if (nLo) HIPCHK(c, hipMemcpyAsync(c->Flo, nLo, nb, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   hipFree(nF);
   hipFree(nV);
+  if (nLo) hipFree(nLo);
   c->cur = 0;
   return alloc_partials(c);
 }
@@ -803,6 +905,23 @@ static int host_copy(ss_ctx* c, int buf, double* host, size_t n, bool up) {
     if (up) HIPCHK(c, hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     else HIPCHK(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else if (c->prec == SS_PREC_FP32C && buf == SS_BUF_FIELD) {
+    // compensated form: the field is hi + lo
+    std::vector<float> hi(n), lo(n);
+    if (up) {
+      for (size_t i = 0; i < n; ++i) {
+        hi[i] = (float)host[i];
+        lo[i] = (float)(host[i] - (double)hi[i]);
+      }
+      HIPCHK(c, hipMemcpyAsync(dev, hi.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->Flo, lo.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+      HIPCHK(c, hipMemcpyAsync(hi.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(lo.data(), c->Flo, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < n; ++i) host[i] = (double)hi[i] + (double)lo[i];
+    }
   } else {
     std::vector<float> tmp(n);
     if (up) {
@@ -934,16 +1053,37 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   PassBufs b = bufs_for(c, c->cur);
   b.part0 = c->part;
   b.part1 = c->part + c->ntiles;
+  // the compensated pass 4 updates the low words in place: time it on a copy
+  float* lo_scratch = nullptr;
+  if (c->prec == SS_PREC_FP32C) {
+    const size_t lb = (size_t)c->g.nf * (size_t)c->g.N * sizeof(float);
+    hipError_t e = hipMalloc(&lo_scratch, lb);
+    if (e == hipSuccess) e = hipMemcpyAsync(lo_scratch, c->Flo, lb, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+      if (lo_scratch) hipFree(lo_scratch);
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return hipfail(c, "scratch for the compensated pass 4 failed: %s", hipGetErrorString(e));
+    }
+    b.Flo = lo_scratch;
+  }
   int pick[5] = {0, 0, 0, 0, 0};
   float bestp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   int rc = 0;
   for (int p = 1; p <= 4 && rc == 0; ++p) {
     b.lout = p <= 3 ? c->l[p - 1] : nullptr;
+    const bool comp = c->prec == SS_PREC_FP32C;
+    const int pid = (comp && p == 4) ? 7 : (comp && p == 1) ? 8 : p;
+    const int v0 = (comp && p == 1) ? 3 : 0;  // the compensated pass 1 streams
     float best = 0.f;
     for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
-      hipError_t e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);  // warm-up
+      if (v < v0) {
+        if (ms) ms[(p - 1) * c->ops->num_shapes + v] = 0.f;
+        continue;
+      }
+      hipError_t e = c->ops->pass(c->prec, pid, v, b, c->tp, c->g, c->stream);  // warm-up
       if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, p, v, b, c->tp, c->g, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, pid, v, b, c->tp, c->g, c->stream);
       if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float t = 0.f;
@@ -954,7 +1094,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
       }
       t /= reps;
       if (ms) ms[(p - 1) * c->ops->num_shapes + v] = t;
-      if (v == 0 || t < best) {
+      if (v == v0 || t < best) {
         best = t;
         pick[p] = v;
       }
@@ -993,8 +1133,8 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   const float sep41 = bestp[4] + bestp[1];
   float best41 = 0.f;
   int pick41 = -1;
-  if (rc == 0 && c->ops->can_fuse41(c->prec, c->g)) rc = ensure_l1alt(c);
-  if (rc == 0 && c->ops->can_fuse41(c->prec, c->g)) {
+  if (rc == 0 && can41(c)) rc = ensure_l1alt(c);
+  if (rc == 0 && can41(c)) {
     b.lout = (b.l1 == c->l[0]) ? c->l1alt : c->l[0];
     b.lout2 = nullptr;
     b.part2 = c->part + 2 * c->ntiles;
@@ -1020,6 +1160,10 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (lo_scratch) {
+    hipStreamSynchronize(c->stream);
+    hipFree(lo_scratch);
+  }
   if (rc) return rc;
   for (int p = 1; p <= 4; ++p) {
     c->tv[p] = pick[p];
@@ -1045,7 +1189,7 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (flags & ~(SS_FUSE_23 | SS_FUSE_41)) return fail(c, "unknown fused-stage flags 0x%x", flags);
   if ((flags & SS_FUSE_23) && !c->ops->can_fuse23(c->prec, c->g))
     return fail(c, "passes 2 and 3 of theory %s cannot share one sweep at this precision and size", c->ops->name);
-  if ((flags & SS_FUSE_41) && !c->ops->can_fuse41(c->prec, c->g))
+  if ((flags & SS_FUSE_41) && !can41(c))
     return fail(c, "pass 4 and pass 1 of theory %s cannot share one sweep at this precision and size", c->ops->name);
   if ((flags & SS_FUSE_41) && ensure_l1alt(c)) return -1;
   c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
@@ -1076,8 +1220,13 @@ static int reduce_kind(ss_ctx* c, int kind, double* out) {
   if (!c) return fail(nullptr, "null context");
   if (!out) return fail(c, "null output");
   if (kind >= c->ops->ncharges) return fail(c, "theory %s has %d charges (asked %d)", c->ops->name, c->ops->ncharges, kind);
-  HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->scal);
+  int np = c->nt0;
+  if (kind < 0) {
+    if (energy_partials(c, c->F[c->cur], &np)) return -1;
+  } else {
+    HIPCHK(c, c->ops->density(c->prec, kind, c->F[c->cur], nullptr, c->part, c->tp, c->g, c->stream));
+  }
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, np, c->scal);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1135,6 +1284,8 @@ int ss_init_field(ss_ctx* c, const ss_init_desc* d) {
   hipError_t e = c->ops->init(c->prec, a, c->F[c->cur], c->g, c->stream);
   if (e == hipSuccess)
     e = hipMemsetAsync(c->V[c->cur], 0, elem_size(c->prec) * (size_t)c->g.nf * (size_t)c->g.N, c->stream);
+  if (e == hipSuccess && c->Flo)
+    e = hipMemsetAsync(c->Flo, 0, sizeof(float) * (size_t)c->g.nf * (size_t)c->g.N, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (sol) hipFree(sol);
   if (e != hipSuccess) return hipfail(c, "device initialisation failed: %s", hipGetErrorString(e));
@@ -1155,6 +1306,7 @@ int ss_add_noise(ss_ctx* c, int comp_begin, int comp_end, int gaussian, double a
     hipLaunchKernelGGL((k_noise<double>), grid, dim3(256), 0, c->stream, (double*)c->F[c->cur], c->g.N, comp_begin,
                        comp_end, gaussian, amplitude, (unsigned long long)seed, renorm_ncomp, renorm_norm);
   HIPCHK(c, hipGetLastError());
+  if (c->Flo) HIPCHK(c, hipMemsetAsync(c->Flo, 0, sizeof(float) * (size_t)c->g.nf * (size_t)c->g.N, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1239,8 +1391,8 @@ int ss_synchronize(ss_ctx* c) {
 int ss_memory_bytes(ss_ctx* c, size_t* bytes) {
   if (!c || !bytes) return fail(c, "null argument");
   const size_t f = (size_t)c->g.nf * (size_t)c->g.N * elem_size(c->prec);
-  *bytes = (c->l1alt ? 8 : 7) * f + sizeof(double) * 3 * (size_t)c->ntiles + sizeof(double) * 2 * (size_t)c->hist_cap +
-           (c->dens ? sizeof(double) * (size_t)c->g.N : 0);
+  *bytes = (c->l1alt ? 8 : 7) * f + (c->Flo ? f : 0) + sizeof(double) * 3 * (size_t)c->ntiles +
+           sizeof(double) * 2 * (size_t)c->hist_cap + (c->dens ? sizeof(double) * (size_t)c->g.N : 0);
   return 0;
 }
 

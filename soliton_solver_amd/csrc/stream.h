@@ -37,6 +37,7 @@ struct PassBufs {
   double* part1;   // pass 4: max |EG| partial
   double* part2;   // fused 4+1: energy partial of the new field
   const DevState* st;
+  float* Flo;      // compensated FP32 form, pass 4: low words of the field (read and written in place)
 };
 
 // Ring of NFR components (NF for one stage field, 2 NF when two stage fields
@@ -202,7 +203,8 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
 // 32-bit (the host keeps nf*X*Y below 2^31); a site outside the lattice loads
 // element 0 and is replaced by 0, so no load sits behind a branch.  PASS
 // selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 41 F, V,
-// l2, l1, l3.
+// l2, l1, l3; pass 1 of the compensated form (PASS 1 with a two-half ring) F
+// and its low words (into l).
 template <int PASS, typename R, int NF, int NFR, int SC>
 __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
@@ -228,8 +230,11 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
       } else {
         p.v[k][a] = R(0);
       }
-      if (PASS >= 3) {
+      if constexpr (PASS >= 3) {
         const R lv = ld<R>(plane<R>(L, a, NP), boff);
+        p.l[k][a] = in ? lv : R(0);
+      } else if constexpr (PASS == 1 && NFR == 2 * NF) {
+        const R lv = ld<R>(plane<R>(b.Flo, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
       } else {
         p.l[k][a] = R(0);
@@ -258,6 +263,29 @@ __device__ __forceinline__ void new_field(const typename TH::TP& tp, const Geom&
     Fn[a] = T4[a] + dt6 * (l1[a] - R(2) * l2[a] + l3[a]);
   }
   if (TH::kUnitConstraint && g.unit_mag && in) TH::template normalize<R>(tp, Fn);
+}
+
+// Compensated FP32 form: the field is hi + lo, hi the stored FP32 field (what
+// every stencil reads) and lo a second FP32 word per component holding the
+// part of the field below hi's rounding.  The step's increment (FP32, relative
+// error ~1e-7 of the increment itself) is added to hi + lo in FP64
+// (compensated_sum), the unit constraint applied in FP64, and the sum split
+// again: hi' = fl32(F'), lo' = fl32(F' - hi') (compensated_split).
+// Increments smaller than an FP32 ulp of the field, which the plain form
+// rounds away, accumulate in lo until they move hi.
+__device__ __forceinline__ double compensated_sum(float hi, float lo, float inc) {
+  return ((double)hi + (double)lo) + (double)inc;
+}
+template <class TH>
+__device__ __forceinline__ void compensated_split(const typename TH::TP& tp, const Geom& g, double* Fd, float* Fn,
+                                                  float* lon) {
+  constexpr int NF = TH::NF;
+  if (TH::kUnitConstraint && g.unit_mag) TH::template normalize<double>(tp, Fd);
+#pragma unroll
+  for (int a = 0; a < NF; ++a) {
+    Fn[a] = (float)Fd[a];
+    lon[a] = (float)(Fd[a] - (double)Fn[a]);
+  }
 }
 
 template <int PASS, typename R>
@@ -289,6 +317,7 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
         q.v[NF + a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
       } else {
         q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+        if constexpr (PASS == 1 && NFR == 2 * NF) q.v[NF + a][slot][c] = p.l[k][a];  // low words
       }
     }
   }
@@ -348,7 +377,10 @@ __device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q,
 // in the 4+1 sweep also l1 and l3 from kA / kC;
 // with FOFF >= 0 it takes the new field F' from ring components FOFF ..
 // FOFF+NF-1 (the 4+1 sweep, which builds F' there) instead of forming it.
-template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int SC>
+// COMP (FP32): pass 4 forms the new field with compensated_sum / _split from
+// F and its low words; pass 1 takes the energy of F + low words (ring
+// components NF..2NF-1).
+template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int FOFF, int SC, bool COMP = false>
 __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
@@ -362,7 +394,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
   // stencils so their latency overlaps the variation (the stores of the loop
   // below would otherwise keep the compiler from hoisting them).  In FP64 the
   // registers are not there to spare, and they are loaded at the combine.
-  constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF < 0;
+  constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF < 0 && !COMP;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
   if (kPre) {
 #pragma unroll
@@ -410,11 +442,25 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
         double fd[NF], d1d[NF][2];
 #pragma unroll
         for (int a = 0; a < NF; ++a) ring_d1<INTERIOR, double, R, NFR>(q, AOFF + a, sl, cy, Kd, s, fd[a], d1d[a]);
+        if constexpr (COMP) {
+          // compensated form: the energy of hi + lo (low words in ring
+          // components NF..2NF-1); the stencils are linear, so
+          // D1(hi + lo) = D1(hi) + D1(lo) in FP64
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            double fl, d1l[2];
+            ring_d1<INTERIOR, double, R, NFR>(q, NF + a, sl, cy, Kd, s, fl, d1l);
+            fd[a] += fl;
+            d1d[a][0] += d1l[0];
+            d1d[a][1] += d1l[1];
+          }
+        }
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
       if (PASS == 4) {
         // F' = Temp4 + dt (l1 - 2 l2 + l3)/6 with Temp4 the ring centre.
         R Fn[NF], Vn[NF];
+        double Fd[COMP ? NF : 1];  // COMP: F + increment in FP64
         const R sixth = R(1.0 / 6.0);
         const R dt6 = dt * sixth;
 #pragma unroll
@@ -434,7 +480,11 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           }
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          if constexpr (FOFF >= 0)
+          if constexpr (COMP)
+            Fd[a] = compensated_sum(ld<float>(plane<float>(b.F, a, NP), boff),
+                                    ld<float>(plane<float>(b.Flo, a, NP), boff),
+                                    dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
+          else if constexpr (FOFF >= 0)
             Fn[a] = q.v[FOFF + a][sl[2]][cy];
           else
             Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
@@ -442,7 +492,14 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           acc0 = fmax(acc0, e);
           acc1 = fmax(acc1, fabs(e));
         }
-        if (TH::kUnitConstraint && g.unit_mag) {
+        if constexpr (COMP) {
+          static_assert(sizeof(R) == 4 && FOFF < 0, "compensated combine: FP32 pass 4 only");
+          float lon[NF];
+          compensated_split<TH>(tp, g, Fd, Fn, lon);
+#pragma unroll
+          for (int a = 0; a < NF; ++a) st<float>(b.Flo, a, NP, boff, lon[a]);
+          if (TH::kUnitConstraint && g.unit_mag) TH::template project<R>(tp, Vn, Fn);
+        } else if (TH::kUnitConstraint && g.unit_mag) {
           if (FOFF < 0) TH::template normalize<R>(tp, Fn);
           TH::template project<R>(tp, Vn, Fn);
         }
@@ -465,20 +522,21 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 // ring, pass 4 runs on the first half (F', V' -> Fout, Vout) and pass 1 on the
 // second (l1 of the next step -> lout, its energy partial -> part2)).
 // Partials: one per strip.
-template <class TH, typename R, int PASS, int CW = 1>
+template <class TH, typename R, int PASS, int CW = 1, bool COMP = false>
 struct StreamRing {
-  static constexpr int NFR = (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
+  static constexpr int NFR = (PASS == 23 || PASS == 41 || (PASS == 1 && COMP)) ? 2 * TH::NF : TH::NF;
   // the 4+1 sweep prefetches five arrays: one row per wave and iteration
   static constexpr int SC = (PASS == 41 ? 1 : 2) | (CW == 2 ? 4 : 0);
   using shape = StreamShape<R, NFR, SC>;
   using type = Ring<R, NFR, SC>;
 };
 
-template <class TH, typename R, int PASS, int CW>
-__device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW>::type& q, double* red,
+template <class TH, typename R, int PASS, int CW, bool COMP = false>
+__device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW, COMP>::type& q, double* red,
                                              const PassBufs& b, const typename TH::TP& tp, const Geom& g, int xs,
                                              int sx, int y0, bool kill, int t) {
-  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS, CW>::NFR, SC = StreamRing<TH, R, PASS, CW>::SC;
+  constexpr int NF = TH::NF, NFR = StreamRing<TH, R, PASS, CW, COMP>::NFR;
+  constexpr int SC = StreamRing<TH, R, PASS, CW, COMP>::SC;
   using S = StreamShape<R, NFR, SC>;
   constexpr int RB = S::RB, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
@@ -532,6 +590,10 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
         q.v[NF + a][r][c] = in ? stage_value<3, R>(fo, v, l, dt) : R(0);
       } else {
         q.v[a][r][c] = in ? stage_value<PASS, R>(fo, v, l, dt) : R(0);
+      }
+      if constexpr (PASS == 1 && COMP) {
+        const R lo = ld<R>(plane<R>(b.Flo, a, NP), boff);
+        q.v[NF + a][r][c] = in ? lo : R(0);
       }
     }
   }
@@ -600,11 +662,11 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
         stream_rows<TH, R, 1, false, NFR, NF, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc2, unused);
       }
     } else if (interior) {
-      stream_rows<TH, R, PASS, true, NFR, 0, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                             acc1);
+      stream_rows<TH, R, PASS, true, NFR, 0, -1, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
+                                                           &keepL[0][0], acc0, acc1);
     } else {
-      stream_rows<TH, R, PASS, false, NFR, 0, -1>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                              acc1);
+      stream_rows<TH, R, PASS, false, NFR, 0, -1, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
+                                                            &keepL[0][0], acc0, acc1);
     }
     __syncthreads();  // every read of this iteration's rows is done
     if (it + 1 < nit) {
@@ -653,16 +715,30 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
   }
 }
 
-template <class TH, typename R, int PASS, int CW>
-__global__ void __launch_bounds__(kThreads) stream_pass(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
-  __shared__ typename StreamRing<TH, R, PASS, CW>::type q;
+template <class TH, typename R, int PASS, int CW, bool COMP>
+__device__ __forceinline__ void stream_pass_body(const PassBufs& b, const typename TH::TP& tp, const Geom& g,
+                                                 const SLaunch& L) {
+  __shared__ typename StreamRing<TH, R, PASS, CW, COMP>::type q;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, L.nblocks, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
   const int sxi = t / L.nsy, syi = t - sxi * L.nsy;
   const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
-  stream_strip<TH, R, PASS, CW>(q, red, b, tp, g, sxi * L.sx, L.sx, syi * StreamRing<TH, R, PASS, CW>::shape::TYW,
-                                kill, t);
+  stream_strip<TH, R, PASS, CW, COMP>(q, red, b, tp, g, sxi * L.sx, L.sx,
+                                      syi * StreamRing<TH, R, PASS, CW, COMP>::shape::TYW, kill, t);
+}
+
+template <class TH, typename R, int PASS, int CW>
+__global__ void __launch_bounds__(kThreads) stream_pass(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
+  stream_pass_body<TH, R, PASS, CW, false>(b, tp, g, L);
+}
+
+// Pass 1 or 4 of the compensated FP32 form, held to the register budget of 3
+// waves per SIMD (the plain form's pass 4 budget).
+template <class TH, typename R, int PASS, int CW>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+stream_pass_comp(PassBufs b, typename TH::TP tp, Geom g, SLaunch L) {
+  stream_pass_body<TH, R, PASS, CW, true>(b, tp, g, L);
 }
 
 }  // namespace ss
