@@ -466,6 +466,9 @@ def main():
     ap.add_argument("--fused41", type=int, default=-1,
                     help="with --tiles: 1 runs pass 4 and the next step's pass 1 as one sweep")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
+    ap.add_argument("--reduced-form", choices=["fp32", "fp32c"], default="fp32",
+                    help="throughput workload: the FP32 form whose rate is the headline value (fp32c: with the "
+                         "compensated field update)")
     ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge"], default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
                          "batch: many baby-Skyrme relaxations of uneven sizes; converge: time-to-converge of the "
@@ -528,7 +531,9 @@ def main():
     n = args.n
     th, p, cfg = chiral_workload(n)
     print(f"[bench] building {n}x{n} chiral workload", file=sys.stderr, flush=True)
-    sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, "fp32", args.steps, args.warmup, args.graph_steps,
+    form = args.reduced_form
+    comp = form == "fp32c"
+    sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, form, args.steps, args.warmup, args.graph_steps,
                                              autotune=not args.no_autotune,
                                              tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None,
                                              fused23=args.fused23, fused41=args.fused41)
@@ -548,19 +553,22 @@ def main():
     N = n * n
     nf = 4
     ups32 = N * args.steps / t32
-    print(f"[bench] fp32 form: {t32 / args.steps * 1e3:.3f} ms/step, {ups32:.3e} updates/s", file=sys.stderr,
+    print(f"[bench] {form} form: {t32 / args.steps * 1e3:.3f} ms/step, {ups32:.3e} updates/s", file=sys.stderr,
           flush=True)
 
     # dominant-kernel roofline from HIP events on the context stream; with the
     # 4+1 sweep the dominant launch is that sweep (pass 4's 7 words per site
-    # plus pass 1's 2) and pass 1 runs only in the first of the timed steps
+    # plus pass 1's 2) and pass 1 runs only in the first of the timed steps.
+    # The compensated form's pass 4 also reads and writes the low words (9
+    # words per site), its pass 1 reads them (3): 19 words per update.
     passes = sim32.time_passes(E32, 20)
     w = 4
     f41 = bool(tiles32.get("fused41", False))
-    alg_pass4 = (9 if f41 else 7) * nf * w * N
+    alg_pass4 = (9 if (f41 or comp) else 7) * nf * w * N
     pass4_gbs = alg_pass4 / (passes[4] * 1e-3) / 1e9
     step_ms_kernels = float(passes.sum())
-    alg_step = 16 * nf * w * N
+    words_step = 19 if comp else 16
+    alg_step = words_step * nf * w * N
     step_gbs = alg_step / (step_ms_kernels * 1e-3) / 1e9
     sim32.close()
 
@@ -580,6 +588,21 @@ def main():
                     "tile_rows_per_pass": tiles64["rows"]}
         print(f"[bench] fp64 form: {t64 / args.steps * 1e3:.3f} ms/step, {ups64:.3e} updates/s, "
               f"relE(fp32 vs fp64)={relE:.2e}", file=sys.stderr, flush=True)
+
+    resultc = None
+    if not args.profile_only and not comp:
+        simc_, Ecc, _, _, tcc = time_form(th, p, cfg, "fp32c", args.steps, args.warmup, args.graph_steps,
+                                          autotune=not args.no_autotune)
+        upsc = N * args.steps / tcc
+        resultc = {"updates_per_s": upsc, "ms_per_step": tcc / args.steps * 1e3,
+                   "hbm_frac_alg": upsc * 19 * nf * 4 / (HBM_PEAK_GBS * 1e9),
+                   "speedup_vs_fp64": (upsc / result64["updates_per_s"]) if result64 else None,
+                   "tile_rows_per_pass": simc_.tiles["rows"],
+                   "note": "compensated field update (FP32 hi + lo words, FP64 combine and FP64 energy of hi + lo),"
+                           " 19 words per update; the form the precision workload converges with"}
+        simc_.close()
+        print(f"[bench] fp32c form: {tcc / args.steps * 1e3:.3f} ms/step, {upsc:.3e} updates/s",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and not args.profile_only:
@@ -618,12 +641,17 @@ def main():
     v4 = tiles32["variants"][3]
     if f41 and v4 < 3:
         v4 = 3
-    kernel = (f"stream_pass<ChiralMagnet, float, {41 if f41 else 4}, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
-              f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>")
+    if comp:
+        kernel = (f"stream_pass_comp<ChiralMagnet, float, 4, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
+                  f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}, true>")
+    else:
+        kernel = (f"stream_pass<ChiralMagnet, float, {41 if f41 else 4}, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
+                  f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>")
     traffic = None
     if not args.no_traffic and not args.profile_only:
         child = ["--tiles", ",".join(str(v) for v in tiles32["variants"]),
-                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)), "--n", str(n)]
+                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)), "--n", str(n),
+                 "--reduced-form", form]
         traffic = measure_traffic(child, kernel)
         if traffic:
             print(f"[bench] traffic of {kernel}: {traffic['bytes'] / 1e9:.3f} GB per launch "
@@ -646,13 +674,14 @@ def main():
         "fused23": int(tiles32.get("fused23", False)),
         "fused41": int(f41),
         "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
-                               "FP32 fields+stencils, FP64 energy sums and arrest test",
+                               "FP32 fields+stencils, FP64 energy sums and arrest test"
+                               + ("; compensated field update (FP32 hi + lo)" if comp else ""),
                    "lattice": [n, n], "fields": nf, "graph_steps": args.graph_steps,
                    "tile_rows_per_pass": tiles32["rows"], "tile_shape": "64 y x rows x",
                    "fused_passes_2_3": tiles32.get("fused23", False),
                    "fused_passes_4_1": f41,
                    "parallelism": "single GPU"},
-        "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
+        "hbm_frac_alg": ups32 * words_step * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "traffic_read_write": [traffic["read_bytes"], traffic["write_bytes"]] if traffic else None,
@@ -661,6 +690,7 @@ def main():
                      "step_kernels_gbs": step_gbs, "pass_ms": [float(x) for x in passes]},
         "fp64_form": result64,
         "speedup_fp32_vs_fp64": (ups32 / result64["updates_per_s"]) if result64 else None,
+        "fp32c_form": resultc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
