@@ -195,7 +195,8 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * per site; where a theory's rings do not fit that wide, they run as 3-5).
  * ss_autotune_tiles times every pass in every variant ``reps`` times on the
  * live buffers, keeps the fastest per pass, and reports chosen[0..3] and
- * ms[9*(pass-1)+variant] (ms holds 48 entries, see ss_set_fused_stages);
+ * ms[9*(pass-1)+variant] (ms holds 51 entries, see ss_set_fused_stages;
+ * variants a pass does not run in read 0);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
  * do not depend on the variants; energies only through FP64 summation order. */
@@ -204,18 +205,20 @@ int ss_get_tile_variants(ss_ctx* ctx, int* variants, int* rows);
 int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
 
 /* Fused sweeps (no counterpart in the reference).  SS_FUSE_23: passes 2 and
- * 3 of a step read the same inputs (F, V, l1), so one streaming sweep builds
- * both stage fields from one set of loads and writes l2 and l3.  SS_FUSE_41:
+ * 3 of a step read the same inputs (F, V, l1), so one sweep builds both stage
+ * fields from one set of loads and writes l2 and l3 (as strips, or as LDS
+ * tiles holding both fields where such a tile fits).  SS_FUSE_41:
  * the field after a step, F' = Temp4 + dt (l1 - 2 l2 + l3)/6 (then the unit
  * constraint), is pointwise in the inputs of pass 4, so one sweep builds Temp4
  * and F' from one set of loads, finishes the step (F', V', err partials) and
  * runs pass 1 of the next step on F' (its l1 and energy); a call then runs
  * pass 1 only in its first step, and l1 alternates between two buffers.
- * ss_autotune_tiles times both sweeps in the streaming variants (ms[36..41]
- * for 2+3, ms[42..47] for 4+1, variants 3..8) and switches each on when it beats the best
+ * ss_autotune_tiles times the 2+3 sweep in every variant it runs in
+ * (ms[36..44], variants 0..8) and the 4+1 sweep in the streaming variants
+ * (ms[45..50], variants 3..8) and switches each on when it beats the best
  * pair of separate passes; ss_set_fused_stages forces them (a fused sweep uses
- * the variant of pass 2 or pass 4, at least 3) and fails when the two halves
- * do not fit one LDS ring for the theory and precision.  Results equal the
+ * the variant of pass 2 or pass 4 where it runs in it, else variant 3) and
+ * fails when neither fits for the theory and precision.  Results equal the
  * separate passes' up to the FP64 summation order of the energy. */
 #define SS_FUSE_23 1
 #define SS_FUSE_41 2

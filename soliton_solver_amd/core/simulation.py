@@ -426,21 +426,24 @@ class Simulation:
         fastest per pass (the relaxation state is unchanged).  Variants 0-2 are LDS
         tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows and
         64 columns in the streaming form, 6-8 the same strips 128 columns wide; the
-        fused sweeps of passes 2+3 and 4+1 are timed in the streaming variants.  Returns the chosen variants, their rows per tile/strip,
-        the fused flags and the ms tables ([pass][variant], fused per variant 3-5)."""
+        fused sweep of passes 2+3 is timed in every variant it runs in (tiles where
+        two stage fields fit in LDS, strips) and the 4+1 sweep in the streaming
+        variants.  Returns the chosen variants, their rows per tile/strip, the fused
+        flags and the ms tables ([pass][variant]; fused 2+3 per variant 0-8, fused
+        4+1 per variant 3-8; 0 where a sweep does not run)."""
         chosen = np.zeros(4, dtype=np.int32)
-        nst = self.NUM_SHAPES - 3
-        ms = np.zeros(4 * self.NUM_SHAPES + 2 * nst)
+        nv = self.NUM_SHAPES
+        ms = np.zeros(5 * nv + (nv - 3))
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
-        info["ms"] = ms[:4 * self.NUM_SHAPES].reshape(4, self.NUM_SHAPES)
-        info["ms_fused23"] = ms[4 * self.NUM_SHAPES:4 * self.NUM_SHAPES + nst]
-        info["ms_fused41"] = ms[4 * self.NUM_SHAPES + nst:]
+        info["ms"] = ms[:4 * nv].reshape(4, nv)
+        info["ms_fused23"] = ms[4 * nv:5 * nv]
+        info["ms_fused41"] = ms[5 * nv:]
         return info
 
     def set_fused_stages(self, fuse23: bool, fuse41: bool = False) -> None:
-        """Run passes 2+3 and/or pass 4 with the next step's pass 1 as single streaming
-        sweeps (see ``ss_set_fused_stages``)."""
+        """Run passes 2+3 and/or pass 4 with the next step's pass 1 as single sweeps
+        (see ``ss_set_fused_stages``)."""
         flags = (_lib.FUSE_23 if fuse23 else 0) | (_lib.FUSE_41 if fuse41 else 0)
         self._check(_lib.lib().ss_set_fused_stages(self._ctx, flags))
 

@@ -49,16 +49,32 @@ inline Launch make_launch(const Geom& g) {
   return L;
 }
 
+// Components of the LDS tile of a pass: PASS 23 (passes 2 and 3 in one sweep)
+// holds Temp2 in components 0..NF-1 and Temp3 in NF..2NF-1.
+template <class TH, int PASS>
+constexpr int tile_nf() {
+  return PASS == 23 ? 2 * TH::NF : TH::NF;
+}
+
+// The tile of PASS 23 with TX rows fits in 64 KiB of LDS.
+template <class TH, typename R, int TX>
+constexpr bool fused23_tile_fits() {
+  return (TX + 4) * 2 * TH::NF * (TY + 4) * (int)sizeof(R) <= 65536;
+}
+
 // One tile of one pass: stage field into LDS, stencils + variation from LDS,
 // outputs for the sites of the tile, and the block's partial (energy sum for
 // pass 1, signed max / max |EG| for pass 4) written to part0[t] / part1[t].
+// PASS 23: both stage fields of passes 2 and 3 from one set of loads, l2 ->
+// lout and l3 -> lout2 (the rows are swept once per stage field).
 // Shared by the single-lattice kernel and the batched kernel.
 template <class TH, typename R, int PASS, int TXR = 0, bool COMP = false>
-__device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* red, const PassBufs& b,
+__device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& tile, double* red, const PassBufs& b,
                                            const typename TH::TP& tp, const Geom& g, int x0, int y0, bool kill,
                                            int t) {
   using S = TileShape<R, TH::NF, TXR>;
   constexpr int NF = TH::NF;
+  constexpr int NFT = tile_nf<TH, PASS>();
   const R* __restrict__ F = (const R*)b.F;
   const R* __restrict__ V = (const R*)b.V;
   const R* __restrict__ l1 = (const R*)b.l1;
@@ -84,7 +100,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       const long long o = off + (long long)a * g.N;
-      R val = R(0);
+      R val = R(0), val3 = R(0);
       if (PASS == 4) {
         Vk[k][a] = R(0);
         L2k[k][a] = R(0);
@@ -99,6 +115,9 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
             val = fo + half * (dt * v);
           } else if (PASS == 3) {
             val = fo + half * (dt * (v + half * l1[o]));
+          } else if (PASS == 23) {
+            val = fo + half * (dt * v);
+            val3 = fo + half * (dt * (v + half * l1[o]));
           } else {
             const R q = l2[o];
             val = fo + dt * (v + half * q);
@@ -108,6 +127,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
         }
       }
       tile.v[a][lx + 2][lane + 2] = val;
+      if constexpr (PASS == 23) tile.v[NF + a][lx + 2][lane + 2] = val3;
     }
   }
   // Phase B: the 2-site halo ring (2 rows above and below, 2 columns either side).
@@ -133,7 +153,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       const long long o = off + (long long)a * g.N;
-      R val = R(0);
+      R val = R(0), val3 = R(0);
       if (in) {
         if (PASS <= 1) {
           val = F[o];
@@ -141,43 +161,52 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
           const R v = kill ? R(0) : V[o];
           if (PASS == 2) val = F[o] + half * (dt * v);
           else if (PASS == 3) val = F[o] + half * (dt * (v + half * l1[o]));
-          else val = F[o] + dt * (v + half * l2[o]);
+          else if (PASS == 23) {
+            val = F[o] + half * (dt * v);
+            val3 = F[o] + half * (dt * (v + half * l1[o]));
+          } else val = F[o] + dt * (v + half * l2[o]);
         }
       }
       tile.v[a][r][c] = val;
+      if constexpr (PASS == 23) tile.v[NF + a][r][c] = val3;
     }
   }
   __syncthreads();
 
-  // Phase C: march down the thread's rows with an x-window in registers.
+  // Phase C: march down the thread's rows with an x-window in registers (PASS
+  // 23: once per stage field, components ao .. ao+NF-1 of the tile).
   const int cy = lane + 2;
+  double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
+  double acc1 = 0.0;
+#pragma unroll
+  for (int stg = 0; stg < NFT / NF; ++stg) {
+  const int ao = stg * NF;
   R win[NF][5];
 #pragma unroll
   for (int a = 0; a < NF; ++a) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) win[a][j] = tile.v[a][w * RW + j][cy];
+    for (int j = 0; j < 4; ++j) win[a][j] = tile.v[ao + a][w * RW + j][cy];
   }
-  double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
-  double acc1 = 0.0;
 #pragma unroll
   for (int k = 0; k < RW; ++k) {
     const int lx = w * RW + k, cx = lx + 2;
     const int x = x0 + lx;
 #pragma unroll
-    for (int a = 0; a < NF; ++a) win[a][4] = tile.v[a][cx + 2][cy];
+    for (int a = 0; a < NF; ++a) win[a][4] = tile.v[ao + a][cx + 2][cy];
     if (x < g.X && y < g.Y) {
       const SiteInfo s = site_info(g, x, y);
       R f[NF], d1[NF][2], d2[NF][3], eg[NF];
 #pragma unroll
-      for (int a = 0; a < NF; ++a) window_derivs<R, R, NF>(tile, a, win[a], cx, cy, g, s, f[a], d1[a], d2[a]);
+      for (int a = 0; a < NF; ++a)
+        window_derivs<R, R, NFT>(tile, ao + a, win[a], cx, cy, g, s, f[a], d1[a], d2[a]);
       TH::template variation<R>(tp, g, s, f, d1, d2, eg);
       const long long off = (long long)x * g.Y + y;
       if (PASS == 0) {
         R* out = (R*)b.lout;
 #pragma unroll
         for (int a = 0; a < NF; ++a) out[off + a * g.N] = eg[a];
-      } else if (PASS <= 3) {
-        R* out = (R*)b.lout;
+      } else if (PASS <= 3 || PASS == 23) {
+        R* out = (R*)((PASS == 23 && stg) ? b.lout2 : b.lout);
 #pragma unroll
         for (int a = 0; a < NF; ++a) out[off + a * g.N] = -dt * eg[a];
       }
@@ -185,7 +214,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
         // FP64 energy density from the same window values (no LDS re-read).
         double fd[NF], d1d[NF][2];
 #pragma unroll
-        for (int a = 0; a < NF; ++a) window_d1<double, R, NF>(tile, a, win[a], cx, cy, g, s, fd[a], d1d[a]);
+        for (int a = 0; a < NF; ++a) window_d1<double, R, NFT>(tile, a, win[a], cx, cy, g, s, fd[a], d1d[a]);
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
       if (PASS == 4) {
@@ -240,6 +269,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, TH::NF, TXR>& tile, double* r
       for (int j = 0; j < 4; ++j) win[a][j] = win[a][j + 1];
     }
   }
+  }  // stage fields
   if (PASS == 1) {
     const double s = block_sum<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = s;
@@ -255,7 +285,7 @@ template <class TH, typename R, int PASS, int TXR = 0, bool COMP = false>
 __global__ void __launch_bounds__(kThreads)
 stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
   using S = TileShape<R, TH::NF, TXR>;
-  __shared__ Tile<R, TH::NF, TXR> tile;
+  __shared__ Tile<R, tile_nf<TH, PASS>(), TXR> tile;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
@@ -796,6 +826,20 @@ struct TheoryLaunchers {
     return fits && (prec ? effective_shape<float>(3, g) : effective_shape<double>(3, g)) == 3 && !TH::kL2Normalize;
   }
 
+  // Passes 2+3 as one sweep in shape variant v: strips where can_fuse23, tiles
+  // (v < 3) where the two-field tile fits in LDS.
+  template <typename R>
+  static int fuse23_variant_r(int v, const Geom& g) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (v >= 3) return effective_shape<R>(v, g) >= 3 && StreamRing<TH, R, 23>::shape::kFits;
+    if (v == 2) return fused23_tile_fits<TH, R, r2>();
+    if (v == 1) return fused23_tile_fits<TH, R, r1>();
+    return fused23_tile_fits<TH, R, r0>();
+  }
+  static int fuse23_variant(int prec, int v, const Geom& g) {
+    return prec ? fuse23_variant_r<float>(v, g) : fuse23_variant_r<double>(v, g);
+  }
+
   static int can_comp(int prec, const Geom& g) {
     return prec && StreamRing<TH, float, 1, 1, true>::shape::kFits && effective_shape<float>(3, g) == 3 &&
            !TH::kL2Normalize;
@@ -813,6 +857,13 @@ struct TheoryLaunchers {
       case 2: hipLaunchKernelGGL((stage_pass<TH, R, 2, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 3: hipLaunchKernelGGL((stage_pass<TH, R, 3, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 4: hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR>), grid, block, 0, st, b, tp, g, L); break;
+      case 5:
+        if constexpr (fused23_tile_fits<TH, R, TXR>()) {
+          hipLaunchKernelGGL((stage_pass<TH, R, 23, TXR>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       case 7:
         if constexpr (sizeof(R) == 4) {
           hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR, true>), grid, block, 0, st, b, tp, g, L);
@@ -830,7 +881,7 @@ struct TheoryLaunchers {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
     v = effective_shape<R>(v, g);
-    if (pass == 5 || pass == 6 || pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
+    if (pass == 6 || pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_v<R>(pass, v, b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -1020,6 +1071,7 @@ struct TheoryOps {
   int (*can_fuse23)(int, const Geom&);
   int (*can_fuse41)(int, const Geom&);
   int (*can_comp)(int, const Geom&);
+  int (*fuse23_variant)(int, int, const Geom&);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -1046,6 +1098,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::can_fuse23,                 \
                                       &ss::TheoryLaunchers<TH>::can_fuse41,                 \
                                       &ss::TheoryLaunchers<TH>::can_comp,                   \
+                                      &ss::TheoryLaunchers<TH>::fuse23_variant,             \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

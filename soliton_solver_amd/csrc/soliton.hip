@@ -239,7 +239,7 @@ struct ss_ctx {
   int ntiles = 0;          // capacity of each partials row (the most tiles over the shape variants)
   int nt0 = 0;             // tiles of the default shape (energy, charge and pass-0 kernels)
   int tv[5] = {0, 0, 0, 0, 0};  // tile-shape variant of each fused pass (index = pass)
-  int fuse23 = 0;          // passes 2 and 3 as one streaming sweep (variant tv[2], at least 3)
+  int fuse23 = 0;          // passes 2 and 3 as one sweep (variant tv[2]: a strip, or a tile where two fields fit)
   int fuse41 = 0;          // pass 4 and the next step's pass 1 as one sweep (variant tv[4], at least 3)
   void* l1alt = nullptr;   // with fuse41: l1 of the steps that start from parity 1
   float* Flo = nullptr;    // SS_PREC_FP32C: low words of the field (nf*N, updated in place by pass 4)
@@ -365,6 +365,12 @@ static bool can41(const ss_ctx* c) { return c->prec != SS_PREC_FP32C && c->ops->
 // The fused 4+1 schedule is active (reference flow only).
 static bool use41(const ss_ctx* c) {
   return c->fuse41 && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec != SS_PREC_FP32C;
+}
+
+// Shape variant of the fused 2+3 sweep: tv[2] where the sweep exists in it,
+// else the first strip variant.
+static int fused23_v(const ss_ctx* c) {
+  return c->ops->fuse23_variant(c->prec, c->tv[2], c->g) ? c->tv[2] : 3;
 }
 
 // Pass 1 of the step: in the compensated form the streaming pass 1 (id 8),
@@ -504,7 +510,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (c->fuse23) {
     b.lout = c->l[1];
     b.lout2 = c->l[2];
-    HIPCHK(c, c->ops->pass(c->prec, 5, c->tv[2] < 3 ? 3 : c->tv[2], b, c->tp, c->g, c->stream));
+    HIPCHK(c, c->ops->pass(c->prec, 5, fused23_v(c), b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate pass 3
   } else {
@@ -878,7 +884,8 @@ int ss_set_precision(ss_ctx* c, int prec) {
   free_graphs(c);
   free_fields(c);
   c->prec = prec;
-  if (c->fuse23 && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
+  if (c->fuse23 && !c->ops->fuse23_variant(prec, c->tv[2], c->g) && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
+  if (c->fuse23) c->tv[2] = fused23_v(c);
   if (c->fuse41 && !can41(c)) c->fuse41 = 0;
   if (alloc_fields(c)) return -1;
   HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
@@ -1105,10 +1112,14 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   // passes 2 + 3 fused (streaming variants only) against the best separate pair
   float best23 = 0.f;
   int pick23 = -1;
-  if (rc == 0 && c->ops->can_fuse23(c->prec, c->g)) {
+  {
     b.lout = c->l[1];
     b.lout2 = c->l[2];
-    for (int v = 3; v < c->ops->num_shapes && rc == 0; ++v) {
+    for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
+      if (!c->ops->fuse23_variant(c->prec, v, c->g)) {
+        if (ms) ms[4 * c->ops->num_shapes + v] = 0.0;
+        continue;
+      }
       hipError_t e = c->ops->pass(c->prec, 5, v, b, c->tp, c->g, c->stream);
       if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
       for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, 5, v, b, c->tp, c->g, c->stream);
@@ -1121,7 +1132,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         break;
       }
       t /= reps;
-      if (ms) ms[4 * c->ops->num_shapes + (v - 3)] = t;
+      if (ms) ms[4 * c->ops->num_shapes + v] = t;
       if (pick23 < 0 || t < best23) {
         best23 = t;
         pick23 = v;
@@ -1151,7 +1162,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         break;
       }
       t /= reps;
-      if (ms) ms[4 * c->ops->num_shapes + (c->ops->num_shapes - 3) + (v - 3)] = t;
+      if (ms) ms[5 * c->ops->num_shapes + (v - 3)] = t;
       if (pick41 < 0 || t < best41) {
         best41 = t;
         pick41 = v;
@@ -1187,14 +1198,14 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
 int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (!c) return fail(nullptr, "null context");
   if (flags & ~(SS_FUSE_23 | SS_FUSE_41)) return fail(c, "unknown fused-stage flags 0x%x", flags);
-  if ((flags & SS_FUSE_23) && !c->ops->can_fuse23(c->prec, c->g))
+  if ((flags & SS_FUSE_23) && !c->ops->fuse23_variant(c->prec, c->tv[2], c->g) && !c->ops->can_fuse23(c->prec, c->g))
     return fail(c, "passes 2 and 3 of theory %s cannot share one sweep at this precision and size", c->ops->name);
   if ((flags & SS_FUSE_41) && !can41(c))
     return fail(c, "pass 4 and pass 1 of theory %s cannot share one sweep at this precision and size", c->ops->name);
   if ((flags & SS_FUSE_41) && ensure_l1alt(c)) return -1;
   c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
   c->fuse41 = (flags & SS_FUSE_41) ? 1 : 0;
-  if (c->fuse23 && c->tv[2] < 3) c->tv[2] = 3;
+  if (c->fuse23) c->tv[2] = fused23_v(c);
   if (c->fuse41 && c->tv[4] < 3) c->tv[4] = 3;
   free_graphs(c);
   return 0;

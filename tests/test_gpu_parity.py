@@ -442,7 +442,8 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     ([0, 3, 4, 5], False, False), ([0, 3, 0, 3], True, False), ([2, 5, 0, 4], True, False),
     ([0, 1, 2, 3], False, True), ([1, 3, 0, 5], True, True), ([2, 4, 4, 4], True, True),
     ([6, 7, 8, 6], False, False), ([0, 6, 7, 8], True, False), ([7, 8, 1, 6], True, True),
-    ([3, 6, 0, 7], False, True)])
+    ([3, 6, 0, 7], False, True), ([0, 1, 0, 3], True, False), ([2, 2, 2, 2], True, False),
+    ([1, 2, 1, 6], True, True)])
 def test_tile_shape_variants_match_oracle(variants, fused, fused41):
     """Every launch-shape variant of the fused passes (tiles and streaming strips,
     lattice 70 x 130: partial strips and partial column blocks), with and without
@@ -571,6 +572,52 @@ def test_fused41_matches_separate_passes(precision, graphs):
     assert dF <= tF and dV <= tF
     a.close()
     b.close()
+
+
+_FUSED23_TILE_CASES = {
+    "Chiral magnet": (lambda th: H.chiral_params(th, 70, 130),
+                      lambda th, pi, pf: H.chiral_field(th, pi, pf, seed=5, rough=True)),
+    **_FP32_CASES,
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", list(_FUSED23_TILE_CASES))
+def test_fused23_tiles_match_separate_passes(name, precision):
+    """Passes 2+3 in one LDS-tile sweep (both stage fields in one tile) against
+    the separate tile passes, in the tile variants whose two-field tile fits
+    (the rest fall back to the streaming sweep, or have no fused sweep): the
+    same fields and energies after 15 steps."""
+    th = ssa.load_theory(name)
+    mkp, mkf = _FUSED23_TILE_CASES[name]
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    ran_tile = False
+    for v in (1, 2):
+        a = _sim(th, p, F, precision=precision)
+        b = _sim(th, p, F, precision=precision)
+        a.set_tile_variants([v, v, v, v])
+        b.set_tile_variants([v, v, v, v])
+        try:
+            b.set_fused_stages(True, False)
+        except ValueError:  # neither a two-field tile nor a fused strip fits this theory and precision
+            a.close()
+            b.close()
+            continue
+        ran_tile |= b.tile_variants()["variants"][1] == v
+        E0 = a.compute_energy()
+        Ea, _, _ = a.steps(E0, 15)
+        Eb, _, _ = b.steps(E0, 15)
+        dF = np.abs(a.get_field() - b.get_field()).max() / max(1.0, np.abs(F).max())
+        print(f"[fused23 tiles {name} {precision} v={v} -> {b.tile_variants()['variants']}] "
+              f"relE={H.rel(Eb, Ea):.2e} max|dF|={dF:.2e}")
+        assert H.rel(Eb, Ea) < tol and dF <= tol
+        a.close()
+        b.close()
+    if name == "Chiral magnet":
+        assert ran_tile  # FP64 and FP32 chiral tiles of 4..16 rows hold both fields
 
 
 def test_fused41_survives_precision_switch_and_autotune():
