@@ -546,7 +546,7 @@ def main():
         print(f"[bench]   fused passes 2+3 ms by shape variant: "
               + " ".join(f"{x:.4f}" for x in tiles32["ms_fused23"]) + f"; fused: {tiles32['fused23']}",
               file=sys.stderr, flush=True)
-        print(f"[bench]   fused passes 4+1 ms by strip variant: "
+        print(f"[bench]   fused passes 4+1 ms by shape variant: "
               + " ".join(f"{x:.4f}" for x in tiles32["ms_fused41"]) + f"; fused: {tiles32['fused41']}",
               file=sys.stderr, flush=True)
     F0 = None

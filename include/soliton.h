@@ -195,7 +195,7 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * per site; where a theory's rings do not fit that wide, they run as 3-5).
  * ss_autotune_tiles times every pass in every variant ``reps`` times on the
  * live buffers, keeps the fastest per pass, and reports chosen[0..3] and
- * ms[9*(pass-1)+variant] (ms holds 51 entries, see ss_set_fused_stages;
+ * ms[9*(pass-1)+variant] (ms holds 54 entries, see ss_set_fused_stages;
  * variants a pass does not run in read 0);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
@@ -211,11 +211,13 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * the field after a step, F' = Temp4 + dt (l1 - 2 l2 + l3)/6 (then the unit
  * constraint), is pointwise in the inputs of pass 4, so one sweep builds Temp4
  * and F' from one set of loads, finishes the step (F', V', err partials) and
- * runs pass 1 of the next step on F' (its l1 and energy); a call then runs
- * pass 1 only in its first step, and l1 alternates between two buffers.
- * ss_autotune_tiles times the 2+3 sweep in every variant it runs in
- * (ms[36..44], variants 0..8) and the 4+1 sweep in the streaming variants
- * (ms[45..50], variants 3..8) and switches each on when it beats the best
+ * runs pass 1 of the next step on F' (its l1 and energy), followed by one
+ * finalize launch for both; a call then runs pass 1 only in its first step,
+ * and l1 alternates between two buffers.
+ * Both sweeps also run as LDS tiles holding two fields where such a tile
+ * fits.  ss_autotune_tiles times each sweep in every variant it runs in
+ * (ms[36..44] for 2+3, ms[45..53] for 4+1, variants 0..8) and switches each
+ * on when it beats the best
  * pair of separate passes; ss_set_fused_stages forces them (a fused sweep uses
  * the variant of pass 2 or pass 4 where it runs in it, else variant 3) and
  * fails when neither fits for the theory and precision.  Results equal the

@@ -426,14 +426,13 @@ class Simulation:
         fastest per pass (the relaxation state is unchanged).  Variants 0-2 are LDS
         tiles (default rows, halved, quartered), 3-5 strips of 64/128/256 rows and
         64 columns in the streaming form, 6-8 the same strips 128 columns wide; the
-        fused sweep of passes 2+3 is timed in every variant it runs in (tiles where
-        two stage fields fit in LDS, strips) and the 4+1 sweep in the streaming
-        variants.  Returns the chosen variants, their rows per tile/strip, the fused
-        flags and the ms tables ([pass][variant]; fused 2+3 per variant 0-8, fused
-        4+1 per variant 3-8; 0 where a sweep does not run)."""
+        fused sweeps of passes 2+3 and 4+1 are timed in every variant they run in
+        (tiles where two fields fit in LDS, strips).  Returns the chosen variants,
+        their rows per tile/strip, the fused flags and the ms tables ([pass][variant],
+        fused 2+3 and 4+1 per variant; 0 where a sweep does not run)."""
         chosen = np.zeros(4, dtype=np.int32)
         nv = self.NUM_SHAPES
-        ms = np.zeros(5 * nv + (nv - 3))
+        ms = np.zeros(6 * nv)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
         info["ms"] = ms[:4 * nv].reshape(4, nv)

@@ -50,15 +50,16 @@ inline Launch make_launch(const Geom& g) {
 }
 
 // Components of the LDS tile of a pass: PASS 23 (passes 2 and 3 in one sweep)
-// holds Temp2 in components 0..NF-1 and Temp3 in NF..2NF-1.
+// holds Temp2 in components 0..NF-1 and Temp3 in NF..2NF-1; PASS 41 (pass 4
+// and pass 1 of the next step) Temp4 and the new field F'.
 template <class TH, int PASS>
 constexpr int tile_nf() {
-  return PASS == 23 ? 2 * TH::NF : TH::NF;
+  return (PASS == 23 || PASS == 41) ? 2 * TH::NF : TH::NF;
 }
 
-// The tile of PASS 23 with TX rows fits in 64 KiB of LDS.
+// A two-field tile (PASS 23 or 41) with TX rows fits in 64 KiB of LDS.
 template <class TH, typename R, int TX>
-constexpr bool fused23_tile_fits() {
+constexpr bool fused_tile_fits() {
   return (TX + 4) * 2 * TH::NF * (TY + 4) * (int)sizeof(R) <= 65536;
 }
 
@@ -66,7 +67,11 @@ constexpr bool fused23_tile_fits() {
 // outputs for the sites of the tile, and the block's partial (energy sum for
 // pass 1, signed max / max |EG| for pass 4) written to part0[t] / part1[t].
 // PASS 23: both stage fields of passes 2 and 3 from one set of loads, l2 ->
-// lout and l3 -> lout2 (the rows are swept once per stage field).
+// lout and l3 -> lout2 (the rows are swept once per stage field).  PASS 41:
+// Temp4 and F' (new_field, pointwise in F, V, l1, l2, l3) for the tile and its
+// halo from one set of loads; pass 4 on Temp4 (F', V' -> Fout, Vout, err
+// partials to part0/part1), then pass 1 of the next step on F' (l1 -> lout,
+// energy partial -> part2).
 // Shared by the single-lattice kernel and the batched kernel.
 template <class TH, typename R, int PASS, int TXR = 0, bool COMP = false>
 __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& tile, double* red, const PassBufs& b,
@@ -90,8 +95,35 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
 
   // Phase A: the interior sites this thread computes (rows w*RW..w*RW+RW-1 of
   // column ``lane``), one coalesced row of 64 values per wave and component.
-  // Pass 4 keeps V and l2 of its own sites in registers for the combine.
-  R Vk[PASS == 4 ? RW : 1][NF], L2k[PASS == 4 ? RW : 1][NF];
+  // Pass 4 keeps V and l2 of its own sites in registers for the combine (the
+  // 4+1 sweep also l1 and l3).
+  constexpr bool kP4 = PASS == 4 || PASS == 41;
+  R Vk[kP4 ? RW : 1][NF], L2k[kP4 ? RW : 1][NF];
+  R L1k[PASS == 41 ? RW : 1][NF], L3k[PASS == 41 ? RW : 1][NF];
+  if constexpr (PASS == 41) {
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int lx = w * RW + k, x = x0 + lx;
+      const bool in = (x < g.X) && (y < g.Y);
+      const long long off = in ? (long long)x * g.Y + y : 0;
+      R f[NF], T4[NF], Fn[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        const long long o = off + (long long)a * g.N;
+        f[a] = in ? F[o] : R(0);
+        Vk[k][a] = (in && !kill) ? V[o] : R(0);
+        L1k[k][a] = in ? l1[o] : R(0);
+        L2k[k][a] = in ? l2[o] : R(0);
+        L3k[k][a] = in ? l3[o] : R(0);
+      }
+      new_field<TH, R>(tp, g, in, f, Vk[k], L1k[k], L2k[k], L3k[k], dt, T4, Fn);
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        tile.v[a][lx + 2][lane + 2] = T4[a];
+        tile.v[NF + a][lx + 2][lane + 2] = Fn[a];
+      }
+    }
+  } else {
 #pragma unroll
   for (int k = 0; k < RW; ++k) {
     const int lx = w * RW + k, x = x0 + lx;
@@ -130,6 +162,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
       if constexpr (PASS == 23) tile.v[NF + a][lx + 2][lane + 2] = val3;
     }
   }
+  }
   // Phase B: the 2-site halo ring (2 rows above and below, 2 columns either side).
   constexpr int kRing = 4 * RY + 4 * TXr;
   for (int idx = threadIdx.x; idx < kRing; idx += kThreads) {
@@ -150,6 +183,25 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
     const int gx = x0 - 2 + r, gy = y0 - 2 + c;
     const bool in = (gx >= 0) && (gx < g.X) && (gy >= 0) && (gy < g.Y);
     const long long off = (long long)gx * g.Y + gy;
+    if constexpr (PASS == 41) {
+      R f[NF], v[NF], a1[NF], a2[NF], a3[NF], T4[NF], Fn[NF];
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        const long long o = (in ? off : 0) + (long long)a * g.N;
+        f[a] = in ? F[o] : R(0);
+        v[a] = (in && !kill) ? V[o] : R(0);
+        a1[a] = in ? l1[o] : R(0);
+        a2[a] = in ? l2[o] : R(0);
+        a3[a] = in ? l3[o] : R(0);
+      }
+      new_field<TH, R>(tp, g, in, f, v, a1, a2, a3, dt, T4, Fn);
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        tile.v[a][r][c] = T4[a];
+        tile.v[NF + a][r][c] = Fn[a];
+      }
+      continue;
+    }
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       const long long o = off + (long long)a * g.N;
@@ -174,10 +226,11 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
   __syncthreads();
 
   // Phase C: march down the thread's rows with an x-window in registers (PASS
-  // 23: once per stage field, components ao .. ao+NF-1 of the tile).
+  // 23 and 41: once per field of the tile, components ao .. ao+NF-1; PASS 41
+  // runs pass 4 on the first and pass 1 on the second).
   const int cy = lane + 2;
-  double acc0 = (PASS == 4) ? -1.0e300 : 0.0;
-  double acc1 = 0.0;
+  double acc0 = kP4 ? -1.0e300 : 0.0;
+  double acc1 = 0.0, acc2 = 0.0;
 #pragma unroll
   for (int stg = 0; stg < NFT / NF; ++stg) {
   const int ao = stg * NF;
@@ -205,24 +258,30 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
         R* out = (R*)b.lout;
 #pragma unroll
         for (int a = 0; a < NF; ++a) out[off + a * g.N] = eg[a];
-      } else if (PASS <= 3 || PASS == 23) {
+      } else if (PASS <= 3 || PASS == 23 || (PASS == 41 && stg == 1)) {
         R* out = (R*)((PASS == 23 && stg) ? b.lout2 : b.lout);
 #pragma unroll
         for (int a = 0; a < NF; ++a) out[off + a * g.N] = -dt * eg[a];
       }
-      if (PASS == 1) {
+      if (PASS == 1 || (PASS == 41 && stg == 1)) {
         // FP64 energy density from the same window values (no LDS re-read).
         double fd[NF], d1d[NF][2];
 #pragma unroll
-        for (int a = 0; a < NF; ++a) window_d1<double, R, NFT>(tile, a, win[a], cx, cy, g, s, fd[a], d1d[a]);
-        acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+        for (int a = 0; a < NF; ++a)
+          window_d1<double, R, NFT>(tile, ao + a, win[a], cx, cy, g, s, fd[a], d1d[a]);
+        const double e1 = TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
+        if (PASS == 41)
+          acc2 += e1;
+        else
+          acc0 += e1;
       }
-      if (PASS == 4) {
+      if (PASS == 4 || (PASS == 41 && stg == 0)) {
         // RK4 combine.  With Temp4 = F + dt (V + l2/2) at the centre (f, from
         // LDS), F' = F + (k1 + 2 k2 + 2 k3 + k4)/6 = Temp4 + dt (l1 - 2 l2 + l3)/6,
         // so F is not re-read; V and l2 are the registers kept in phase A.
         // COMP (FP32): the new field is formed by compensated_field from F and
-        // its low words instead.
+        // its low words instead.  PASS 41: F' is the tile's second field
+        // (normalised by new_field), l1 and l3 the registers of phase A.
         R Fn[NF], Vn[NF];
         double Fd[COMP ? NF : 1];  // COMP: F + increment in FP64
         const R sixth = R(1.0 / 6.0);
@@ -232,11 +291,13 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
           const long long o = off + a * g.N;
           const R Vc = Vk[k][a];
           const R a2 = L2k[k][a];
-          const R a1 = l1[o], a3 = l3[o];
+          const R a1 = (PASS == 41) ? L1k[k][a] : l1[o], a3 = (PASS == 41) ? L3k[k][a] : l3[o];
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
           if constexpr (COMP)
             Fd[a] = compensated_sum(F[o], b.Flo[o], dt * (Vc + half * a2) + dt6 * (a1 - R(2) * a2 + a3));
+          else if constexpr (PASS == 41)
+            Fn[a] = tile.v[NF + a][cx][cy];
           else
             Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
           const double e = (double)eg[a];
@@ -251,7 +312,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
           for (int a = 0; a < NF; ++a) b.Flo[off + a * g.N] = lon[a];
           if (TH::kUnitConstraint && g.unit_mag) TH::template project<R>(tp, Vn, Fn);
         } else if (TH::kUnitConstraint && g.unit_mag) {
-          TH::template normalize<R>(tp, Fn);
+          if (PASS != 41) TH::template normalize<R>(tp, Fn);
           TH::template project<R>(tp, Vn, Fn);
         }
         R* Fo = (R*)b.Fout;
@@ -273,11 +334,15 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
   if (PASS == 1) {
     const double s = block_sum<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = s;
-  } else if (PASS == 4) {
+  } else if (kP4) {
     const double m0 = block_max<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = m0;
     const double m1 = block_max<kThreads>(acc1, red);
     if (threadIdx.x == 0) b.part1[t] = m1;
+    if (PASS == 41) {
+      const double e = block_sum<kThreads>(acc2, red);
+      if (threadIdx.x == 0) b.part2[t] = e;
+    }
   }
 }
 
@@ -832,12 +897,26 @@ struct TheoryLaunchers {
   static int fuse23_variant_r(int v, const Geom& g) {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     if (v >= 3) return effective_shape<R>(v, g) >= 3 && StreamRing<TH, R, 23>::shape::kFits;
-    if (v == 2) return fused23_tile_fits<TH, R, r2>();
-    if (v == 1) return fused23_tile_fits<TH, R, r1>();
-    return fused23_tile_fits<TH, R, r0>();
+    if (v == 2) return fused_tile_fits<TH, R, r2>();
+    if (v == 1) return fused_tile_fits<TH, R, r1>();
+    return fused_tile_fits<TH, R, r0>();
   }
   static int fuse23_variant(int prec, int v, const Geom& g) {
     return prec ? fuse23_variant_r<float>(v, g) : fuse23_variant_r<double>(v, g);
+  }
+  // Pass 4 + pass 1 as one sweep in shape variant v: strips where can_fuse41,
+  // tiles (v < 3) where the two-field tile fits.
+  template <typename R>
+  static int fuse41_variant_r(int v, const Geom& g) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (TH::kL2Normalize) return 0;
+    if (v >= 3) return effective_shape<R>(v, g) >= 3 && StreamRing<TH, R, 41>::shape::kFits;
+    if (v == 2) return fused_tile_fits<TH, R, r2>();
+    if (v == 1) return fused_tile_fits<TH, R, r1>();
+    return fused_tile_fits<TH, R, r0>();
+  }
+  static int fuse41_variant(int prec, int v, const Geom& g) {
+    return prec ? fuse41_variant_r<float>(v, g) : fuse41_variant_r<double>(v, g);
   }
 
   static int can_comp(int prec, const Geom& g) {
@@ -858,8 +937,15 @@ struct TheoryLaunchers {
       case 3: hipLaunchKernelGGL((stage_pass<TH, R, 3, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 4: hipLaunchKernelGGL((stage_pass<TH, R, 4, TXR>), grid, block, 0, st, b, tp, g, L); break;
       case 5:
-        if constexpr (fused23_tile_fits<TH, R, TXR>()) {
+        if constexpr (fused_tile_fits<TH, R, TXR>()) {
           hipLaunchKernelGGL((stage_pass<TH, R, 23, TXR>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
+      case 6:
+        if constexpr (fused_tile_fits<TH, R, TXR>() && !TH::kL2Normalize) {
+          hipLaunchKernelGGL((stage_pass<TH, R, 41, TXR>), grid, block, 0, st, b, tp, g, L);
           break;
         } else {
           return hipErrorInvalidValue;
@@ -881,7 +967,7 @@ struct TheoryLaunchers {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
     v = effective_shape<R>(v, g);
-    if (pass == 6 || pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
+    if (pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_v<R>(pass, v, b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -1072,6 +1158,7 @@ struct TheoryOps {
   int (*can_fuse41)(int, const Geom&);
   int (*can_comp)(int, const Geom&);
   int (*fuse23_variant)(int, int, const Geom&);
+  int (*fuse41_variant)(int, int, const Geom&);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -1099,6 +1186,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::can_fuse41,                 \
                                       &ss::TheoryLaunchers<TH>::can_comp,                   \
                                       &ss::TheoryLaunchers<TH>::fuse23_variant,             \
+                                      &ss::TheoryLaunchers<TH>::fuse41_variant,             \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

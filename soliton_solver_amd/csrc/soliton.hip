@@ -140,6 +140,38 @@ k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histE
   }
 }
 
+// The two finalizes of the 4+1 schedule in one launch: err of step n (as
+// k_fin_err), then the energy of the new field and the arrest decision (as
+// k_fin_energy with the step counter already advanced).
+__global__ void __launch_bounds__(1024)
+k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, DevState* st, double* histErr,
+                 double* histE, int cap, int killkinen) {
+  __shared__ double red[16];
+  double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    m0 = fmax(m0, p0[i]);
+    m1 = fmax(m1, p1[i]);
+    const double y = p2[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  const double a = block_max<1024>(m0, red);
+  const double b = block_max<1024>(m1, red);
+  const double E = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) {
+    st->err_max = a;
+    st->err_absmax = b;
+    const int step = st->step;
+    if (step < cap) histErr[step] = a;
+    st->step = step + 1;
+    st->last_energy = E;
+    st->kill = (killkinen && E > st->prev_energy) ? 1 : 0;
+    st->prev_energy = E;
+    if (step < cap) histE[step] = E;
+  }
+}
+
 // Plain sum of partials into out[0] (observables).
 __global__ void __launch_bounds__(1024) k_fin_sum(const double* part, int n, double* out) {
   __shared__ double red[16];
@@ -359,8 +391,18 @@ static int ensure_l1alt(ss_ctx* c) {
 static void* l1_buf(ss_ctx* c, int parity) { return (c->fuse41 && parity) ? c->l1alt : c->l[0]; }
 
 // The fused 4+1 sweep is available (not in the compensated form, whose pass 4
-// forms the new field from F and its low words).
-static bool can41(const ss_ctx* c) { return c->prec != SS_PREC_FP32C && c->ops->can_fuse41(c->prec, c->g); }
+// forms the new field from F and its low words): in the variant of pass 4 or
+// in strips.
+static bool can41(const ss_ctx* c) {
+  return c->prec != SS_PREC_FP32C &&
+         (c->ops->fuse41_variant(c->prec, c->tv[4], c->g) || c->ops->can_fuse41(c->prec, c->g));
+}
+
+// Shape variant of the fused 4+1 sweep: tv[4] where the sweep exists in it,
+// else the first strip variant.
+static int fused41_v(const ss_ctx* c) {
+  return c->ops->fuse41_variant(c->prec, c->tv[4], c->g) ? c->tv[4] : 3;
+}
 
 // The fused 4+1 schedule is active (reference flow only).
 static bool use41(const ss_ctx* c) {
@@ -525,17 +567,14 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   b.part0 = p0;
   b.part1 = p1;
   if (f41) {
-    const int v4 = c->tv[4] < 3 ? 3 : c->tv[4];
+    const int v4 = fused41_v(c);
     b.lout = l1_buf(c, parity ^ 1);
     b.part2 = p2;
     HIPCHK(c, c->ops->pass(c->prec, 6, v4, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     const int nt = c->ops->ntiles(c->prec, v4, c->g);
-    hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1, nt, c->st,
-                       histErr, c->hist_cap);
-    HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p2, nt, c->st, histE,
-                       c->hist_cap, c->g.killkinen);
+    hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
+                       (const double*)p2, nt, c->st, histErr, histE, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     return 0;
@@ -854,6 +893,7 @@ int ss_set_precision(ss_ctx* c, int prec) {
       c->prec = prec;
     }
     if (c->fuse41 && !can41(c)) c->fuse41 = 0;
+    if (c->fuse41) c->tv[4] = fused41_v(c);
     return 0;
   }
   const size_t nb = (size_t)n * elem_size(prec);
@@ -887,6 +927,7 @@ int ss_set_precision(ss_ctx* c, int prec) {
   if (c->fuse23 && !c->ops->fuse23_variant(prec, c->tv[2], c->g) && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
   if (c->fuse23) c->tv[2] = fused23_v(c);
   if (c->fuse41 && !can41(c)) c->fuse41 = 0;
+  if (c->fuse41) c->tv[4] = fused41_v(c);
   if (alloc_fields(c)) return -1;
   HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->V[0], nV, nb, hipMemcpyDeviceToDevice, c->stream));
@@ -1144,12 +1185,19 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   const float sep41 = bestp[4] + bestp[1];
   float best41 = 0.f;
   int pick41 = -1;
-  if (rc == 0 && can41(c)) rc = ensure_l1alt(c);
-  if (rc == 0 && can41(c)) {
+  bool any41 = false;
+  for (int v = 0; v < c->ops->num_shapes && c->prec != SS_PREC_FP32C; ++v)
+    any41 = any41 || c->ops->fuse41_variant(c->prec, v, c->g);
+  if (rc == 0 && any41) rc = ensure_l1alt(c);
+  if (rc == 0 && any41) {
     b.lout = (b.l1 == c->l[0]) ? c->l1alt : c->l[0];
     b.lout2 = nullptr;
     b.part2 = c->part + 2 * c->ntiles;
-    for (int v = 3; v < c->ops->num_shapes && rc == 0; ++v) {
+    for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
+      if (!c->ops->fuse41_variant(c->prec, v, c->g)) {
+        if (ms) ms[5 * c->ops->num_shapes + v] = 0.0;
+        continue;
+      }
       hipError_t e = c->ops->pass(c->prec, 6, v, b, c->tp, c->g, c->stream);
       if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
       for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, 6, v, b, c->tp, c->g, c->stream);
@@ -1162,7 +1210,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
         break;
       }
       t /= reps;
-      if (ms) ms[5 * c->ops->num_shapes + (v - 3)] = t;
+      if (ms) ms[5 * c->ops->num_shapes + v] = t;
       if (pick41 < 0 || t < best41) {
         best41 = t;
         pick41 = v;
@@ -1206,7 +1254,7 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
   c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
   c->fuse41 = (flags & SS_FUSE_41) ? 1 : 0;
   if (c->fuse23) c->tv[2] = fused23_v(c);
-  if (c->fuse41 && c->tv[4] < 3) c->tv[4] = 3;
+  if (c->fuse41) c->tv[4] = fused41_v(c);
   free_graphs(c);
   return 0;
 }

@@ -443,7 +443,8 @@ def test_liquid_crystal_variation_energy_charges_and_steps(deformation, depol):
     ([0, 1, 2, 3], False, True), ([1, 3, 0, 5], True, True), ([2, 4, 4, 4], True, True),
     ([6, 7, 8, 6], False, False), ([0, 6, 7, 8], True, False), ([7, 8, 1, 6], True, True),
     ([3, 6, 0, 7], False, True), ([0, 1, 0, 3], True, False), ([2, 2, 2, 2], True, False),
-    ([1, 2, 1, 6], True, True)])
+    ([1, 2, 1, 6], True, True), ([2, 2, 2, 2], True, True), ([1, 0, 1, 1], False, True),
+    ([0, 1, 0, 2], True, True)])
 def test_tile_shape_variants_match_oracle(variants, fused, fused41):
     """Every launch-shape variant of the fused passes (tiles and streaming strips,
     lattice 70 x 130: partial strips and partial column blocks), with and without
@@ -620,6 +621,55 @@ def test_fused23_tiles_match_separate_passes(name, precision):
         assert ran_tile  # FP64 and FP32 chiral tiles of 4..16 rows hold both fields
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", list(_FUSED23_TILE_CASES))
+def test_fused41_tiles_match_separate_passes(name, precision):
+    """Pass 4 + the next step's pass 1 in one LDS-tile sweep (Temp4 and F' in
+    one tile, F' of the halo sites formed from their own loads) against the
+    separate tile passes, over calls of 0, 1, 2, 7 and 5 steps (chiral: also in
+    graphs of 4 steps): the same energies, errs and fields."""
+    th = ssa.load_theory(name)
+    mkp, mkf = _FUSED23_TILE_CASES[name]
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    tE, tF, tR = (1e-12, 1e-12, 1e-10) if precision == "fp64" else (1e-7, 1e-5, 1e-4)
+    counts = [0, 1, 2, 7, 5]
+    ran_tile = False
+    for v, graphs in ((2, 0), (1, 0), (2, 4 if name == "Chiral magnet" else 0)):
+        a = _sim(th, p, F, precision=precision)
+        b = _sim(th, p, F, precision=precision)
+        a.set_tile_variants([v, v, v, v])
+        b.set_tile_variants([v, v, v, v])
+        try:
+            a.set_fused_stages(False, True)
+        except ValueError:  # no 4+1 sweep for this theory and precision
+            a.close()
+            b.close()
+            continue
+        ran_tile |= a.tile_variants()["variants"][3] == v
+        if graphs:
+            a.use_graphs(graphs)
+            b.use_graphs(graphs)
+        E0 = a.compute_energy()
+        Ea, hEa, hRa = _relax_calls(a, E0, counts)
+        Eb, hEb, hRb = _relax_calls(b, E0, counts)
+        scale = max(1.0, np.abs(F).max())
+        dF = np.abs(a.get_field() - b.get_field()).max() / scale
+        dV = np.abs(a.get_velocity() - b.get_velocity()).max() / scale
+        print(f"[fused41 tiles {name} {precision} v={v} graphs={graphs} -> {a.tile_variants()['variants']}] "
+              f"relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e} max|dV|={dV:.2e}")
+        assert len(hEa) == sum(counts)
+        assert np.all(np.abs(hEa - hEb) <= tE * np.abs(hEb))
+        assert np.all(np.abs(hRa - hRb) <= tR * max(np.abs(hRb).max(), 1e-300))
+        assert H.rel(Ea, Eb) < tE
+        assert dF <= tF and dV <= tF
+        a.close()
+        b.close()
+    if name == "Chiral magnet":
+        assert ran_tile
+
+
 def test_fused41_survives_precision_switch_and_autotune():
     th = ssa.load_theory("Chiral magnet")
     p = H.chiral_params(th, 128, 160)
@@ -627,7 +677,10 @@ def test_fused41_survives_precision_switch_and_autotune():
     F = H.chiral_field(th, p_i, p_f, seed=2)
     a = _sim(th, p, F, precision="fp32")
     info = a.autotune_tiles(reps=2)
-    assert len(info["ms_fused41"]) == 6 and np.all(info["ms_fused41"] > 0)
+    # 9 variants; FP32 chiral: the 32-row two-field tile does not fit LDS, the
+    # 16- and 8-row tiles and every strip do
+    assert len(info["ms_fused41"]) == 9 and info["ms_fused41"][0] == 0 and np.all(info["ms_fused41"][1:] > 0)
+    assert len(info["ms_fused23"]) == 9 and info["ms_fused23"][0] == 0 and np.all(info["ms_fused23"][1:] > 0)
     a.set_tile_variants([0, 3, 3, 3])
     a.set_fused_stages(True, True)
     b = _sim(th, p, F, precision="fp32")
