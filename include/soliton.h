@@ -277,13 +277,20 @@ int ss_batch_memory_bytes(ss_batch* b, size_t* bytes);
  * the single-lattice variants (0-2 LDS tiles of the default rows, halved,
  * quartered; 3-8 streaming strips of 64/128/256 rows, 64 or 128 columns
  * wide), one shape for every instance.  ss_batch_autotune_tiles times the
- * four passes in each variant ``reps`` times on the live buffers and keeps
- * the fastest; ms[0..8] = mean time of the four passes per variant.  The relaxation state is
- * unchanged; results do not depend on the variant beyond FP64 summation
- * order. */
+ * four passes in each variant ``reps`` times on the live buffers, and passes
+ * 1, 2+3 (one sweep, SS_FUSE_23 as for a single lattice), 4 where the variant
+ * has the fused sweep, and keeps the fastest variant and schedule; ms[0..8] =
+ * mean time of the four passes per variant, ms[9..17] = the same with passes
+ * 2+3 fused (0 where the variant has no fused sweep; ms holds 18 entries).
+ * The relaxation state is unchanged; results do not depend on the variant or
+ * schedule beyond FP64 summation order.  ss_batch_set_fused_stages (flags 0
+ * or SS_FUSE_23) fails where the current variant has no fused sweep;
+ * ss_batch_get_fused_stages reports whether steps use it. */
 int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms);
 int ss_batch_set_tile_variant(ss_batch* b, int variant);
 int ss_batch_tile_variant(ss_batch* b, int* variant, int* rows);
+int ss_batch_set_fused_stages(ss_batch* b, int flags);
+int ss_batch_get_fused_stages(ss_batch* b, int* flags);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,8 @@ SIGNATURES = {
     "ss_batch_autotune_tiles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _f64p]),
     "ss_batch_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_batch_tile_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ss_batch_set_fused_stages": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_batch_get_fused_stages": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
 }
 
 _lib = None

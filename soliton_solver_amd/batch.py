@@ -103,22 +103,29 @@ class BatchSimulation:
     def autotune_tiles(self, reps: int = 10) -> dict:
         """Time the four RK4 passes in each shape variant (0-2 LDS tiles of the
         default rows, halved, quartered; 3-8 streaming strips, as for a single
-        lattice) on the live buffers and keep the fastest; the relaxation state is
-        unchanged."""
+        lattice), and with passes 2+3 as one sweep where the variant has it, on
+        the live buffers and keep the fastest; the relaxation state is unchanged."""
         chosen = C.c_int()
-        ms = np.zeros(9)
+        ms = np.zeros(18)
         self._check(_lib.lib().ss_batch_autotune_tiles(self._b, int(reps), C.byref(chosen), ms))
         info = self.tile_variant()
-        info["ms"] = ms
+        info["ms"] = ms[:9]
+        info["ms_fused23"] = ms[9:]
         return info
 
     def set_tile_variant(self, variant: int) -> None:
         self._check(_lib.lib().ss_batch_set_tile_variant(self._b, int(variant)))
 
+    def set_fused_stages(self, fuse23: bool) -> None:
+        """Run passes 2 and 3 of every instance as one sweep (fails where the
+        current shape variant has no such sweep)."""
+        self._check(_lib.lib().ss_batch_set_fused_stages(self._b, _lib.FUSE_23 if fuse23 else 0))
+
     def tile_variant(self) -> dict:
-        v, rows = C.c_int(), C.c_int()
+        v, rows, f = C.c_int(), C.c_int(), C.c_int()
         self._check(_lib.lib().ss_batch_tile_variant(self._b, C.byref(v), C.byref(rows)))
-        return {"variant": v.value, "rows": rows.value}
+        self._check(_lib.lib().ss_batch_get_fused_stages(self._b, C.byref(f)))
+        return {"variant": v.value, "rows": rows.value, "fused23": bool(f.value & _lib.FUSE_23)}
 
     def use_graphs(self, steps_per_graph: int):
         self._check(_lib.lib().ss_batch_use_graphs(self._b, int(steps_per_graph)))

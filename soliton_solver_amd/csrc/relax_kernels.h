@@ -603,10 +603,19 @@ struct BatchBufs {
   int sx;              // streaming shapes: rows per strip
 };
 
+// Output of pass PASS (1..3, or 23: l2 and l3) inside instance buffers at ``off``.
+template <int PASS>
+__device__ __forceinline__ void batch_outputs(PassBufs& b, const BatchBufs& B, long long off, size_t es) {
+  b.lout = PASS == 23                ? (char*)B.l[1] + off * es
+           : PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + off * es
+                                    : nullptr;
+  b.lout2 = PASS == 23 ? (char*)B.l[2] + off * es : nullptr;
+}
+
 template <class TH, typename R, int PASS, int TXR = 0>
 __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
   using S = TileShape<R, TH::NF, TXR>;
-  __shared__ Tile<R, TH::NF, TXR> tile;
+  __shared__ Tile<R, tile_nf<TH, PASS>(), TXR> tile;
   __shared__ double red[kWaves];
   const int t = xcd_tile(blockIdx.x, B.ntiles, B.per_xcd);
   if (t < 0) return;
@@ -624,7 +633,7 @@ __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
   b.l1 = (const char*)B.l[0] + I.off * es;
   b.l2 = (const char*)B.l[1] + I.off * es;
   b.l3 = (const char*)B.l[2] + I.off * es;
-  b.lout = PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + I.off * es : nullptr;
+  batch_outputs<PASS>(b, B, I.off, es);
   b.Fout = (char*)B.F[cur ^ 1] + I.off * es;
   b.Vout = (char*)B.V[cur ^ 1] + I.off * es;
   b.part0 = B.part0;
@@ -658,7 +667,7 @@ __global__ void __launch_bounds__(kThreads) batch_stream_pass(BatchBufs B) {
   b.l1 = (const char*)B.l[0] + I.off * es;
   b.l2 = (const char*)B.l[1] + I.off * es;
   b.l3 = (const char*)B.l[2] + I.off * es;
-  b.lout = PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + I.off * es : nullptr;
+  batch_outputs<PASS>(b, B, I.off, es);
   b.Fout = (char*)B.F[cur ^ 1] + I.off * es;
   b.Vout = (char*)B.V[cur ^ 1] + I.off * es;
   b.part0 = B.part0;
@@ -919,6 +928,18 @@ struct TheoryLaunchers {
     return prec ? fuse41_variant_r<float>(v, g) : fuse41_variant_r<double>(v, g);
   }
 
+  // Passes 2+3 as one sweep in the batched form, shape variant v (instances
+  // address their buffers from their own offsets, so strips need no size check).
+  template <typename R>
+  static int batch_fuse23_r(int v) {
+    constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
+    if (v >= 3) return StreamRing<TH, R, 23>::shape::kFits;
+    if (v == 2) return fused_tile_fits<TH, R, r2>();
+    if (v == 1) return fused_tile_fits<TH, R, r1>();
+    return fused_tile_fits<TH, R, r0>();
+  }
+  static int batch_fuse23(int prec, int v) { return prec ? batch_fuse23_r<float>(v) : batch_fuse23_r<double>(v); }
+
   static int can_comp(int prec, const Geom& g) {
     return prec && StreamRing<TH, float, 1, 1, true>::shape::kFits && effective_shape<float>(3, g) == 3 &&
            !TH::kL2Normalize;
@@ -1074,6 +1095,13 @@ struct TheoryLaunchers {
       case 3: hipLaunchKernelGGL((batch_stage_pass<TH, R, 3, TXR>), grid, block, 0, st, B); break;
       case 4: hipLaunchKernelGGL((batch_stage_pass<TH, R, 4, TXR>), grid, block, 0, st, B); break;
       case 5: hipLaunchKernelGGL((batch_energy_pass<TH, R, TXR>), grid, block, 0, st, B); break;
+      case 6:
+        if constexpr (fused_tile_fits<TH, R, TXR>()) {
+          hipLaunchKernelGGL((batch_stage_pass<TH, R, 23, TXR>), grid, block, 0, st, B);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1095,6 +1123,13 @@ struct TheoryLaunchers {
       case 3: hipLaunchKernelGGL((batch_stream_pass<TH, R, 3, CW>), grid, block, 0, st, B); break;
       case 4: hipLaunchKernelGGL((batch_stream_pass<TH, R, 4, CW>), grid, block, 0, st, B); break;
       case 5: hipLaunchKernelGGL((batch_stream_pass<TH, R, 1, CW, true>), grid, block, 0, st, B); break;
+      case 6:
+        if constexpr (StreamRing<TH, R, 23, CW>::shape::kFits) {
+          hipLaunchKernelGGL((batch_stream_pass<TH, R, 23, CW>), grid, block, 0, st, B);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1114,8 +1149,9 @@ struct TheoryLaunchers {
     *tyw = (strip_cw(v) == 2 && wide) ? 2 * TY : TY;
   }
 
-  // pass 1..4: fused RK4 passes; 5: energy partials of every instance.  ``v``:
-  // shape variant 0..8 (the tile table of the batch must be built for it).
+  // pass 1..4: fused RK4 passes; 5: energy partials of every instance; 6:
+  // passes 2 and 3 in one sweep (where batch_fuse23).  ``v``: shape variant
+  // 0..8 (the tile table of the batch must be built for it).
   static hipError_t batch_pass(int prec, int pass, int v, const BatchBufs& B, hipStream_t st) {
     if (v >= 3) return prec ? batch_stream_v<float>(pass, v, B, st) : batch_stream_v<double>(pass, v, B, st);
     return prec ? batch_pass_r<float>(pass, v, B, st) : batch_pass_r<double>(pass, v, B, st);
@@ -1159,6 +1195,7 @@ struct TheoryOps {
   int (*can_comp)(int, const Geom&);
   int (*fuse23_variant)(int, int, const Geom&);
   int (*fuse41_variant)(int, int, const Geom&);
+  int (*batch_fuse23)(int, int);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -1187,6 +1224,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::can_comp,                   \
                                       &ss::TheoryLaunchers<TH>::fuse23_variant,             \
                                       &ss::TheoryLaunchers<TH>::fuse41_variant,             \
+                                      &ss::TheoryLaunchers<TH>::batch_fuse23,               \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

@@ -1475,6 +1475,7 @@ struct ss_batch {
   int ntiles = 0;
   int ntcap = 0;        // capacity of the tile table and of each partials row (the most tiles over the variants)
   int tv = 0;           // shape variant (0..2 LDS tiles, 3..8 streaming strips, as for a single lattice)
+  int fuse23 = 0;       // passes 2 and 3 in one sweep where the variant has it
   int sx = 0;           // streaming variants: rows per strip
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
@@ -1558,12 +1559,18 @@ static BatchBufs batch_bufs(ss_batch* b) {
   return B;
 }
 
+static bool batch_use23(const ss_batch* b) { return b->fuse23 && b->ops->batch_fuse23(b->prec, b->tv); }
+
 static int batch_launch_step(ss_batch* b, double tol) {
   const BatchBufs B = batch_bufs(b);
   BHIP(b, b->ops->batch_pass(b->prec, 1, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(0, b->ninst, B, 0, 0.0, b->stream));
-  BHIP(b, b->ops->batch_pass(b->prec, 2, b->tv, B, b->stream));
-  BHIP(b, b->ops->batch_pass(b->prec, 3, b->tv, B, b->stream));
+  if (batch_use23(b)) {
+    BHIP(b, b->ops->batch_pass(b->prec, 6, b->tv, B, b->stream));
+  } else {
+    BHIP(b, b->ops->batch_pass(b->prec, 2, b->tv, B, b->stream));
+    BHIP(b, b->ops->batch_pass(b->prec, 3, b->tv, B, b->stream));
+  }
   BHIP(b, b->ops->batch_pass(b->prec, 4, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(1, b->ninst, B, 0, tol, b->stream));
   return 0;
@@ -1825,37 +1832,51 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
   hipEvent_t e0, e1;
   BHIP(b, hipEventCreate(&e0));
   BHIP(b, hipEventCreate(&e1));
-  int best_v = 0;
+  int best_v = 0, best_f = 0;
   float best = 0.f;
   int rc = 0;
-  for (int v = 0; v < b->ops->num_shapes && rc == 0; ++v) {
+  const int nv = b->ops->num_shapes;
+  for (int v = 0; v < nv && rc == 0; ++v) {
     rc = batch_tiles(b, v);
     if (rc) break;
     const BatchBufs B = batch_bufs(b);
-    hipError_t e = hipSuccess;
-    for (int p = 1; p <= 4 && e == hipSuccess; ++p) e = b->ops->batch_pass(b->prec, p, v, B, b->stream);  // warm-up
-    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r)
-      for (int p = 1; p <= 4 && e == hipSuccess; ++p) e = b->ops->batch_pass(b->prec, p, v, B, b->stream);
-    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float t = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-    if (e != hipSuccess) {
-      rc = bfail(b, -2, "batch tile autotuning of variant %d failed: %s", v, hipGetErrorString(e));
-      break;
-    }
-    t /= reps;
-    if (ms) ms[v] = t;
-    if (v == 0 || t < best) {
-      best = t;
-      best_v = v;
+    // f = 0: passes 1, 2, 3, 4; f = 1: passes 1, 2+3, 4 (where the variant has the sweep)
+    for (int f = 0; f < 2 && rc == 0; ++f) {
+      if (f == 1 && !b->ops->batch_fuse23(b->prec, v)) {
+        if (ms) ms[nv + v] = 0.0;
+        continue;
+      }
+      const int seq[2][4] = {{1, 2, 3, 4}, {1, 6, 4, 0}};
+      hipError_t e = hipSuccess;
+      for (int k = 0; k < 4 && seq[f][k] && e == hipSuccess; ++k)  // warm-up
+        e = b->ops->batch_pass(b->prec, seq[f][k], v, B, b->stream);
+      if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r)
+        for (int k = 0; k < 4 && seq[f][k] && e == hipSuccess; ++k)
+          e = b->ops->batch_pass(b->prec, seq[f][k], v, B, b->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        rc = bfail(b, -2, "batch tile autotuning of variant %d%s failed: %s", v, f ? " (fused 2+3)" : "",
+                   hipGetErrorString(e));
+        break;
+      }
+      t /= reps;
+      if (ms) ms[f * nv + v] = t;
+      if ((v == 0 && f == 0) || t < best) {
+        best = t;
+        best_v = v;
+        best_f = f;
+      }
     }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) return rc;
   if (batch_tiles(b, best_v)) return -2;
+  b->fuse23 = best_f;
   if (chosen) *chosen = best_v;
   return 0;
 }
@@ -1865,6 +1886,26 @@ int ss_batch_set_tile_variant(ss_batch* b, int variant) {
   if (variant < 0 || variant >= b->ops->num_shapes)
     return bfail(b, -1, "batch shape variant is %d (valid: 0 .. %d)", variant, b->ops->num_shapes - 1);
   return batch_tiles(b, variant);
+}
+
+int ss_batch_set_fused_stages(ss_batch* b, int flags) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (flags & ~SS_FUSE_23) return bfail(b, -1, "batches fuse passes 2+3 only (flags 0x%x)", flags);
+  if ((flags & SS_FUSE_23) && !b->ops->batch_fuse23(b->prec, b->tv))
+    return bfail(b, -1, "passes 2 and 3 of theory %s cannot share one sweep in shape variant %d at this precision",
+                 b->ops->name, b->tv);
+  b->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return 0;
+}
+
+int ss_batch_get_fused_stages(ss_batch* b, int* flags) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (flags) *flags = batch_use23(b) ? SS_FUSE_23 : 0;
+  return 0;
 }
 
 int ss_batch_tile_variant(ss_batch* b, int* variant, int* rows) {

@@ -158,6 +158,7 @@ def test_batch_autotune_keeps_state_and_trajectory():
     F0 = [a.get_field(i).copy() for i in range(len(params))]
     info = a.autotune_tiles(reps=3)
     assert 0 <= info["variant"] < 9 and len(info["ms"]) == 9 and np.all(info["ms"] > 0)
+    assert len(info["ms_fused23"]) == 9 and np.all(info["ms_fused23"][3:] > 0)  # every strip has the 2+3 sweep
     for i in range(len(params)):
         assert np.array_equal(a.get_field(i), F0[i])
     a.run(4, tol=0.0)
@@ -174,3 +175,39 @@ def test_batch_autotune_keeps_state_and_trajectory():
         assert sa["steps"][i] == sb["steps"][i] == 45
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_fused23_matches_separate_passes(precision):
+    """Passes 2+3 of every instance in one sweep (tiles where the two-field tile
+    fits, strips) against the separate passes: the same energies and fields."""
+    th, params, cfgs = _bs_instances([48, 100, 64, 130], seed=11)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    ran = []
+    for v in (1, 2, 3, 6):
+        a = BatchSimulation(th, params, precision=precision)
+        b = BatchSimulation(th, params, precision=precision)
+        a.initialize(cfgs)
+        b.initialize(cfgs)
+        a.set_tile_variant(v)
+        b.set_tile_variant(v)
+        try:
+            a.set_fused_stages(True)
+        except ValueError:  # no two-field tile of this variant fits
+            a.close()
+            b.close()
+            continue
+        assert a.tile_variant()["fused23"] and not b.tile_variant()["fused23"]
+        ran.append(v)
+        a.run(12, tol=0.0)
+        b.run(12, tol=0.0)
+        sa, sb = a.state(), b.state()
+        for i in range(len(params)):
+            dF = np.abs(a.get_field(i) - b.get_field(i)).max()
+            assert abs(sa["energy"][i] - sb["energy"][i]) <= tol * abs(sb["energy"][i])
+            assert dF <= tol
+            assert sa["steps"][i] == sb["steps"][i] == 12
+        a.close()
+        b.close()
+    print(f"[batch fused23 {precision}] variants run fused: {ran}")
+    assert 3 in ran and 6 in ran
