@@ -121,9 +121,13 @@ template <typename R>
 __device__ __forceinline__ R ld(const char* plane_base, unsigned boff) {
   return *(const R*)(plane_base + boff);
 }
+// Outputs of a streaming pass are read by the next launch only: non-temporal
+// stores keep them from displacing the input rows (halo lines still to be read
+// by neighbouring strips) in L2.  Measured at 4096^2: pass 4 380 -> 351 us,
+// the 2+3 sweep 298 -> 280 us (FP32).
 template <typename R>
 __device__ __forceinline__ void st(void* base, int a, unsigned N, unsigned boff, R v) {
-  *(R*)((char*)base + (size_t)a * N * sizeof(R) + boff) = v;
+  __builtin_nontemporal_store(v, (R*)((char*)base + (size_t)a * N * sizeof(R) + boff));
 }
 
 // Stencils of component a at ring column cy; sl[0..4] are the ring slots of
