@@ -121,6 +121,12 @@ template <typename R>
 __device__ __forceinline__ R ld(const char* plane_base, unsigned boff) {
   return *(const R*)(plane_base + boff);
 }
+// A value read once, at the reading thread's own site (no neighbour reads it
+// from L2 later): a non-temporal load leaves the L2 lines to the halo rows.
+template <typename R>
+__device__ __forceinline__ R ldnt(const char* plane_base, unsigned boff) {
+  return __builtin_nontemporal_load((const R*)(plane_base + boff));
+}
 // Outputs of a streaming pass are read by the next launch only: non-temporal
 // stores keep them from displacing the input rows (halo lines still to be read
 // by neighbouring strips) in L2.  Measured at 4096^2: pass 4 380 -> 351 us,
@@ -408,8 +414,8 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        pl1[k][a] = ld<R>(plane<R>(b.l1, a, NP), boff);
-        pl3[k][a] = ld<R>(plane<R>(b.l3, a, NP), boff);
+        pl1[k][a] = ldnt<R>(plane<R>(b.l1, a, NP), boff);
+        pl3[k][a] = ldnt<R>(plane<R>(b.l3, a, NP), boff);
       }
     }
   }
@@ -478,15 +484,15 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           } else if constexpr (kPre) {  // issued before the stencils
             a1 = pl1[k][a];
             a3 = pl3[k][a];
-          } else {
+          } else {  // FP64: issued at the combine, where a non-temporal load measured slower
             a1 = ld<R>(plane<R>(b.l1, a, NP), boff);
             a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
           }
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
           if constexpr (COMP)
-            Fd[a] = compensated_sum(ld<float>(plane<float>(b.F, a, NP), boff),
-                                    ld<float>(plane<float>(b.Flo, a, NP), boff),
+            Fd[a] = compensated_sum(ldnt<float>(plane<float>(b.F, a, NP), boff),
+                                    ldnt<float>(plane<float>(b.Flo, a, NP), boff),
                                     dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
           else if constexpr (FOFF >= 0)
             Fn[a] = q.v[FOFF + a][sl[2]][cy];
