@@ -167,6 +167,44 @@ __device__ __forceinline__ void ring_derivs(const Ring<R, NFR, SC>& q, int a, co
   }
 }
 
+// ring_derivs in R and, from the same ring values (one set of LDS reads),
+// ring_d1 in FP64: the inputs of the variation and of the energy density of
+// pass 1, each identical to what the two functions give separately.
+template <bool INTERIOR, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_derivs_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy,
+                                               const StencilK<R>& K, const StencilK<double>& Kd, const SiteInfo& s,
+                                               R& f, R d1[2], R d2[3], double& fd, double d1d[2]) {
+  const R c12 = R(SS_C12), c23 = R(SS_C23), c43 = R(SS_C43);
+  const R c = q.v[a][sl[2]][cy];
+  const R xm2 = q.v[a][sl[0]][cy], xm1 = q.v[a][sl[1]][cy];
+  const R xp1 = q.v[a][sl[3]][cy], xp2 = q.v[a][sl[4]][cy];
+  const R ym2 = q.v[a][sl[2]][cy - 2], ym1 = q.v[a][sl[2]][cy - 1];
+  const R yp1 = q.v[a][sl[2]][cy + 1], yp2 = q.v[a][sl[2]][cy + 2];
+  const R sxx = c43 * ((xm1 - c) + (xp1 - c)) - c12 * ((xm2 - c) + (xp2 - c));
+  const R syy = c43 * ((ym1 - c) + (yp1 - c)) - c12 * ((ym2 - c) + (yp2 - c));
+  const R sx1 = (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * K.ilx;
+  const R sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * K.ily;
+  f = c;
+  d1[0] = (INTERIOR || s.inx) ? sx1 : R(0);
+  d1[1] = (INTERIOR || s.iny) ? sy1 : R(0);
+  d2[0] = (INTERIOR || s.inx) ? sxx * K.ilx2 : R(0);
+  d2[1] = (INTERIOR || s.iny) ? syy * K.ily2 : R(0);
+  if (INTERIOR || s.inxy) {
+    const R dm2 = q.v[a][sl[0]][cy - 2], dm1 = q.v[a][sl[1]][cy - 1];
+    const R dp1 = q.v[a][sl[3]][cy + 1], dp2 = q.v[a][sl[4]][cy + 2];
+    const R sdd = c43 * ((dm1 - c) + (dp1 - c)) - c12 * ((dm2 - c) + (dp2 - c));
+    d2[2] = sdd * K.i2gv - sxx * K.i2lx2 - syy * K.i2ly2;
+  } else {
+    d2[2] = R(0);
+  }
+  const double e12 = SS_C12, e23 = SS_C23;
+  fd = (double)c;
+  const double tx = (e12 * ((double)xm2 - (double)xp2) + e23 * ((double)xp1 - (double)xm1)) * Kd.ilx;
+  const double ty = (e12 * ((double)ym2 - (double)yp2) + e23 * ((double)yp1 - (double)ym1)) * Kd.ily;
+  d1d[0] = (INTERIOR || s.inx) ? tx : 0.0;
+  d1d[1] = (INTERIOR || s.iny) ? ty : 0.0;
+}
+
 template <bool INTERIOR, typename T, typename R, int NFR, int SC>
 __device__ __forceinline__ void ring_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                         const SiteInfo& s, T& f, T d1[2]) {
@@ -437,8 +475,14 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
         s = site_info(g, x, y);
       }
       R f[NF], d1[NF][2], d2[NF][3], eg[NF];
+      double fd[NF], d1d[NF][2];  // pass 1: FP64 inputs of the energy density, from the same ring reads
 #pragma unroll
-      for (int a = 0; a < NF; ++a) ring_derivs<INTERIOR, R, R, NFR>(q, AOFF + a, sl, cy, K, s, f[a], d1[a], d2[a]);
+      for (int a = 0; a < NF; ++a) {
+        if constexpr (PASS == 1)
+          ring_derivs_d1<INTERIOR, R, NFR>(q, AOFF + a, sl, cy, K, Kd, s, f[a], d1[a], d2[a], fd[a], d1d[a]);
+        else
+          ring_derivs<INTERIOR, R, R, NFR>(q, AOFF + a, sl, cy, K, s, f[a], d1[a], d2[a]);
+      }
       TH::template variation<R>(tp, g, s, f, d1, d2, eg);
       const unsigned boff = (unsigned)(x * g.Y + y) * (unsigned)sizeof(R);
       if (PASS == 0) {
@@ -449,9 +493,6 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
         for (int a = 0; a < NF; ++a) st<R>(b.lout, a, NP, boff, -dt * eg[a]);
       }
       if (PASS == 1) {
-        double fd[NF], d1d[NF][2];
-#pragma unroll
-        for (int a = 0; a < NF; ++a) ring_d1<INTERIOR, double, R, NFR>(q, AOFF + a, sl, cy, Kd, s, fd[a], d1d[a]);
         if constexpr (COMP) {
           // compensated form: the energy of hi + lo (low words in ring
           // components NF..2NF-1); the stencils are linear, so
