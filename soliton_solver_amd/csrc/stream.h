@@ -205,6 +205,23 @@ __device__ __forceinline__ void ring_derivs_d1(const Ring<R, NFR, SC>& q, int a,
   d1d[1] = (INTERIOR || s.iny) ? ty : 0.0;
 }
 
+// FP64 value and first derivatives of hi + lo, hi in ring component ah and lo
+// in al (the compensated form's field), from the FP64 sums of the ring values.
+template <bool INTERIOR, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_d1_hilo(const Ring<R, NFR, SC>& q, int ah, int al, const int sl[5], int cy,
+                                             const StencilK<double>& Kd, const SiteInfo& s, double& f,
+                                             double d1[2]) {
+  auto v = [&](int r, int c) { return (double)q.v[ah][r][c] + (double)q.v[al][r][c]; };
+  const double c12 = SS_C12, c23 = SS_C23;
+  const double xm2 = v(sl[0], cy), xm1 = v(sl[1], cy), xp1 = v(sl[3], cy), xp2 = v(sl[4], cy);
+  const double ym2 = v(sl[2], cy - 2), ym1 = v(sl[2], cy - 1), yp1 = v(sl[2], cy + 1), yp2 = v(sl[2], cy + 2);
+  f = v(sl[2], cy);
+  const double sx1 = (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * Kd.ilx;
+  const double sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * Kd.ily;
+  d1[0] = (INTERIOR || s.inx) ? sx1 : 0.0;
+  d1[1] = (INTERIOR || s.iny) ? sy1 : 0.0;
+}
+
 template <bool INTERIOR, typename T, typename R, int NFR, int SC>
 __device__ __forceinline__ void ring_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
                                         const SiteInfo& s, T& f, T d1[2]) {
@@ -478,7 +495,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       double fd[NF], d1d[NF][2];  // pass 1: FP64 inputs of the energy density, from the same ring reads
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        if constexpr (PASS == 1)
+        if constexpr (PASS == 1 && !COMP)
           ring_derivs_d1<INTERIOR, R, NFR>(q, AOFF + a, sl, cy, K, Kd, s, f[a], d1[a], d2[a], fd[a], d1d[a]);
         else
           ring_derivs<INTERIOR, R, R, NFR>(q, AOFF + a, sl, cy, K, s, f[a], d1[a], d2[a]);
@@ -495,16 +512,10 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       if (PASS == 1) {
         if constexpr (COMP) {
           // compensated form: the energy of hi + lo (low words in ring
-          // components NF..2NF-1); the stencils are linear, so
-          // D1(hi + lo) = D1(hi) + D1(lo) in FP64
+          // components NF..2NF-1), read after the variation so that the FP64
+          // values are not held across it
 #pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            double fl, d1l[2];
-            ring_d1<INTERIOR, double, R, NFR>(q, NF + a, sl, cy, Kd, s, fl, d1l);
-            fd[a] += fl;
-            d1d[a][0] += d1l[0];
-            d1d[a][1] += d1l[1];
-          }
+          for (int a = 0; a < NF; ++a) ring_d1_hilo<INTERIOR, R, NFR>(q, AOFF + a, NF + a, sl, cy, Kd, s, fd[a], d1d[a]);
         }
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
