@@ -442,6 +442,15 @@ static int energy_partials(ss_ctx* c, const void* F, int* nparts) {
   return 0;
 }
 
+// The variants the passes run in: a fused sweep or the compensated pass 1
+// falls back to the first strip variant where the chosen one lacks it; the
+// stored variants are set to what runs, so ss_get_tile_variants reports it.
+static void settle_variants(ss_ctx* c) {
+  if (c->fuse23) c->tv[2] = fused23_v(c);
+  if (c->fuse41) c->tv[4] = fused41_v(c);
+  c->tv[1] = pass1_variant(c);
+}
+
 static int alloc_low_words(ss_ctx* c) {
   const size_t bytes = (size_t)c->g.nf * (size_t)c->g.N * sizeof(float);
   if (!c->Flo) HIPCHK(c, hipMalloc(&c->Flo, bytes));
@@ -795,6 +804,7 @@ int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f,
     hipfail(c, "hipStreamCreate failed");
     return bail(-2);
   }
+  settle_variants(c);
   int rc = alloc_fields(c);
   if (rc == 0) rc = alloc_partials(c);
   if (rc) return bail(rc);
@@ -893,7 +903,7 @@ int ss_set_precision(ss_ctx* c, int prec) {
       c->prec = prec;
     }
     if (c->fuse41 && !can41(c)) c->fuse41 = 0;
-    if (c->fuse41) c->tv[4] = fused41_v(c);
+    settle_variants(c);
     return 0;
   }
   const size_t nb = (size_t)n * elem_size(prec);
@@ -925,9 +935,8 @@ int ss_set_precision(ss_ctx* c, int prec) {
   free_fields(c);
   c->prec = prec;
   if (c->fuse23 && !c->ops->fuse23_variant(prec, c->tv[2], c->g) && !c->ops->can_fuse23(prec, c->g)) c->fuse23 = 0;
-  if (c->fuse23) c->tv[2] = fused23_v(c);
   if (c->fuse41 && !can41(c)) c->fuse41 = 0;
-  if (c->fuse41) c->tv[4] = fused41_v(c);
+  settle_variants(c);
   if (alloc_fields(c)) return -1;
   HIPCHK(c, hipMemcpyAsync(c->F[0], nF, nb, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->V[0], nV, nb, hipMemcpyDeviceToDevice, c->stream));
@@ -1073,6 +1082,7 @@ int ss_set_tile_variants(ss_ctx* c, const int* variants) {
     if (variants[p] < 0 || variants[p] >= c->ops->num_shapes)
       return fail(c, "shape variant of pass %d is %d (valid: 0 .. %d)", p + 1, variants[p], c->ops->num_shapes - 1);
   for (int p = 0; p < 4; ++p) c->tv[p + 1] = variants[p];
+  settle_variants(c);
   free_graphs(c);  // captured graphs hold the old launch shapes
   return 0;
 }
@@ -1239,6 +1249,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     c->fuse41 = 1;
     c->tv[4] = pick41;
   }
+  settle_variants(c);
   free_graphs(c);
   return 0;
 }
@@ -1253,8 +1264,7 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
   if ((flags & SS_FUSE_41) && ensure_l1alt(c)) return -1;
   c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
   c->fuse41 = (flags & SS_FUSE_41) ? 1 : 0;
-  if (c->fuse23) c->tv[2] = fused23_v(c);
-  if (c->fuse41) c->tv[4] = fused41_v(c);
+  settle_variants(c);
   free_graphs(c);
   return 0;
 }

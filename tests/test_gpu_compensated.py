@@ -167,3 +167,22 @@ def test_fp32c_refused_where_unavailable():
     with pytest.raises(ValueError):
         ssa.Simulation(p, th, precision="fp32c", variation="exact_adjoint")
     assert _lib.PREC_FP32C == 2
+
+
+def test_reported_variants_are_the_ones_that_run():
+    """Where a shape variant lacks a sweep (the compensated pass 1 streams; an
+    FP64 chiral tile of 16 rows cannot hold two fields), the pass runs in the
+    first strip variant and the context reports that variant."""
+    th, p, F = _chiral(96, 80)
+    c = _sim(th, p, F, precision="fp32c")
+    assert c.tile_variants()["variants"][0] >= 3
+    c.set_tile_variants([0, 0, 0, 0])
+    assert c.tile_variants()["variants"] == [3, 0, 0, 0]
+    c.close()
+    a = _sim(th, p, F)
+    a.set_fused_stages(True, True)
+    a.set_tile_variants([0, 0, 0, 0])
+    assert a.tile_variants()["variants"] == [0, 3, 0, 3]
+    a.set_tile_variants([2, 2, 2, 2])  # 4-row tiles hold both fields
+    assert a.tile_variants()["variants"] == [2, 2, 2, 2]
+    a.close()
