@@ -1009,6 +1009,9 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
   if (!c) return fail(nullptr, "null context");
   if (nsteps < 0) return fail(c, "nsteps must be >= 0");
   if (ensure_hist(c, nsteps + 1)) return -1;
+  // the exact-adjoint and L2-normalised flows allocate their scratch on first
+  // use: do it here, never inside a graph capture
+  if ((c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) && nsteps > 0 && ensure_exact(c)) return -1;
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, c->stream, c->st, prev_energy);
   HIPCHK(c, hipGetLastError());
   int done = 0;
