@@ -242,6 +242,8 @@ def converge_workload(args):
     ]
     res = {}
     for key, name, kw, cfg, tol, max_steps in cases:
+        if args.converge_cap:  # a bounded run (profiling): at most this many steps per case
+            max_steps = min(max_steps, args.converge_cap)
         th = ssa.load_theory(name)
         sim = ssa.Simulation(th.params.default_params(**kw), th, precision="fp64")
         sim.initialize(cfg)
@@ -484,6 +486,8 @@ def main():
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
+    ap.add_argument("--converge-cap", type=int, default=0,
+                    help="converge workload: stop each case after at most this many steps (0: run to the tolerance)")
     ap.add_argument("--form", choices=["fp32c", "fp32"], default="fp32c",
                     help="precision workload: FP32 phase with the compensated field update (fp32c) or plain (fp32)")
     ap.add_argument("--seed", type=int, default=4)
