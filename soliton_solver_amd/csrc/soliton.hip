@@ -94,20 +94,25 @@ __global__ void k_begin_call(DevState* st, double prev_energy) {
   }
 }
 
+// Threads of the one-block finalize kernels of a step: four waves (a 1024-
+// thread block measured 1 us slower per step on a 320^2 lattice, where these
+// launches are on the critical path).
+constexpr int kFinThreads = 256;
+
 // Sum of per-tile energy partials (fixed order: Kahan per thread, then a
 // shuffle tree), then the arrest decision of the previous step
 // (core/integrator.py:338-343): kill iff killkinen and E_new > E_prev.
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kFinThreads)
 k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, int killkinen) {
   __shared__ double red[16];
   double s = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
+  for (int i = threadIdx.x; i < n; i += kFinThreads) {
     const double y = part[i] - c;
     const double t = s + y;
     c = (t - s) - y;
     s = t;
   }
-  const double E = block_sum<1024>(s, red);
+  const double E = block_sum<kFinThreads>(s, red);
   if (threadIdx.x == 0) {
     st->last_energy = E;
     const int step = st->step;
@@ -121,16 +126,16 @@ k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, in
   }
 }
 
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kFinThreads)
 k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histErr, int cap) {
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
+  for (int i = threadIdx.x; i < n; i += kFinThreads) {
     m0 = fmax(m0, p0[i]);
     m1 = fmax(m1, p1[i]);
   }
-  const double a = block_max<1024>(m0, red);
-  const double b = block_max<1024>(m1, red);
+  const double a = block_max<kFinThreads>(m0, red);
+  const double b = block_max<kFinThreads>(m1, red);
   if (threadIdx.x == 0) {
     st->err_max = a;
     st->err_absmax = b;
@@ -143,12 +148,12 @@ k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histE
 // The two finalizes of the 4+1 schedule in one launch: err of step n (as
 // k_fin_err), then the energy of the new field and the arrest decision (as
 // k_fin_energy with the step counter already advanced).
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kFinThreads)
 k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, DevState* st, double* histErr,
                  double* histE, int cap, int killkinen) {
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
+  for (int i = threadIdx.x; i < n; i += kFinThreads) {
     m0 = fmax(m0, p0[i]);
     m1 = fmax(m1, p1[i]);
     const double y = p2[i] - c;
@@ -156,9 +161,9 @@ k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, De
     c = (t - s) - y;
     s = t;
   }
-  const double a = block_max<1024>(m0, red);
-  const double b = block_max<1024>(m1, red);
-  const double E = block_sum<1024>(s, red);
+  const double a = block_max<kFinThreads>(m0, red);
+  const double b = block_max<kFinThreads>(m1, red);
+  const double E = block_sum<kFinThreads>(s, red);
   if (threadIdx.x == 0) {
     st->err_max = a;
     st->err_absmax = b;
@@ -550,7 +555,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     const int v1 = pass1_variant(c);
     HIPCHK(c, c->ops->pass(c->prec, pass1_id(c), v1, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0,
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
                        c->ops->ntiles(c->prec, v1, c->g), c->st, histE, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
@@ -582,7 +587,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     HIPCHK(c, c->ops->pass(c->prec, 6, v4, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     const int nt = c->ops->ntiles(c->prec, v4, c->g);
-    hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
+    hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
                        (const double*)p2, nt, c->st, histErr, histE, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
@@ -591,7 +596,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   b.lout = nullptr;
   HIPCHK(c, c->ops->pass(c->prec, c->prec == SS_PREC_FP32C ? 7 : 4, c->tv[4], b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)p0, (const double*)p1,
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
                      c->ops->ntiles(c->prec, c->tv[4], c->g), c->st, histErr, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
@@ -626,7 +631,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   if (ensure_exact(c)) return -1;
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, c->nt0, c->st,
                      c->hist, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
@@ -646,7 +651,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   b.part1 = c->xpart + c->xpart_n;
   int nblk = 0;
   HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->tp, c->g, c->stream, &nblk));
-  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
                      nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -671,7 +676,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   if (ensure_exact(c)) return -1;
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, c->nt0, c->st,
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, c->nt0, c->st,
                      c->hist, c->hist_cap, c->g.killkinen);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
@@ -701,7 +706,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   b.part1 = c->xpart + c->xpart_n;
   int nblk = 0;
   HIPCHK(c, c->ops->combine(c->prec, b, c->G, c->tp, c->g, c->stream, &nblk));
-  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(1024), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)b.part0, (const double*)b.part1,
                      nblk, c->st, c->hist + c->hist_cap, c->hist_cap);
   HIPCHK(c, hipGetLastError());
   return l2_normalize_buf(c, (double*)b.Fout);
@@ -714,7 +719,7 @@ static int launch_finish(ss_ctx* c, int nsteps) {
   if (!(use41(c) && nsteps > 0)) {
     int np = 0;
     if (energy_partials(c, c->F[c->cur], &np)) return -1;
-    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part, np, c->st,
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, np, c->st,
                        c->hist, c->hist_cap, c->g.killkinen);
     HIPCHK(c, hipGetLastError());
   }
