@@ -298,6 +298,74 @@ class Simulation:
                 break
         return E, ea, done
 
+    def minimize_fast(self, tol: float = 1e-6, max_steps: int = 200000, chunk: int = 10000, polish: int = 200,
+                      form: str = "fp32c", check_every: int = 20, autotune: bool = False):
+        """Relax to max |EG| < tol with the FP32 form doing the bulk of the steps.
+
+        The FP32 form (``form``: "fp32c", the compensated one, or "fp32") runs in
+        chunks of at most ``chunk`` steps; after each chunk the state is
+        converted to FP64 in place and polished for at most ``polish`` steps
+        (max |EG| of an FP32 field has a floor of ~1e-4 from the field's own
+        rounding, so convergence is judged on the FP64 field).  The next chunk is
+        sized from the decay of the polished max |EG| over the last two checks;
+        if a chunk improves it by less than 30 % the FP32 form has stalled and
+        the relaxation finishes in FP64.  The launch shapes (tile variants and
+        fused sweeps) are kept per precision: as set before the call for the
+        precision the context is in, the same for the other one, or, with
+        ``autotune``, tuned once for each.  The context ends in FP64 on the
+        relaxed state.
+
+        Returns ``(energy, err_absmax, fp32_steps, fp64_steps)``.
+        """
+        if form not in ("fp32c", "fp32"):
+            raise ValueError('form must be "fp32c" or "fp32"')
+
+        def shapes():
+            if autotune:
+                self.autotune_tiles(reps=3)
+            return self.tile_variants()
+
+        def switch(prec, info):
+            self.set_precision(prec)
+            self.set_tile_variants(info["variants"])
+            self.set_fused_stages(info["fused23"], info["fused41"] and prec == "fp64")
+
+        first = self.precision
+        self.set_precision("fp64")
+        tiles64 = shapes()
+        self.set_precision(form)
+        tiles32 = shapes() if (autotune or first == form) else self.tile_variants()
+        if first != "fp64" and not autotune:  # shapes set for the FP32 form: keep them for FP64 too
+            tiles64 = tiles32
+        E = self.compute_energy()
+        n32 = n64 = 0
+        checks = []
+        size = chunk
+        ea = float("inf")
+        while True:
+            n = min(size, max_steps - n32 - n64)
+            if n > 0:
+                E, _, _ = self.steps(E, n)
+                n32 += n
+            switch("fp64", tiles64)
+            E = self.compute_energy()
+            E, ea, npol = self.minimize(tol, polish, check_every, prev_energy=E)
+            n64 += npol
+            if ea < tol or n32 + n64 >= max_steps:
+                return E, ea, n32, n64
+            checks.append((n32 + n64, ea))
+            if len(checks) >= 2 and checks[-1][1] > 0.7 * checks[-2][1]:
+                E, ea, ntail = self.minimize(tol, max(0, max_steps - n32 - n64), max(check_every, 100),
+                                             prev_energy=E)
+                return E, ea, n32, n64 + ntail
+            if len(checks) >= 2 and checks[-1][1] < checks[-2][1]:
+                (s0, e0), (s1, e1) = checks[-2], checks[-1]
+                need = np.log(tol / e1) / (np.log(e1 / e0) / (s1 - s0))
+                size = int(min(max(1.05 * need, chunk / 10), chunk))
+            else:
+                size = chunk
+            switch(form, tiles32)
+
     # -- observables --------------------------------------------------------
     def compute_energy(self) -> float:
         out = C.c_double()

@@ -186,3 +186,24 @@ def test_reported_variants_are_the_ones_that_run():
     a.set_tile_variants([2, 2, 2, 2])  # 4-row tiles hold both fields
     assert a.tile_variants()["variants"] == [2, 2, 2, 2]
     a.close()
+
+
+def test_minimize_fast_reaches_the_fp64_minimum():
+    """Simulation.minimize_fast (compensated FP32 chunks, FP64 polish checks)
+    against the plain FP64 relaxation from the same field, both to max|EG|
+    < 1e-6: energies within 1e-7 relative (the precision form's bar), the
+    same skyrmion number, and the context ends in FP64."""
+    th, p, F = _chiral(96, 80, seed=6, demag=False)
+    a = _sim(th, p, F)
+    Ea, eaa, na = a.minimize(tol=1e-6, max_steps=300000, check_every=500)
+    b = _sim(th, p, F)
+    Eb, eab, n32, n64 = b.minimize_fast(tol=1e-6, max_steps=300000, chunk=4000)
+    Qa, Qb = a.observables()["skyrmion_number"], b.observables()["skyrmion_number"]
+    print(f"[minimize_fast] fp64: {na} steps E={Ea:.12f} err={eaa:.2e}; fast: {n32} fp32c + {n64} fp64 steps "
+          f"E={Eb:.12f} err={eab:.2e} relE={H.rel(Eb, Ea):.2e} Q {Qa:.6f} {Qb:.6f}")
+    assert eaa < 1e-6 and eab < 1e-6
+    assert b.precision == "fp64"
+    assert H.rel(Eb, Ea) < 1e-7
+    assert round(Qa) == round(Qb)
+    a.close()
+    b.close()
