@@ -105,7 +105,7 @@ def relax_timed(sim, E, tol, max_steps, check_every, floor=False):
     return E, ea, done, time.perf_counter() - t0
 
 
-def precision_workload(args):
+def precision_workload(args, emit=True):
     """Time to max |EG| < tol: the plain FP64 form against the FP32 form with
     FP64 polish, from the same seeded field.  ``--form fp32c`` (the default)
     runs the FP32 phase with the compensated field update (FP32 hi + lo words,
@@ -219,7 +219,9 @@ def precision_workload(args):
         "max_dm_vs_fp64_form": dm,
         "target_relE": 1e-7,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def converge_workload(args):
@@ -486,6 +488,8 @@ def main():
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="throughput workload: skip the 1024^2 time-to-converge check of the FP32C form")
     ap.add_argument("--converge-cap", type=int, default=0,
                     help="converge workload: stop each case after at most this many steps (0: run to the tolerance)")
     ap.add_argument("--form", choices=["fp32c", "fp32"], default="fp32c",
@@ -648,6 +652,24 @@ def main():
         cpu["numpy_oracle_single_thread"] = numpy_oracle_rates(gl_steps, cm_steps)
         print(f"[bench] numpy oracle: {cpu['numpy_oracle_single_thread']}", file=sys.stderr, flush=True)
 
+    # time to converge of the compensated FP32 form against the FP64 form, to
+    # max|EG| < 1e-6 from one seeded field: the precision workload at 1024^2
+    # (the 4096^2 run, ~2.5 min, is `--workload precision --n 4096`)
+    ttc = None
+    if not args.no_precision and not args.profile_only:
+        import copy
+        pa = copy.copy(args)
+        pa.n, pa.nsk, pa.demag, pa.tol, pa.form, pa.seed = 1024, 1, 0, 1e-6, "fp32c", 4
+        pr = precision_workload(pa, emit=False)
+        ttc = {"workload": pr["config"]["workload"], "speedup_vs_fp64": pr["value"], "converged": pr["converged"],
+               "fp64_seconds": pr["fp64_form"]["seconds"], "fp64_steps": pr["fp64_form"]["steps"],
+               "fp32c_seconds": pr["fp32_phase"]["seconds"] + pr["fp64_polish"]["seconds"],
+               "fp32c_steps": pr["fp32_phase"]["steps"], "fp64_polish_steps": pr["fp64_polish"]["steps_total"],
+               "relE_vs_fp64_form": pr["relE_vs_fp64_form"],
+               "skyrmion_number_identical": pr["skyrmion_number"]["identical_rounded"]}
+        print(f"[bench] time to converge (1024^2, fp32c vs fp64): {ttc['speedup_vs_fp64']:.2f}x, "
+              f"relE {ttc['relE_vs_fp64_form']:.1e}", file=sys.stderr, flush=True)
+
     v4 = tiles32["variants"][3]
     if f41 and v4 < 3:
         v4 = 3
@@ -701,6 +723,7 @@ def main():
         "fp64_form": result64,
         "speedup_fp32_vs_fp64": (ups32 / result64["updates_per_s"]) if result64 else None,
         "fp32c_form": resultc,
+        "time_to_converge": ttc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
