@@ -167,9 +167,9 @@ __device__ __forceinline__ void ring_derivs(const Ring<R, NFR, SC>& q, int a, co
   }
 }
 
-// ring_derivs in R and, from the same ring values (one set of LDS reads),
-// ring_d1 in FP64: the inputs of the variation and of the energy density of
-// pass 1, each identical to what the two functions give separately.
+// ring_derivs in R and, from the same ring values (one set of LDS reads), the
+// value and first derivatives in FP64: the inputs of the variation and of the
+// energy density of pass 1 (the FP32 form converts the FP32 ring values).
 template <bool INTERIOR, typename R, int NFR, int SC>
 __device__ __forceinline__ void ring_derivs_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy,
                                                const StencilK<R>& K, const StencilK<double>& Kd, const SiteInfo& s,
@@ -220,21 +220,6 @@ __device__ __forceinline__ void ring_d1_hilo(const Ring<R, NFR, SC>& q, int ah, 
   const double sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * Kd.ily;
   d1[0] = (INTERIOR || s.inx) ? sx1 : 0.0;
   d1[1] = (INTERIOR || s.iny) ? sy1 : 0.0;
-}
-
-template <bool INTERIOR, typename T, typename R, int NFR, int SC>
-__device__ __forceinline__ void ring_d1(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy, const StencilK<T>& K,
-                                        const SiteInfo& s, T& f, T d1[2]) {
-  const T c12 = T(SS_C12), c23 = T(SS_C23);
-  const T xm2 = T(q.v[a][sl[0]][cy]), xm1 = T(q.v[a][sl[1]][cy]);
-  const T xp1 = T(q.v[a][sl[3]][cy]), xp2 = T(q.v[a][sl[4]][cy]);
-  const T ym2 = T(q.v[a][sl[2]][cy - 2]), ym1 = T(q.v[a][sl[2]][cy - 1]);
-  const T yp1 = T(q.v[a][sl[2]][cy + 1]), yp2 = T(q.v[a][sl[2]][cy + 2]);
-  f = T(q.v[a][sl[2]][cy]);
-  const T sx1 = (c12 * (xm2 - xp2) + c23 * (xp1 - xm1)) * K.ilx;
-  const T sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * K.ily;
-  d1[0] = (INTERIOR || s.inx) ? sx1 : T(0);
-  d1[1] = (INTERIOR || s.iny) ? sy1 : T(0);
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
