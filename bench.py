@@ -145,13 +145,49 @@ def precision_workload(args, emit=True):
     s32.initialize({"mode": "array", "field": F0})
     s32.autotune_tiles(reps=3)
     tiles32 = s32.tile_variants()
-    E = s32.compute_energy()
-    n32 = npol_total = nchecks = 0
-    t32 = tpol = 0.0
+    # FP64 lead (--start fp64): run the FP64 form while it converges faster
+    # than the FP32 form would, and hand over once a chunk shrinks max|EG| by
+    # less than --lead-rate (the FP32 form's late-phase rate, ~2.7x per 10 000
+    # of its steps at ~0.7x the FP64 step cost, is ~1.33x per 2000 FP64 steps'
+    # time).  The arrest kills per chunk are reported alongside.
+    nA, tA, killsA, lead_converged = 0, 0.0, [], False
+    rates = []
     eap = np.inf
+    if args.start == "fp64":
+        s32.set_precision("fp64")
+        s32.use_graphs(args.graph_steps)
+        E = s32.compute_energy()
+        while nA < args.max_steps:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            Eprev = E
+            E, _, eap = s32.steps(E, args.lead_chunk, history=True)
+            s32.synchronize()
+            tA += time.perf_counter() - t0
+            nA += args.lead_chunk
+            kills = int(np.sum(np.diff(np.concatenate([[Eprev], s32.last_history[0]])) > 0))
+            killsA.append(kills)
+            if eap < tol:
+                lead_converged = True
+                break
+            if nA > args.lead_chunk:
+                rates.append(ea_prev / eap)
+                if rates[-1] < args.lead_rate:
+                    break
+            ea_prev = eap
+        print(f"[bench] fp64 lead: {nA} steps, {tA:.2f} s, max|EG|={eap:.2e}, kills per {args.lead_chunk} steps "
+              f"{killsA}, max|EG| decay per chunk {[round(r, 2) for r in rates]}", file=sys.stderr, flush=True)
+        if not lead_converged:
+            s32.set_precision(form)
+            s32.set_tile_variants(tiles32["variants"])
+            s32.set_fused_stages(tiles32["fused23"], tiles32["fused41"])
+    Ep = E
+    E = s32.compute_energy()
+    n32 = npol_total = nchecks = npol = 0
+    t32 = tpol = 0.0
     chunk, checks = args.chunk, []
     fp64_tail = 0
-    while True:
+    while not lead_converged:
         s32.use_graphs(args.graph_steps)
         E, _, ea32, dn, dt_ = _chunk(s32, E, min(chunk, args.max_steps - n32))
         n32 += dn
@@ -196,7 +232,7 @@ def precision_workload(args, emit=True):
     s32.close()
     dE = abs(Ep - E64) / abs(E64)
     dm = float(np.abs(Fp[:3] - F64[:3]).max())
-    t_mixed = t32 + tpol
+    t_mixed = tA + t32 + tpol
     out = {
         "metric": f"time-to-converge, {form} form + FP64 polish vs FP64 form",
         "value": t64 / t_mixed,
@@ -211,6 +247,9 @@ def precision_workload(args, emit=True):
                    "lattice": [n, n], "fp32_chunk_steps": args.chunk, "max_polish_steps": args.polish},
         "converged": bool(ea64 < tol and eap < tol),
         "fp64_form": {"steps": n64, "seconds": t64, "energy": E64, "err_absmax": ea64},
+        "fp64_lead": {"steps": nA, "seconds": tA, "kills_per_chunk": killsA, "decay_per_chunk": rates,
+                      "chunk": args.lead_chunk, "hand_over_rate": args.lead_rate,
+                      "converged": lead_converged} if args.start == "fp64" else None,
         "fp32_phase": {"form": form, "steps": n32, "seconds": t32, "polish_checks": nchecks, "tiles": tiles32},
         "fp64_polish": {"steps_total": npol_total, "steps_final": npol, "tail_steps": fp64_tail, "seconds": tpol,
                         "energy": Ep, "err_absmax": eap},
@@ -492,6 +531,13 @@ def main():
                     help="throughput workload: skip the 1024^2 time-to-converge check of the FP32C form")
     ap.add_argument("--converge-cap", type=int, default=0,
                     help="converge workload: stop each case after at most this many steps (0: run to the tolerance)")
+    ap.add_argument("--start", choices=["fp64", "reduced"], default="reduced",
+                    help="precision workload: begin in FP64 and hand over to the FP32 form once FP64 converges "
+                         "slower than it would (fp64), or begin in the FP32 form (reduced)")
+    ap.add_argument("--lead-chunk", type=int, default=2000, help="precision workload: FP64 lead steps per check")
+    ap.add_argument("--lead-rate", type=float, default=1.33,
+                    help="precision workload: hand over to the FP32 form once an FP64 lead chunk shrinks max|EG| "
+                         "by less than this factor")
     ap.add_argument("--form", choices=["fp32c", "fp32"], default="fp32c",
                     help="precision workload: FP32 phase with the compensated field update (fp32c) or plain (fp32)")
     ap.add_argument("--seed", type=int, default=4)
