@@ -181,8 +181,8 @@ def precision_workload(args, emit=True):
             s32.set_precision(form)
             s32.set_tile_variants(tiles32["variants"])
             s32.set_fused_stages(tiles32["fused23"], tiles32["fused41"])
-    Ep = E
     E = s32.compute_energy()
+    Ep = E
     n32 = npol_total = nchecks = npol = 0
     t32 = tpol = 0.0
     chunk, checks = args.chunk, []
