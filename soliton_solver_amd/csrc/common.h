@@ -81,6 +81,7 @@ struct DevState {
   int kill;             // velocity of the current step is treated as zero
   int step;             // steps done in the current call
   int pad[2];
+  double eslot[2];      // energy of the last field finalized in buffer parity p (read by folded finalizes)
 };
 
 // ---------------------------------------------------------------------------

@@ -355,7 +355,11 @@ stage_pass(PassBufs b, typename TH::TP tp, Geom g, Launch L) {
   const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
   const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
-  const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
+  bool kill;
+  if constexpr (PASS == 23)  // the previous step's finalize folded in, or its decision
+    kill = b.fin_n > 0 ? fold_finalize<kThreads>(b, t == 0, red) : (b.st->kill != 0);
+  else
+    kill = (PASS >= 2) ? (b.st->kill != 0) : false;
   stage_tile<TH, R, PASS, TXR, COMP>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
 }
 

@@ -89,6 +89,7 @@ static const TheoryOps* theory_ops(int id) {
 __global__ void k_begin_call(DevState* st, double prev_energy) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     st->prev_energy = prev_energy;
+    st->eslot[0] = st->eslot[1] = prev_energy;  // the first kill test compares with the caller's energy
     st->kill = 0;
     st->step = 0;
   }
@@ -103,7 +104,7 @@ constexpr int kFinThreads = 256;
 // shuffle tree), then the arrest decision of the previous step
 // (core/integrator.py:338-343): kill iff killkinen and E_new > E_prev.
 __global__ void __launch_bounds__(kFinThreads)
-k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, int killkinen) {
+k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, int killkinen, int parity) {
   __shared__ double red[16];
   double s = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < n; i += kFinThreads) {
@@ -119,6 +120,7 @@ k_fin_energy(const double* part, int n, DevState* st, double* histE, int cap, in
     if (step > 0) {
       st->kill = (killkinen && E > st->prev_energy) ? 1 : 0;
       st->prev_energy = E;
+      st->eslot[parity] = E;
       if (step - 1 < cap) histE[step - 1] = E;
     } else {
       st->kill = 0;
@@ -150,7 +152,7 @@ k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histE
 // k_fin_energy with the step counter already advanced).
 __global__ void __launch_bounds__(kFinThreads)
 k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, DevState* st, double* histErr,
-                 double* histE, int cap, int killkinen) {
+                 double* histE, int cap, int killkinen, int parity) {
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < n; i += kFinThreads) {
@@ -173,6 +175,7 @@ k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, De
     st->last_energy = E;
     st->kill = (killkinen && E > st->prev_energy) ? 1 : 0;
     st->prev_energy = E;
+    st->eslot[parity] = E;
     if (step < cap) histE[step] = E;
   }
 }
@@ -414,6 +417,18 @@ static bool use41(const ss_ctx* c) {
   return c->fuse41 && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec != SS_PREC_FP32C;
 }
 
+// Most tiles of the 4+1 sweep whose finalize is folded into the next step's
+// 2+3 sweep: every workgroup of that sweep reduces all of the partials again,
+// which pays only where the lattice is small and the one-block finalize launch
+// is a sizeable part of the step.
+constexpr int kFoldMaxTiles = 512;
+
+// The 4+1 schedule with each step's finalize folded into the next step's 2+3
+// sweep (not in the timed schedule, which separates every launch).
+static bool fold_mode(const ss_ctx* c, bool timed) {
+  return use41(c) && c->fuse23 && !timed && c->ops->ntiles(c->prec, fused41_v(c), c->g) <= kFoldMaxTiles;
+}
+
 // Shape variant of the fused 2+3 sweep: tv[2] where the sweep exists in it,
 // else the first strip variant.
 static int fused23_v(const ss_ctx* c) {
@@ -526,6 +541,17 @@ static PassBufs bufs_for(ss_ctx* c, int parity) {
 static int launch_step_exact(ss_ctx* c, int parity);
 static int launch_step_l2(ss_ctx* c, int parity);
 
+// Finalize of a 4+1 sweep that produced the field of parity ``parity``: err of
+// the step, energy of the new field, arrest decision.
+static int launch_fin41(ss_ctx* c, int parity) {
+  const int nt = c->ops->ntiles(c->prec, fused41_v(c), c->g);
+  hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part,
+                     (const double*)(c->part + c->ntiles), (const double*)(c->part + 2 * c->ntiles), nt, c->st,
+                     c->hist + c->hist_cap, c->hist, c->hist_cap, c->g.killkinen, parity);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
 // Launches of one step from buffer parity ``parity``.  With ``ev`` non-null an
 // event is recorded before the first and after every launch (7 events: pass 1,
 // energy finalize, pass 2 or 2+3, pass 3, pass 4 or 4+1, err finalize).
@@ -541,6 +567,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
   const bool f41 = use41(c);
+  const bool fold = fold_mode(c, ev != nullptr);
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
   double* p1 = c->part + c->ntiles;
@@ -556,7 +583,7 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     HIPCHK(c, c->ops->pass(c->prec, pass1_id(c), v1, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
-                       c->ops->ntiles(c->prec, v1, c->g), c->st, histE, c->hist_cap, c->g.killkinen);
+                       c->ops->ntiles(c->prec, v1, c->g), c->st, histE, c->hist_cap, c->g.killkinen, parity);
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   } else if (ev) {
@@ -566,7 +593,20 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (c->fuse23) {
     b.lout = c->l[1];
     b.lout2 = c->l[2];
+    if (fold && !first) {  // the previous step's finalize (its partials are those of the last 4+1 sweep)
+      b.fin_p0 = p0;
+      b.fin_p1 = p1;
+      b.fin_p2 = p2;
+      b.fin_n = c->ops->ntiles(c->prec, fused41_v(c), c->g);
+      b.fin_st = c->st;
+      b.histE = histE;
+      b.histErr = histErr;
+      b.hist_cap = c->hist_cap;
+      b.killkinen = c->g.killkinen;
+      b.parity = parity;
+    }
     HIPCHK(c, c->ops->pass(c->prec, 5, fused23_v(c), b, c->tp, c->g, c->stream));
+    b.fin_n = 0;
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate pass 3
   } else {
@@ -586,10 +626,8 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     b.part2 = p2;
     HIPCHK(c, c->ops->pass(c->prec, 6, v4, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-    const int nt = c->ops->ntiles(c->prec, v4, c->g);
-    hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
-                       (const double*)p2, nt, c->st, histErr, histE, c->hist_cap, c->g.killkinen);
-    HIPCHK(c, hipGetLastError());
+    if (fold) return 0;  // finalized by the next step's 2+3 sweep, or at the end of the call
+    if (launch_fin41(c, parity ^ 1)) return -1;
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     return 0;
   }
@@ -632,7 +670,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
   hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, c->nt0, c->st,
-                     c->hist, c->hist_cap, c->g.killkinen);
+                     c->hist, c->hist_cap, c->g.killkinen, parity);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
   const double cb[4] = {0.0, 0.0, 0.5, 0.5};
@@ -677,7 +715,7 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   PassBufs b = bufs_for(c, parity);
   HIPCHK(c, c->ops->density(c->prec, -1, b.F, nullptr, c->part, c->tp, c->g, c->stream));
   hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, c->nt0, c->st,
-                     c->hist, c->hist_cap, c->g.killkinen);
+                     c->hist, c->hist_cap, c->g.killkinen, parity);
   HIPCHK(c, hipGetLastError());
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
   const double cb[4] = {0.0, 0.0, 0.5, 0.5};
@@ -720,7 +758,7 @@ static int launch_finish(ss_ctx* c, int nsteps) {
     int np = 0;
     if (energy_partials(c, c->F[c->cur], &np)) return -1;
     hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part, np, c->st,
-                       c->hist, c->hist_cap, c->g.killkinen);
+                       c->hist, c->hist_cap, c->g.killkinen, c->cur);
     HIPCHK(c, hipGetLastError());
   }
   const long long n = (long long)c->g.nf * c->g.N;
@@ -1053,6 +1091,7 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
     }
     c->cur ^= 1;
   }
+  if (fold_mode(c, ev != nullptr) && nsteps > 0 && launch_fin41(c, c->cur)) return -1;
   if (launch_finish(c, nsteps)) return -1;
   c->last_nsteps = nsteps;
   return 0;

@@ -38,7 +38,58 @@ struct PassBufs {
   double* part2;   // fused 4+1: energy partial of the new field
   const DevState* st;
   float* Flo;      // compensated FP32 form, pass 4: low words of the field (read and written in place)
+  // Passes 2+3 of the 4+1 schedule on small lattices (fin_n > 0): the
+  // previous step's finalize folded in (see fold_finalize).
+  const double* fin_p0;
+  const double* fin_p1;
+  const double* fin_p2;
+  DevState* fin_st;
+  double* histE;
+  double* histErr;
+  int fin_n, hist_cap, killkinen, parity;
 };
+
+// The finalize of the 4+1 schedule (err of step n, energy of F_{n+1} and the
+// arrest decision) folded into the passes 2+3 of step n+1: every workgroup
+// reduces the same partials in the same order, so all reach the same kill
+// decision with no atomics; the workgroup of tile 0 (``writer``) writes the
+// arrest state.  The energy it compares with is eslot[parity ^ 1], the
+// previous field's (parity = that of F_{n+1}), which no workgroup of this
+// launch writes.  Same arithmetic as k_fin_err_energy.  All threads of the
+// block call it.
+template <int NT>
+__device__ __forceinline__ bool fold_finalize(const PassBufs& b, bool writer, double* red) {
+  double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < b.fin_n; i += NT) {
+    m0 = fmax(m0, b.fin_p0[i]);
+    m1 = fmax(m1, b.fin_p1[i]);
+    const double y = b.fin_p2[i] - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  const double a = block_max<NT>(m0, red);
+  const double e = block_max<NT>(m1, red);
+  const double E = block_sum<NT>(s, red);
+  __shared__ int s_kill;  // the reductions are valid in thread 0 only: it broadcasts the decision
+  DevState* st = b.fin_st;
+  if (threadIdx.x == 0) s_kill = (b.killkinen && E > st->eslot[b.parity ^ 1]) ? 1 : 0;
+  if (writer && threadIdx.x == 0) {
+    const bool kill = s_kill != 0;
+    st->err_max = a;
+    st->err_absmax = e;
+    const int step = st->step;
+    if (step < b.hist_cap) b.histErr[step] = a;
+    st->step = step + 1;
+    st->last_energy = E;
+    st->kill = kill ? 1 : 0;
+    st->prev_energy = E;
+    st->eslot[b.parity] = E;
+    if (step < b.hist_cap) b.histE[step] = E;
+  }
+  __syncthreads();
+  return s_kill != 0;
+}
 
 // Ring of NFR components (NF for one stage field, 2 NF when two stage fields
 // share a pass).
@@ -770,7 +821,11 @@ __device__ __forceinline__ void stream_pass_body(const PassBufs& b, const typena
   const int t = xcd_tile(blockIdx.x, L.nblocks, L.per_xcd);
   if (t < 0) return;  // whole block exits together, before any barrier
   const int sxi = t / L.nsy, syi = t - sxi * L.nsy;
-  const bool kill = (PASS >= 2) ? (b.st->kill != 0) : false;
+  bool kill;
+  if constexpr (PASS == 23)  // the previous step's finalize folded in, or its decision
+    kill = b.fin_n > 0 ? fold_finalize<kThreads>(b, t == 0, red) : (b.st->kill != 0);
+  else
+    kill = (PASS >= 2) ? (b.st->kill != 0) : false;
   stream_strip<TH, R, PASS, CW, COMP>(q, red, b, tp, g, sxi * L.sx, L.sx,
                                       syi * StreamRing<TH, R, PASS, CW, COMP>::shape::TYW, kill, t);
 }

@@ -606,6 +606,40 @@ def test_fused41_matches_separate_passes(precision, graphs):
     b.close()
 
 
+@pytest.mark.parametrize("graphs", [0, 4])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_folded_finalize_is_bit_exact(precision, graphs):
+    """On small lattices the 4+1 schedule folds each step's finalize (err, the
+    new field's energy, the arrest decision) into the next step's 2+3 sweep,
+    where every workgroup reduces the partials in the finalize kernel's order;
+    the timed schedule (ss_time_passes) launches the finalize on its own.  After
+    1, 2 and 9 steps from a rough field (arrests included) the fields and
+    velocities agree bit for bit, and the returned energy is the new field's."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 96, 80)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=8, rough=True)
+    for n in (1, 2, 9):
+        a = _sim(th, p, F, precision=precision)
+        b = _sim(th, p, F, precision=precision)
+        for s in (a, b):
+            s.set_fused_stages(True, True)
+        if graphs:
+            a.use_graphs(graphs)
+        E0 = a.compute_energy()
+        Ea, _, _ = a.steps(E0, n, history=True)
+        hE = a.last_history[0]
+        b.time_passes(E0, n)
+        rises = int(np.sum(np.diff(np.concatenate([[E0], hE])) > 0))
+        print(f"[folded finalize {precision} graphs={graphs} n={n}] energy rises={rises} "
+              f"relE={H.rel(Ea, a.compute_energy()):.2e}")
+        assert np.array_equal(a.get_field(), b.get_field())
+        assert np.array_equal(a.get_velocity(), b.get_velocity())
+        assert H.rel(Ea, a.compute_energy()) < (1e-12 if precision == "fp64" else 1e-6)
+        a.close()
+        b.close()
+
+
 _FUSED23_TILE_CASES = {
     "Chiral magnet": (lambda th: H.chiral_params(th, 70, 130),
                       lambda th, pi, pf: H.chiral_field(th, pi, pf, seed=5, rough=True)),
