@@ -10,8 +10,9 @@ renormalised.  One step = one arrested-Newton RK4 step over every site.
 The headline ``value`` is grid-point updates/s of the FP32-stencil /
 FP64-reduction form; the plain FP64 form is timed beside it on the same
 initial field.  ``roofline`` prices the dominant kernel (the RK4 pass 4
-launch, 7 nf w bytes per site, or with the fused 4+1 sweep that launch, 9 nf w
-bytes per site) from HIP events on the context's stream.
+launch, 7 nf w bytes per site; with the fused 4+1 sweep that launch, 9 nf w
+bytes per site; with the paired schedule the 3+4 sweep, 6 nf w bytes per site)
+from HIP events on the context's stream.
 ``cpu_baseline`` times the oracle's C++/OpenMP FP64 restatement of the
 reference path on a bounded sample of the same workload.
 """
@@ -48,7 +49,7 @@ def chiral_workload(n: int, seed: int = 0, nsk: int = 8, demag: bool = True):
 
 
 def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotune=True, tiles=None, fused23=-1,
-              fused41=-1):
+              fused41=-1, pairs=-1):
     import soliton_solver_amd as ssa
     sim = ssa.Simulation(p, th, precision=precision)
     if F0 is None:
@@ -57,8 +58,8 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
         sim.initialize({"mode": "array", "field": F0})
     if tiles is not None:
         sim.set_tile_variants(tiles)
-        if fused23 >= 0 or fused41 >= 0:
-            sim.set_fused_stages(fused23 > 0, fused41 > 0)
+        if fused23 >= 0 or fused41 >= 0 or pairs >= 0:
+            sim.set_fused_stages(fused23 > 0, fused41 > 0, pairs=pairs > 0)
         sim.tiles = sim.tile_variants()
     else:
         sim.tiles = sim.autotune_tiles(reps=20) if autotune else sim.tile_variants()
@@ -514,6 +515,8 @@ def main():
     ap.add_argument("--fused23", type=int, default=-1, help="with --tiles: 1 runs passes 2+3 as one sweep")
     ap.add_argument("--fused41", type=int, default=-1,
                     help="with --tiles: 1 runs pass 4 and the next step's pass 1 as one sweep")
+    ap.add_argument("--pairs", type=int, default=-1,
+                    help="with --tiles: 1 runs a step as the two paired sweeps, passes 1+2 and 3+4")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
     ap.add_argument("--reduced-form", choices=["fp32", "fp32c"], default="fp32",
                     help="throughput workload: the FP32 form whose rate is the headline value (fp32c: with the "
@@ -596,7 +599,7 @@ def main():
     sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, form, args.steps, args.warmup, args.graph_steps,
                                              autotune=not args.no_autotune,
                                              tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None,
-                                             fused23=args.fused23, fused41=args.fused41)
+                                             fused23=args.fused23, fused41=args.fused41, pairs=args.pairs)
     tiles32 = sim32.tiles
     print(f"[bench] fp32 tile rows per pass {tiles32['rows']}", file=sys.stderr, flush=True)
     if "ms" in tiles32:
@@ -609,6 +612,10 @@ def main():
         print(f"[bench]   fused passes 4+1 ms by shape variant: "
               + " ".join(f"{x:.4f}" for x in tiles32["ms_fused41"]) + f"; fused: {tiles32['fused41']}",
               file=sys.stderr, flush=True)
+        for key, label in (("ms_pairs12", "1+2"), ("ms_pairs34", "3+4")):
+            print(f"[bench]   paired sweep {label} ms by shape variant: "
+                  + " ".join(f"{x:.4f}" for x in tiles32[key]) + f"; paired: {tiles32['pairs']}",
+                  file=sys.stderr, flush=True)
     F0 = None
     N = n * n
     nf = 4
@@ -621,13 +628,16 @@ def main():
     # plus pass 1's 2) and pass 1 runs only in the first of the timed steps.
     # The compensated form's pass 4 also reads and writes the low words (9
     # words per site), its pass 1 reads them (3): 19 words per update.
+    # With the paired schedule the dominant launch is the 3+4 sweep (F, V, l1,
+    # l2 in; F', V' out: 6 words) and a step moves 10 words per update.
     passes = sim32.time_passes(E32, 20)
     w = 4
     f41 = bool(tiles32.get("fused41", False))
-    alg_pass4 = (9 if (f41 or comp) else 7) * nf * w * N
+    pairs = bool(tiles32.get("pairs", False))
+    alg_pass4 = (6 if pairs else 9 if (f41 or comp) else 7) * nf * w * N
     pass4_gbs = alg_pass4 / (passes[4] * 1e-3) / 1e9
     step_ms_kernels = float(passes.sum())
-    words_step = 19 if comp else 16
+    words_step = 19 if comp else 10 if pairs else 16
     alg_step = words_step * nf * w * N
     step_gbs = alg_step / (step_ms_kernels * 1e-3) / 1e9
     sim32.close()
@@ -723,12 +733,14 @@ def main():
         kernel = (f"stream_pass_comp<ChiralMagnet, float, 4, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
                   f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}, true>")
     else:
-        kernel = (f"stream_pass<ChiralMagnet, float, {41 if f41 else 4}, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
+        kernel = (f"stream_pass<ChiralMagnet, float, {34 if pairs else 41 if f41 else 4}, {2 if v4 >= 6 else 1}>"
+                  if v4 >= 3 else
                   f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>")
     traffic = None
     if not args.no_traffic and not args.profile_only:
         child = ["--tiles", ",".join(str(v) for v in tiles32["variants"]),
-                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)), "--n", str(n),
+                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)),
+                 "--pairs", str(int(pairs)), "--n", str(n),
                  "--reduced-form", form]
         traffic = measure_traffic(child, kernel)
         if traffic:
@@ -751,6 +763,7 @@ def main():
         "tile_variants": tiles32["variants"],
         "fused23": int(tiles32.get("fused23", False)),
         "fused41": int(f41),
+        "paired_sweeps": int(pairs),
         "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
                                "FP32 fields+stencils, FP64 energy sums and arrest test"
                                + ("; compensated field update (FP32 hi + lo)" if comp else ""),
@@ -758,6 +771,7 @@ def main():
                    "tile_rows_per_pass": tiles32["rows"], "tile_shape": "64 y x rows x",
                    "fused_passes_2_3": tiles32.get("fused23", False),
                    "fused_passes_4_1": f41,
+                   "paired_sweeps_1_2_and_3_4": pairs, "words_per_update": words_step,
                    "parallelism": "single GPU"},
         "hbm_frac_alg": ups32 * words_step * nf * w / (HBM_PEAK_GBS * 1e9),
         "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -195,7 +195,7 @@ int ss_time_passes(ss_ctx* ctx, double prev_energy, int nsteps, double* avg_ms);
  * per site; where a theory's rings do not fit that wide, they run as 3-5).
  * ss_autotune_tiles times every pass in every variant ``reps`` times on the
  * live buffers, keeps the fastest per pass, and reports chosen[0..3] and
- * ms[9*(pass-1)+variant] (ms holds 54 entries, see ss_set_fused_stages;
+ * ms[9*(pass-1)+variant] (ms holds 72 entries, see ss_set_fused_stages;
  * variants a pass does not run in read 0);
  * it writes only scratch buffers, so the relaxation state is unchanged.
  * ss_get_tile_variants reports rows per tile or strip.  Fields after a step
@@ -221,9 +221,22 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * pair of separate passes; ss_set_fused_stages forces them (a fused sweep uses
  * the variant of pass 2 or pass 4 where it runs in it, else variant 3) and
  * fails when neither fits for the theory and precision.  Results equal the
- * separate passes' up to the FP64 summation order of the energy. */
+ * separate passes' up to the FP64 summation order of the energy.
+ * SS_FUSE_PAIRS (set alone; plain FP32 and FP64 forms, strip variants): the
+ * stage field of pass 2, F + dt V/2, does not depend on l1, so a step runs as
+ * two sweeps, passes 1+2 (F, V in; l1, l2 and the energy partials out) and
+ * passes 3+4 (F, V, l1, l2 in; Temp3 and Temp4 in one ring, l3 kept in
+ * registers; F', V' and err partials out): 10 words per site and component
+ * instead of 14.  Passes 1+2 read V before the arrest decision; after an
+ * arrest passes 3+4 take l2 = l1, the l2 of V = 0.  Pass 1+2 uses the strip
+ * variant of pass 1, pass 3+4 that of pass 4 (variant 3 where the one set
+ * does not run the sweep).  ss_autotune_tiles times both sweeps in every
+ * strip variant (ms[54..62] for 1+2, ms[63..71] for 3+4) and switches the
+ * paired schedule on when its two sweeps beat the schedule it chose
+ * otherwise by 3 %. */
 #define SS_FUSE_23 1
 #define SS_FUSE_41 2
+#define SS_FUSE_PAIRS 4
 int ss_set_fused_stages(ss_ctx* ctx, int flags);
 int ss_get_fused_stages(ss_ctx* ctx, int* flags);
 

@@ -31,6 +31,7 @@ VAR_EXACT_ADJOINT = 1
 AUX_HIGGS, AUX_HIGGS1, AUX_HIGGS2, AUX_PHASE_DIFFERENCE, AUX_SUPERCURRENT, AUX_MAGNETIC_FIELD = range(6)
 FUSE_23 = 1
 FUSE_41 = 2
+FUSE_PAIRS = 4
 BUF_FIELD = 0
 BUF_VELOCITY = 1
 

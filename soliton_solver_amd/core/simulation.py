@@ -497,21 +497,25 @@ class Simulation:
         fused sweeps of passes 2+3 and 4+1 are timed in every variant they run in
         (tiles where two fields fit in LDS, strips).  Returns the chosen variants,
         their rows per tile/strip, the fused flags and the ms tables ([pass][variant],
-        fused 2+3 and 4+1 per variant; 0 where a sweep does not run)."""
+        fused 2+3 and 4+1 per variant, the paired sweeps 1+2 and 3+4 per variant;
+        0 where a sweep does not run)."""
         chosen = np.zeros(4, dtype=np.int32)
         nv = self.NUM_SHAPES
-        ms = np.zeros(6 * nv)
+        ms = np.zeros(8 * nv)
         self._check(_lib.lib().ss_autotune_tiles(self._ctx, int(reps), chosen, ms))
         info = self.tile_variants()
         info["ms"] = ms[:4 * nv].reshape(4, nv)
         info["ms_fused23"] = ms[4 * nv:5 * nv]
-        info["ms_fused41"] = ms[5 * nv:]
+        info["ms_fused41"] = ms[5 * nv:6 * nv]
+        info["ms_pairs12"] = ms[6 * nv:7 * nv]
+        info["ms_pairs34"] = ms[7 * nv:]
         return info
 
-    def set_fused_stages(self, fuse23: bool, fuse41: bool = False) -> None:
-        """Run passes 2+3 and/or pass 4 with the next step's pass 1 as single sweeps
-        (see ``ss_set_fused_stages``)."""
-        flags = (_lib.FUSE_23 if fuse23 else 0) | (_lib.FUSE_41 if fuse41 else 0)
+    def set_fused_stages(self, fuse23: bool, fuse41: bool = False, pairs: bool = False) -> None:
+        """Run passes 2+3 and/or pass 4 with the next step's pass 1 as single sweeps,
+        or (``pairs``, alone) the step as two sweeps, passes 1+2 and 3+4 (see
+        ``ss_set_fused_stages``)."""
+        flags = (_lib.FUSE_23 if fuse23 else 0) | (_lib.FUSE_41 if fuse41 else 0) | (_lib.FUSE_PAIRS if pairs else 0)
         self._check(_lib.lib().ss_set_fused_stages(self._ctx, flags))
 
     def set_tile_variants(self, variants) -> None:
@@ -528,7 +532,7 @@ class Simulation:
         on = C.c_int()
         self._check(_lib.lib().ss_get_fused_stages(self._ctx, C.byref(on)))
         return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value & _lib.FUSE_23),
-                "fused41": bool(on.value & _lib.FUSE_41)}
+                "fused41": bool(on.value & _lib.FUSE_41), "pairs": bool(on.value & _lib.FUSE_PAIRS)}
 
     # -- output bundle and resume (core/simulation.py:411-510, core/io.py) ---
     def save_output(self, output_dir: str | None = None, precision: int = 32) -> str:

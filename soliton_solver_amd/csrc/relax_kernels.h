@@ -878,6 +878,20 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 9:
+        if constexpr (StreamRing<TH, R, 12, CW>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 12, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
+      case 10:
+        if constexpr (StreamRing<TH, R, 34, CW>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass<TH, R, 34, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -930,6 +944,26 @@ struct TheoryLaunchers {
   }
   static int fuse41_variant(int prec, int v, const Geom& g) {
     return prec ? fuse41_variant_r<float>(v, g) : fuse41_variant_r<double>(v, g);
+  }
+
+  // The paired schedule (pass ids 9 = passes 1+2, 10 = passes 3+4) in strip
+  // variant v: both sweeps hold two stage fields in one ring.
+  template <typename R, int CW>
+  static constexpr bool pairs_fit() {
+    return StreamRing<TH, R, 12, CW>::shape::kFits && StreamRing<TH, R, 34, CW>::shape::kFits;
+  }
+  template <typename R>
+  static int pairs_variant_r(int v, const Geom& g) {
+    if (TH::kL2Normalize || v < 3) return 0;
+    v = effective_shape<R>(v, g);
+    if (v < 3) return 0;
+    if constexpr (wide_fits<R>()) {
+      if (strip_cw(v) == 2) return pairs_fit<R, 2>();
+    }
+    return pairs_fit<R, 1>();
+  }
+  static int pairs_variant(int prec, int v, const Geom& g) {
+    return prec ? pairs_variant_r<float>(v, g) : pairs_variant_r<double>(v, g);
   }
 
   // Passes 2+3 as one sweep in the batched form, shape variant v (instances
@@ -992,7 +1026,7 @@ struct TheoryLaunchers {
     constexpr int r0 = variant_rows<R, TH::NF>(0), r1 = variant_rows<R, TH::NF>(1), r2 = variant_rows<R, TH::NF>(2);
     static_assert(64 % StreamShape<R, TH::NF>::RB == 0, "strip rows must be a multiple of the ring group");
     v = effective_shape<R>(v, g);
-    if (pass == 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
+    if (pass >= 8) return v >= 3 ? stream_v<R>(pass, v, b, tp, g, st) : hipErrorInvalidValue;
     if (v >= 3) return stream_v<R>(pass, v, b, tp, g, st);
     if (v == 2 && r2 != r1) return pass_rv<R, r2>(pass, b, tp, g, st);
     if (v >= 1 && r1 != r0) return pass_rv<R, r1>(pass, b, tp, g, st);
@@ -1199,6 +1233,7 @@ struct TheoryOps {
   int (*can_comp)(int, const Geom&);
   int (*fuse23_variant)(int, int, const Geom&);
   int (*fuse41_variant)(int, int, const Geom&);
+  int (*pairs_variant)(int, int, const Geom&);
   int (*batch_fuse23)(int, int);
   int (*aux_components)(int);
   unsigned (*init_modes)();
@@ -1228,6 +1263,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::can_comp,                   \
                                       &ss::TheoryLaunchers<TH>::fuse23_variant,             \
                                       &ss::TheoryLaunchers<TH>::fuse41_variant,             \
+                                      &ss::TheoryLaunchers<TH>::pairs_variant,              \
                                       &ss::TheoryLaunchers<TH>::batch_fuse23,               \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \

@@ -281,6 +281,7 @@ struct ss_ctx {
   int tv[5] = {0, 0, 0, 0, 0};  // tile-shape variant of each fused pass (index = pass)
   int fuse23 = 0;          // passes 2 and 3 as one sweep (variant tv[2]: a strip, or a tile where two fields fit)
   int fuse41 = 0;          // pass 4 and the next step's pass 1 as one sweep (variant tv[4], at least 3)
+  int pairs = 0;           // paired schedule: passes 1+2 (variant tv[1]) and 3+4 (tv[4]) as two sweeps
   void* l1alt = nullptr;   // with fuse41: l1 of the steps that start from parity 1
   float* Flo = nullptr;    // SS_PREC_FP32C: low words of the field (nf*N, updated in place by pass 4)
   double* part = nullptr;  // 3 x ntiles
@@ -429,6 +430,15 @@ static bool fold_mode(const ss_ctx* c, bool timed) {
   return use41(c) && c->fuse23 && !timed && c->ops->ntiles(c->prec, fused41_v(c), c->g) <= kFoldMaxTiles;
 }
 
+// The paired schedule is active (reference flow, plain FP32 or FP64 form, and
+// both sweeps fit the first strip variant).
+static bool use_pairs(const ss_ctx* c) {
+  return c->pairs && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec != SS_PREC_FP32C &&
+         c->ops->pairs_variant(c->prec, 3, c->g);
+}
+// Strip variant of a paired sweep: v where both sweeps run in it, else 3.
+static int pairs_v(const ss_ctx* c, int v) { return c->ops->pairs_variant(c->prec, v, c->g) ? v : 3; }
+
 // Shape variant of the fused 2+3 sweep: tv[2] where the sweep exists in it,
 // else the first strip variant.
 static int fused23_v(const ss_ctx* c) {
@@ -466,6 +476,10 @@ static int energy_partials(ss_ctx* c, const void* F, int* nparts) {
 // falls back to the first strip variant where the chosen one lacks it; the
 // stored variants are set to what runs, so ss_get_tile_variants reports it.
 static void settle_variants(ss_ctx* c) {
+  if (c->pairs) {
+    c->tv[1] = pairs_v(c, c->tv[1]);
+    c->tv[4] = pairs_v(c, c->tv[4]);
+  }
   if (c->fuse23) c->tv[2] = fused23_v(c);
   if (c->fuse41) c->tv[4] = fused41_v(c);
   c->tv[1] = pass1_variant(c);
@@ -561,11 +575,45 @@ static int launch_fin41(ss_ctx* c, int parity) {
 // runs only in the first step of a call (``first``); every step ends with the
 // 4+1 sweep, which also gives l1 and the energy of the new field, so the
 // arrest decision follows it directly.
+// Paired schedule: passes 1+2 (l1 and the energy of F; l2 from Temp2 = F +
+// dt V/2, V read before the arrest decision) -> the decision -> passes 3+4
+// (after an arrest with l2 = l1, which is the l2 of V = 0) -> err.  Events as
+// in launch_step, passes 2 and 3 empty.
+static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
+  PassBufs b = bufs_for(c, parity);
+  double* p0 = c->part;
+  double* p1 = c->part + c->ntiles;
+  const int v12 = pairs_v(c, c->tv[1]), v34 = pairs_v(c, c->tv[4]);
+  int e = 0;
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = l1_buf(c, parity);
+  b.lout2 = c->l[1];
+  b.part0 = p0;
+  HIPCHK(c, c->ops->pass(c->prec, 9, v12, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
+                     c->ops->ntiles(c->prec, v12, c->g), c->st, c->hist, c->hist_cap, c->g.killkinen, parity);
+  HIPCHK(c, hipGetLastError());
+  if (ev)
+    for (int k = 0; k < 3; ++k) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = nullptr;
+  b.lout2 = nullptr;
+  b.part1 = p1;
+  HIPCHK(c, c->ops->pass(c->prec, 10, v34, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
+                     c->ops->ntiles(c->prec, v34, c->g), c->st, c->hist + c->hist_cap, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  return 0;
+}
+
 static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) {
     if (ev) return fail(c, "pass timing is available for the fused reference flow only");
     return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
+  if (use_pairs(c)) return launch_step_pairs(c, parity, ev);
   const bool f41 = use41(c);
   const bool fold = fold_mode(c, ev != nullptr);
   PassBufs b = bufs_for(c, parity);
@@ -1274,6 +1322,42 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
       }
     }
   }
+  // the paired schedule: passes 1+2 (id 9) and 3+4 (id 10) in the strip
+  // variants where both run; their outputs are the l buffers and the other
+  // parity's field, which the state does not hold
+  float best12 = 0.f, best34 = 0.f;
+  int pick12 = -1, pick34 = -1;
+  const bool any_pairs = c->prec != SS_PREC_FP32C && c->ops->pairs_variant(c->prec, 3, c->g);
+  for (int k = 0; k < 2 && rc == 0; ++k) {
+    const int pid = 9 + k;
+    b.lout = k == 0 ? c->l[0] : nullptr;
+    b.lout2 = k == 0 ? c->l[1] : nullptr;
+    for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
+      if (!any_pairs || !c->ops->pairs_variant(c->prec, v, c->g)) {
+        if (ms) ms[(6 + k) * c->ops->num_shapes + v] = 0.0;
+        continue;
+      }
+      hipError_t e = c->ops->pass(c->prec, pid, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = c->ops->pass(c->prec, pid, v, b, c->tp, c->g, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+      if (e != hipSuccess) {
+        rc = hipfail(c, "tile autotuning of paired sweep %d variant %d failed: %s", pid, v, hipGetErrorString(e));
+        break;
+      }
+      t /= reps;
+      if (ms) ms[(6 + k) * c->ops->num_shapes + v] = t;
+      float& best = k == 0 ? best12 : best34;
+      int& pick = k == 0 ? pick12 : pick34;
+      if (pick < 0 || t < best) {
+        best = t;
+        pick = v;
+      }
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (lo_scratch) {
@@ -1286,6 +1370,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     if (chosen) chosen[p - 1] = pick[p];
   }
   // a fused sweep replaces the separate passes only when at least 3 % faster
+  c->pairs = 0;
   c->fuse23 = 0;
   if (pick23 >= 0 && best23 < 0.97f * sep23) {
     c->fuse23 = 1;
@@ -1296,6 +1381,18 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
     c->fuse41 = 1;
     c->tv[4] = pick41;
   }
+  // the paired schedule replaces all of these when at least 3 % faster per step
+  const float sched = (c->fuse41 ? best41 : sep41) + (c->fuse23 ? best23 : sep23);
+  if (pick12 >= 0 && pick34 >= 0 && best12 + best34 < 0.97f * sched) {
+    c->fuse23 = c->fuse41 = 0;
+    c->pairs = 1;
+    c->tv[1] = pick12;
+    c->tv[4] = pick34;
+    if (chosen) {
+      chosen[0] = pick12;
+      chosen[3] = pick34;
+    }
+  }
   settle_variants(c);
   free_graphs(c);
   return 0;
@@ -1303,7 +1400,13 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
 
 int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (!c) return fail(nullptr, "null context");
-  if (flags & ~(SS_FUSE_23 | SS_FUSE_41)) return fail(c, "unknown fused-stage flags 0x%x", flags);
+  if (flags & ~(SS_FUSE_23 | SS_FUSE_41 | SS_FUSE_PAIRS)) return fail(c, "unknown fused-stage flags 0x%x", flags);
+  if ((flags & SS_FUSE_PAIRS) && (flags & (SS_FUSE_23 | SS_FUSE_41)))
+    return fail(c, "the paired schedule replaces the 2+3 and 4+1 sweeps: set it alone");
+  if ((flags & SS_FUSE_PAIRS) && !(c->ops->pairs_variant(c->prec, 3, c->g) && c->prec != SS_PREC_FP32C &&
+                                   c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize))
+    return fail(c, "passes 1+2 and 3+4 of theory %s cannot run as paired sweeps in this form and size",
+                c->ops->name);
   if ((flags & SS_FUSE_23) && !c->ops->fuse23_variant(c->prec, c->tv[2], c->g) && !c->ops->can_fuse23(c->prec, c->g))
     return fail(c, "passes 2 and 3 of theory %s cannot share one sweep at this precision and size", c->ops->name);
   if ((flags & SS_FUSE_41) && !can41(c))
@@ -1311,6 +1414,7 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
   if ((flags & SS_FUSE_41) && ensure_l1alt(c)) return -1;
   c->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
   c->fuse41 = (flags & SS_FUSE_41) ? 1 : 0;
+  c->pairs = (flags & SS_FUSE_PAIRS) ? 1 : 0;
   settle_variants(c);
   free_graphs(c);
   return 0;
@@ -1318,7 +1422,7 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
 
 int ss_get_fused_stages(ss_ctx* c, int* flags) {
   if (!c) return fail(nullptr, "null context");
-  if (flags) *flags = (c->fuse23 ? SS_FUSE_23 : 0) | (c->fuse41 ? SS_FUSE_41 : 0);
+  if (flags) *flags = (c->fuse23 ? SS_FUSE_23 : 0) | (c->fuse41 ? SS_FUSE_41 : 0) | (c->pairs ? SS_FUSE_PAIRS : 0);
   return 0;
 }
 

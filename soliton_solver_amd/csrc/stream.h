@@ -303,8 +303,8 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
 // Issue the loads of the group of rows starting at rb.  Element offsets are
 // 32-bit (the host keeps nf*X*Y below 2^31); a site outside the lattice loads
 // element 0 and is replaced by 0, so no load sits behind a branch.  PASS
-// selects the arrays: 0/1 F; 2 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 41 F, V,
-// l2, l1, l3; pass 1 of the compensated form (PASS 1 with a two-half ring) F
+// selects the arrays: 0/1 F; 2 and 12 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 34
+// F, V, l1, l2 (into la); 41 F, V, l2, l1, l3; pass 1 of the compensated form (PASS 1 with a two-half ring) F
 // and its low words (into l).
 template <int PASS, typename R, int NF, int NFR, int SC>
 __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, const PassBufs& b,
@@ -331,7 +331,7 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
       } else {
         p.v[k][a] = R(0);
       }
-      if constexpr (PASS >= 3) {
+      if constexpr (PASS >= 3 && PASS != 12) {
         const R lv = ld<R>(plane<R>(L, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
       } else if constexpr (PASS == 1 && NFR == 2 * NF) {
@@ -344,6 +344,10 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
         const R l1v = ld<R>(plane<R>(b.l1, a, NP), boff), l3v = ld<R>(plane<R>(b.l3, a, NP), boff);
         p.la[k][a] = in ? l1v : R(0);
         p.lc[k][a] = in ? l3v : R(0);
+      }
+      if (PASS == 34) {  // l2 (l1 after an arrest: the caller points b.l2 there)
+        const R l2v = ld<R>(plane<R>(b.l2, a, NP), boff);
+        p.la[k][a] = in ? l2v : R(0);
       }
     }
   }
@@ -399,7 +403,8 @@ __device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
 }
 
 // Write a group into the ring; slot0 is the ring slot of its first row.  PASS
-// 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to NF..2NF-1.
+// 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to
+// NF..2NF-1; PASS 12 F and Temp2; PASS 34 Temp3 and Temp4.
 template <int PASS, typename R, int NF, int NFR, int SC>
 __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
                                             const StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, int slot0,
@@ -416,6 +421,12 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
       if (PASS == 23) {
         q.v[a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
         q.v[NF + a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+      } else if (PASS == 12) {
+        q.v[a][slot][c] = p.f[k][a];
+        q.v[NF + a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], R(0), dt);
+      } else if (PASS == 34) {
+        q.v[a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
+        q.v[NF + a][slot][c] = stage_value<4, R>(p.f[k][a], p.v[k][a], p.la[k][a], dt);
       } else {
         q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
         if constexpr (PASS == 1 && NFR == 2 * NF) q.v[NF + a][slot][c] = p.l[k][a];  // low words
@@ -475,9 +486,10 @@ __device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q,
 // row wave; column y), from
 // the stage field in ring components AOFF .. AOFF+NF-1.
 // Pass 4 takes V and l2 of its sites from kV / kL ([k][a], kill applied), and
-// in the 4+1 sweep also l1 and l3 from kA / kC;
-// with FOFF >= 0 it takes the new field F' from ring components FOFF ..
+// in the 4+1 and 3+4 sweeps (FOFF >= 0, FOFF == -2) also l1 and l3 from kA /
+// kC; with FOFF >= 0 it takes the new field F' from ring components FOFF ..
 // FOFF+NF-1 (the 4+1 sweep, which builds F' there) instead of forming it.
+// Passes 0-3 with FOFF == -3 leave l in kOut ([k][a]) instead of storing it.
 // COMP (FP32): pass 4 forms the new field with compensated_sum / _split from
 // F and its low words; pass 1 takes the energy of F + low words (ring
 // components NF..2NF-1).
@@ -485,7 +497,8 @@ template <class TH, typename R, int PASS, bool INTERIOR, int NFR, int AOFF, int 
 __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const PassBufs& b, const typename TH::TP& tp,
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
-                                            double& acc1, const R* kA = nullptr, const R* kC = nullptr) {
+                                            double& acc1, const R* kA = nullptr, const R* kC = nullptr,
+                                            R* kOut = nullptr) {
   using S = StreamShape<R, NFR, SC>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
@@ -495,7 +508,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
   // stencils so their latency overlaps the variation (the stores of the loop
   // below would otherwise keep the compiler from hoisting them).  In FP64 the
   // registers are not there to spare, and they are loaded at the combine.
-  constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF < 0 && !COMP;
+  constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF == -1 && !COMP;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
   if (kPre) {
 #pragma unroll
@@ -541,6 +554,9 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       if (PASS == 0) {
 #pragma unroll
         for (int a = 0; a < NF; ++a) st<R>(b.lout, a, NP, boff, eg[a]);
+      } else if (PASS <= 3 && FOFF == -3) {  // kept in registers for the pass 4 that follows
+#pragma unroll
+        for (int a = 0; a < NF; ++a) kOut[k * NF + a] = -dt * eg[a];
       } else if (PASS <= 3) {
 #pragma unroll
         for (int a = 0; a < NF; ++a) st<R>(b.lout, a, NP, boff, -dt * eg[a]);
@@ -566,7 +582,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           const R Vc = kV[k * NF + a];
           const R a2 = kL[k * NF + a];
           R a1, a3;
-          if constexpr (FOFF >= 0) {  // 4+1 sweep: kept from its front loads
+          if constexpr (FOFF >= 0 || FOFF == -2) {  // 4+1 sweep: kept from its front loads; 3+4: l3 just formed
             a1 = kA[k * NF + a];
             a3 = kC[k * NF + a];
           } else if constexpr (kPre) {  // issued before the stencils
@@ -618,13 +634,20 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 // field F' = Temp4 + dt (l1 - 2 l2 + l3)/6 is pointwise in the inputs of pass
 // 4, so Temp4 and F' are built from the same loads into the two halves of the
 // ring, pass 4 runs on the first half (F', V' -> Fout, Vout) and pass 1 on the
-// second (l1 of the next step -> lout, its energy partial -> part2)).
+// second (l1 of the next step -> lout, its energy partial -> part2)); 12 =
+// passes 1 and 2 (Temp2 = F + dt V/2 does not depend on l1: F and Temp2 in the
+// two halves, l1 -> lout with the energy partial, l2 -> lout2; V is read
+// without the arrest, which the caller applies afterwards by taking l2 = l1,
+// exactly the l2 of an arrested V); 34 = passes 3 and 4 (Temp3 from F, V, l1
+// and Temp4 from F, V, l2 in the two halves; l3 stays in registers for the
+// combine at the same sites).
 // Partials: one per strip.
 template <class TH, typename R, int PASS, int CW = 1, bool COMP = false>
 struct StreamRing {
-  static constexpr int NFR = (PASS == 23 || PASS == 41 || (PASS == 1 && COMP)) ? 2 * TH::NF : TH::NF;
-  // the 4+1 sweep prefetches five arrays: one row per wave and iteration
-  static constexpr int SC = (PASS == 41 ? 1 : 2) | (CW == 2 ? 4 : 0);
+  static constexpr int NFR =
+      (PASS == 23 || PASS == 41 || PASS == 12 || PASS == 34 || (PASS == 1 && COMP)) ? 2 * TH::NF : TH::NF;
+  // the 4+1 and 3+4 sweeps prefetch five and four arrays: one row per wave and iteration
+  static constexpr int SC = (PASS == 41 || PASS == 34 ? 1 : 2) | (CW == 2 ? 4 : 0);
   using shape = StreamShape<R, NFR, SC>;
   using type = Ring<R, NFR, SC>;
 };
@@ -680,10 +703,17 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       const R fo = ld<R>(plane<R>(b.F, a, NP), boff);
       const R vr = (PASS >= 2) ? ld<R>(plane<R>(b.V, a, NP), boff) : R(0);
       const R v = kill ? R(0) : vr;
-      const R l = (PASS == 3 || PASS == 23) ? ld<R>(plane<R>(b.l1, a, NP), boff)
-                : PASS == 4                 ? ld<R>(plane<R>(b.l2, a, NP), boff)
-                                            : R(0);
-      if (PASS == 23) {
+      const R l = (PASS == 3 || PASS == 23 || PASS == 34) ? ld<R>(plane<R>(b.l1, a, NP), boff)
+                : PASS == 4                               ? ld<R>(plane<R>(b.l2, a, NP), boff)
+                                                          : R(0);
+      if (PASS == 12) {
+        q.v[a][r][c] = in ? fo : R(0);
+        q.v[NF + a][r][c] = in ? stage_value<2, R>(fo, v, R(0), dt) : R(0);
+      } else if (PASS == 34) {
+        const R l2 = ld<R>(plane<R>(b.l2, a, NP), boff);
+        q.v[a][r][c] = in ? stage_value<3, R>(fo, v, l, dt) : R(0);
+        q.v[NF + a][r][c] = in ? stage_value<4, R>(fo, v, l2, dt) : R(0);
+      } else if (PASS == 23) {
         q.v[a][r][c] = in ? stage_value<2, R>(fo, v, l, dt) : R(0);
         q.v[NF + a][r][c] = in ? stage_value<3, R>(fo, v, l, dt) : R(0);
       } else {
@@ -701,7 +731,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
   R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
   // 4+1 sweep: l1 and l3 of the same sites (its front loads carry them already)
   R keepA[RPW][NF], keepC[RPW][NF], holdA[RPW][NF], holdC[RPW][NF];
-  if (PASS == 4 || PASS == 41) {
+  if (PASS == 4 || PASS == 41 || PASS == 34) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int xk = xs + w * RPW + k, xh = xs + RB + w * RPW + k;
@@ -716,6 +746,11 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
         keepL[k][a] = ink ? q0 : R(0);
         holdV[k][a] = (inh && !kill) ? v1 : R(0);
         holdL[k][a] = inh ? q1 : R(0);
+        if (PASS == 34) {
+          const R a0 = ld<R>(plane<R>(b.l1, a, NP), bk), a1 = ld<R>(plane<R>(b.l1, a, NP), bh);
+          keepA[k][a] = ink ? a0 : R(0);
+          holdA[k][a] = inh ? a1 : R(0);
+        }
         if (PASS == 41) {
           const R a0 = ld<R>(plane<R>(b.l1, a, NP), bk), c0v = ld<R>(plane<R>(b.l3, a, NP), bk);
           const R a1 = ld<R>(plane<R>(b.l1, a, NP), bh), c1v = ld<R>(plane<R>(b.l3, a, NP), bh);
@@ -732,7 +767,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
 
   PassBufs b2 = b, b3 = b;
   b3.lout = b.lout2;
-  double acc0 = (PASS == 4 || PASS == 41) ? -1.0e300 : 0.0;
+  double acc0 = (PASS == 4 || PASS == 41 || PASS == 34) ? -1.0e300 : 0.0;
   double acc1 = 0.0, acc2 = 0.0, unused = 0.0;
   for (int it = 0; it < nit; ++it) {
     __syncthreads();  // ring holds rows xs + it*RB - 2 .. xs + it*RB + RB + 1
@@ -747,6 +782,34 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       } else {
         stream_rows<TH, R, 2, false, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
         stream_rows<TH, R, 3, false, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+      }
+    } else if (PASS == 12) {
+      // pass 1 from F (ring 0..NF-1: l1 and the energy), pass 2 from Temp2 (ring NF..2NF-1)
+      if (interior) {
+        stream_rows<TH, R, 1, true, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 2, true, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused);
+      } else {
+        stream_rows<TH, R, 1, false, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 2, false, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused,
+                                                  unused);
+      }
+    } else if (PASS == 34) {
+      // pass 3 from Temp3 (ring 0..NF-1, l3 into registers), then pass 4 from Temp4 (ring NF..2NF-1)
+      R l3r[RPW][NF];
+#pragma unroll
+      for (int k = 0; k < RPW; ++k)
+#pragma unroll
+        for (int a = 0; a < NF; ++a) l3r[k][a] = R(0);
+      if (interior) {
+        stream_rows<TH, R, 3, true, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
+                                                nullptr, nullptr, &l3r[0][0]);
+        stream_rows<TH, R, 4, true, NFR, NF, -2>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                                 acc1, &keepA[0][0], &l3r[0][0]);
+      } else {
+        stream_rows<TH, R, 3, false, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
+                                                 nullptr, nullptr, &l3r[0][0]);
+        stream_rows<TH, R, 4, false, NFR, NF, -2>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
+                                                  acc1, &keepA[0][0], &l3r[0][0]);
       }
     } else if (PASS == 41) {
       // pass 4 from Temp4 (ring 0..NF-1), then pass 1 of the next step from F' (ring NF..2NF-1)
@@ -774,7 +837,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
         group_store41<TH, R, NFR, SC>(q, pf, (it * RB + RB + 4) & (NR - 1), dt, tp, b, g, c0 + RB + 2, y0, kill);
       else
         group_store<PASS, R, NF, NFR, SC>(q, pf, (it * RB + RB + 4) & (NR - 1), dt);
-      if (PASS == 4 || PASS == 41) {
+      if (PASS == 4 || PASS == 41 || PASS == 34) {
         // next iteration's sites: the group just stored, or (first two compute
         // rows) the hold from the group before; the hold takes the new group.
 #pragma unroll
@@ -782,10 +845,16 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
           const bool lag = w * RPW + k < 2;
 #pragma unroll
           for (int a = 0; a < NF; ++a) {
+            // 3+4: the prefetch holds l1 in l and l2 in la
+            const R nl = PASS == 34 ? pf.la[k][a] : pf.l[k][a];
             keepV[k][a] = lag ? holdV[k][a] : pf.v[k][a];
-            keepL[k][a] = lag ? holdL[k][a] : pf.l[k][a];
+            keepL[k][a] = lag ? holdL[k][a] : nl;
             holdV[k][a] = pf.v[k][a];
-            holdL[k][a] = pf.l[k][a];
+            holdL[k][a] = nl;
+            if (PASS == 34) {
+              keepA[k][a] = lag ? holdA[k][a] : pf.l[k][a];
+              holdA[k][a] = pf.l[k][a];
+            }
             if (PASS == 41) {
               keepA[k][a] = lag ? holdA[k][a] : pf.la[k][a];
               keepC[k][a] = lag ? holdC[k][a] : pf.lc[k][a];
@@ -798,10 +867,10 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       if (it + 2 < nit) group_load<PASS, R, NF, NFR, SC>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
     }
   }
-  if (PASS == 1) {
+  if (PASS == 1 || PASS == 12) {
     const double s = block_sum<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = s;
-  } else if (PASS == 4 || PASS == 41) {
+  } else if (PASS == 4 || PASS == 41 || PASS == 34) {
     const double m0 = block_max<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = m0;
     const double m1 = block_max<kThreads>(acc1, red);
@@ -825,8 +894,10 @@ __device__ __forceinline__ void stream_pass_body(const PassBufs& b, const typena
   if constexpr (PASS == 23)  // the previous step's finalize folded in, or its decision
     kill = b.fin_n > 0 ? fold_finalize<kThreads>(b, t == 0, red) : (b.st->kill != 0);
   else
-    kill = (PASS >= 2) ? (b.st->kill != 0) : false;
-  stream_strip<TH, R, PASS, CW, COMP>(q, red, b, tp, g, sxi * L.sx, L.sx,
+    kill = (PASS >= 2 && PASS != 12) ? (b.st->kill != 0) : false;  // 1+2 reads V before the arrest decision
+  PassBufs ba = b;
+  if (PASS == 34 && kill) ba.l2 = b.l1;  // l2 of an arrested V is l1 (Temp2 = F)
+  stream_strip<TH, R, PASS, CW, COMP>(q, red, ba, tp, g, sxi * L.sx, L.sx,
                                       syi * StreamRing<TH, R, PASS, CW, COMP>::shape::TYW, kill, t);
 }
 

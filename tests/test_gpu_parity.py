@@ -640,6 +640,88 @@ def test_folded_finalize_is_bit_exact(precision, graphs):
         b.close()
 
 
+def _paired_cases():
+    return ["Chiral magnet", *_FP32_CASES]
+
+
+@pytest.mark.parametrize("variants", [[3, 3, 3, 3], [6, 6, 6, 8]])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", _paired_cases())
+def test_paired_schedule_matches_separate_passes(name, precision, variants):
+    """Passes 1+2 and 3+4 as two sweeps (Temp2 = F + dt V/2 needs no l1; l3
+    stays in registers for the combine) against the separate passes in the
+    same strips: calls of 0, 1, 2, 7 and 5 steps (chiral: also in graphs of 4
+    steps), then 2 steps from prev_energy = -inf, whose first step is arrested
+    (passes 3+4 take l2 = l1): the same energies, errs, fields and velocities."""
+    th = ssa.load_theory(name)
+    if name == "Chiral magnet":
+        p = H.chiral_params(th, 70, 130)
+        p_i, p_f = H.packed(th, p)
+        F = H.chiral_field(th, p_i, p_f, seed=5, rough=True)
+    else:
+        mkp, mkf = _FP32_CASES[name]
+        p = mkp(th)
+        p_i, p_f = H.packed(th, p)
+        F = mkf(th, p_i, p_f)
+    tE, tF, tR = (1e-12, 1e-12, 1e-10) if precision == "fp64" else (1e-7, 1e-5, 1e-4)
+    a = _sim(th, p, F, precision=precision)
+    b = _sim(th, p, F, precision=precision)
+    a.set_tile_variants(variants)
+    b.set_tile_variants(variants)
+    try:
+        a.set_fused_stages(False, False, pairs=True)
+    except ValueError:  # the two-field ring does not fit this theory and precision
+        a.close()
+        b.close()
+        pytest.skip(f"no paired sweeps for {name} in {precision}")
+    assert a.tile_variants()["pairs"]
+    if name == "Chiral magnet":
+        a.use_graphs(4)
+        b.use_graphs(4)
+    E0 = a.compute_energy()
+    counts = [0, 1, 2, 7, 5]
+    Ea, hEa, hRa = _relax_calls(a, E0, counts)
+    Eb, hEb, hRb = _relax_calls(b, E0, counts)
+    scale = max(1.0, np.abs(F).max())
+    dF = np.abs(a.get_field() - b.get_field()).max() / scale
+    dV = np.abs(a.get_velocity() - b.get_velocity()).max() / scale
+    print(f"[paired {name} {precision} {variants}] relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e} max|dV|={dV:.2e}")
+    assert len(hEa) == sum(counts)
+    assert np.all(np.abs(hEa - hEb) <= tE * np.abs(hEb))
+    assert np.all(np.abs(hRa - hRb) <= tR * np.abs(hRb).max())
+    assert dF <= tF and dV <= tF
+    # an arrested first step: V is zeroed and l2 = l1
+    Ea, _, _ = a.steps(-np.inf, 2)
+    Eb, _, _ = b.steps(-np.inf, 2)
+    dF = np.abs(a.get_field() - b.get_field()).max() / scale
+    dV = np.abs(a.get_velocity() - b.get_velocity()).max() / scale
+    print(f"[paired {name} {precision} arrested] relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e} max|dV|={dV:.2e}")
+    assert H.rel(Ea, Eb) < tE
+    assert dF <= tF and dV <= tF
+    a.close()
+    b.close()
+
+
+def test_paired_schedule_refusals():
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 48, 40)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=1)
+    s = _sim(th, p, F)
+    with pytest.raises(ValueError):
+        s.set_fused_stages(True, False, pairs=True)  # the pairs replace the other sweeps
+    s.close()
+    c = _sim(th, p, F, precision="fp32c")
+    with pytest.raises(ValueError):
+        c.set_fused_stages(False, False, pairs=True)
+    c.close()
+    th = ssa.load_theory("Bose-Einstein condensate")
+    s = ssa.Simulation(H.bec_params(th, 32, 32), th)
+    with pytest.raises(ValueError):
+        s.set_fused_stages(False, False, pairs=True)
+    s.close()
+
+
 _FUSED23_TILE_CASES = {
     "Chiral magnet": (lambda th: H.chiral_params(th, 70, 130),
                       lambda th, pi, pf: H.chiral_field(th, pi, pf, seed=5, rough=True)),
@@ -746,6 +828,8 @@ def test_fused41_survives_precision_switch_and_autotune():
     # 16- and 8-row tiles and every strip do
     assert len(info["ms_fused41"]) == 9 and info["ms_fused41"][0] == 0 and np.all(info["ms_fused41"][1:] > 0)
     assert len(info["ms_fused23"]) == 9 and info["ms_fused23"][0] == 0 and np.all(info["ms_fused23"][1:] > 0)
+    for key in ("ms_pairs12", "ms_pairs34"):  # the paired sweeps run in the strip variants only
+        assert len(info[key]) == 9 and np.all(info[key][:3] == 0) and np.all(info[key][3:] > 0)
     a.set_tile_variants([0, 3, 3, 3])
     a.set_fused_stages(True, True)
     b = _sim(th, p, F, precision="fp32")
