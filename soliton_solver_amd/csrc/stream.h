@@ -647,7 +647,7 @@ struct StreamRing {
   static constexpr int NFR =
       (PASS == 23 || PASS == 41 || PASS == 12 || PASS == 34 || (PASS == 1 && COMP)) ? 2 * TH::NF : TH::NF;
   // the 4+1 and 3+4 sweeps prefetch five and four arrays: one row per wave and iteration
-  static constexpr int SC = (PASS == 41 || PASS == 34 ? 1 : 2) | (CW == 2 ? 4 : 0);
+  static constexpr int SC = (PASS == 41 || PASS == 34 || PASS == 12 ? 1 : 2) | (CW == 2 ? 4 : 0);
   using shape = StreamShape<R, NFR, SC>;
   using type = Ring<R, NFR, SC>;
 };
