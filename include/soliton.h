@@ -222,7 +222,7 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * the variant of pass 2 or pass 4 where it runs in it, else variant 3) and
  * fails when neither fits for the theory and precision.  Results equal the
  * separate passes' up to the FP64 summation order of the energy.
- * SS_FUSE_PAIRS (set alone; plain FP32 and FP64 forms, strip variants): the
+ * SS_FUSE_PAIRS (set alone; strip variants): the
  * stage field of pass 2, F + dt V/2, does not depend on l1, so a step runs as
  * two sweeps, passes 1+2 (F, V in; l1, l2 and the energy partials out) and
  * passes 3+4 (F, V, l1, l2 in; Temp3 and Temp4 in one ring, l3 kept in
@@ -230,7 +230,10 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * instead of 14.  Passes 1+2 read V before the arrest decision; after an
  * arrest passes 3+4 take l2 = l1, the l2 of V = 0.  Pass 1+2 uses the strip
  * variant of pass 1, pass 3+4 that of pass 4 (variant 3 where the one set
- * does not run the sweep).  ss_autotune_tiles times both sweeps in every
+ * does not run the sweep).  In the compensated form (whose pass 1 carries
+ * the low words in the second ring half) only passes 3+4 pair, as the
+ * compensated combine after a separate pass 2: pass 1 (3 words) + pass 2
+ * (3) + 3+4 (8) = 14 words instead of 17.  ss_autotune_tiles times both sweeps in every
  * strip variant (ms[54..62] for 1+2, ms[63..71] for 3+4) and switches the
  * paired schedule on when its two sweeps beat the schedule it chose
  * otherwise by 3 %. */

@@ -892,6 +892,13 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 11:
+        if constexpr (sizeof(R) == 4 && StreamRing<TH, R, 34, CW, true>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass_comp<TH, R, 34, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -436,6 +436,12 @@ static bool use_pairs(const ss_ctx* c) {
   return c->pairs && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec != SS_PREC_FP32C &&
          c->ops->pairs_variant(c->prec, 3, c->g);
 }
+// The compensated form's paired schedule: its pass 1 (id 8) carries the low
+// words in the second ring half, so only passes 3 and 4 pair (id 11).
+static bool use_comp_pairs(const ss_ctx* c) {
+  return c->pairs && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize && c->prec == SS_PREC_FP32C &&
+         c->ops->pairs_variant(c->prec, 3, c->g);
+}
 // Strip variant of a paired sweep: v where both sweeps run in it, else 3.
 static int pairs_v(const ss_ctx* c, int v) { return c->ops->pairs_variant(c->prec, v, c->g) ? v : 3; }
 
@@ -608,12 +614,45 @@ static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
   return 0;
 }
 
+// Compensated form, paired: pass 1 (low words, energy of hi + lo) -> the
+// arrest decision -> pass 2 (V arrested) -> compensated 3+4 -> err.
+static int launch_step_comp_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
+  PassBufs b = bufs_for(c, parity);
+  double* p0 = c->part;
+  double* p1 = c->part + c->ntiles;
+  const int v1 = pass1_variant(c), v34 = pairs_v(c, c->tv[4]);
+  int e = 0;
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = l1_buf(c, parity);
+  b.part0 = p0;
+  HIPCHK(c, c->ops->pass(c->prec, 8, v1, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
+                     c->ops->ntiles(c->prec, v1, c->g), c->st, c->hist, c->hist_cap, c->g.killkinen, parity);
+  HIPCHK(c, hipGetLastError());
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  b.lout = c->l[1];
+  HIPCHK(c, c->ops->pass(c->prec, 2, c->tv[2], b, c->tp, c->g, c->stream));
+  if (ev)
+    for (int k = 0; k < 2; ++k) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate pass 3
+  b.lout = nullptr;
+  b.part1 = p1;
+  HIPCHK(c, c->ops->pass(c->prec, 11, v34, b, c->tp, c->g, c->stream));
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
+                     c->ops->ntiles(c->prec, v34, c->g), c->st, c->hist + c->hist_cap, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
+  if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  return 0;
+}
+
 static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   if (c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) {
     if (ev) return fail(c, "pass timing is available for the fused reference flow only");
     return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
   if (use_pairs(c)) return launch_step_pairs(c, parity, ev);
+  if (use_comp_pairs(c)) return launch_step_comp_pairs(c, parity, ev);
   const bool f41 = use41(c);
   const bool fold = fold_mode(c, ev != nullptr);
   PassBufs b = bufs_for(c, parity);
@@ -1327,9 +1366,15 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   // parity's field, which the state does not hold
   float best12 = 0.f, best34 = 0.f;
   int pick12 = -1, pick34 = -1;
-  const bool any_pairs = c->prec != SS_PREC_FP32C && c->ops->pairs_variant(c->prec, 3, c->g);
+  // (compensated form: only its 3+4 sweep, id 11, after a separate pass 2)
+  const bool compf = c->prec == SS_PREC_FP32C;
+  const bool any_pairs = c->ops->pairs_variant(c->prec, 3, c->g);
   for (int k = 0; k < 2 && rc == 0; ++k) {
-    const int pid = 9 + k;
+    const int pid = k == 0 ? 9 : compf ? 11 : 10;
+    if (compf && k == 0) {
+      for (int v = 0; v < c->ops->num_shapes && ms; ++v) ms[6 * c->ops->num_shapes + v] = 0.0;
+      continue;
+    }
     b.lout = k == 0 ? c->l[0] : nullptr;
     b.lout2 = k == 0 ? c->l[1] : nullptr;
     for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
@@ -1393,6 +1438,16 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
       chosen[3] = pick34;
     }
   }
+  if (compf && pick34 >= 0 && bestp[2] + best34 < 0.97f * ((c->fuse23 ? best23 : sep23) + bestp[4])) {
+    c->fuse23 = 0;
+    c->pairs = 1;
+    c->tv[2] = pick[2];
+    c->tv[4] = pick34;
+    if (chosen) {
+      chosen[1] = pick[2];
+      chosen[3] = pick34;
+    }
+  }
   settle_variants(c);
   free_graphs(c);
   return 0;
@@ -1403,8 +1458,8 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (flags & ~(SS_FUSE_23 | SS_FUSE_41 | SS_FUSE_PAIRS)) return fail(c, "unknown fused-stage flags 0x%x", flags);
   if ((flags & SS_FUSE_PAIRS) && (flags & (SS_FUSE_23 | SS_FUSE_41)))
     return fail(c, "the paired schedule replaces the 2+3 and 4+1 sweeps: set it alone");
-  if ((flags & SS_FUSE_PAIRS) && !(c->ops->pairs_variant(c->prec, 3, c->g) && c->prec != SS_PREC_FP32C &&
-                                   c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize))
+  if ((flags & SS_FUSE_PAIRS) &&
+      !(c->ops->pairs_variant(c->prec, 3, c->g) && c->var_mode == SS_VAR_REFERENCE && !c->ops->l2_normalize))
     return fail(c, "passes 1+2 and 3+4 of theory %s cannot run as paired sweeps in this form and size",
                 c->ops->name);
   if ((flags & SS_FUSE_23) && !c->ops->fuse23_variant(c->prec, c->tv[2], c->g) && !c->ops->can_fuse23(c->prec, c->g))

@@ -640,7 +640,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 // without the arrest, which the caller applies afterwards by taking l2 = l1,
 // exactly the l2 of an arrested V); 34 = passes 3 and 4 (Temp3 from F, V, l1
 // and Temp4 from F, V, l2 in the two halves; l3 stays in registers for the
-// combine at the same sites).
+// combine at the same sites; with COMP the combine is the compensated one).
 // Partials: one per strip.
 template <class TH, typename R, int PASS, int CW = 1, bool COMP = false>
 struct StreamRing {
@@ -803,13 +803,13 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       if (interior) {
         stream_rows<TH, R, 3, true, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
                                                 nullptr, nullptr, &l3r[0][0]);
-        stream_rows<TH, R, 4, true, NFR, NF, -2>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                                 acc1, &keepA[0][0], &l3r[0][0]);
+        stream_rows<TH, R, 4, true, NFR, NF, -2, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
+                                                           &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0]);
       } else {
         stream_rows<TH, R, 3, false, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
                                                  nullptr, nullptr, &l3r[0][0]);
-        stream_rows<TH, R, 4, false, NFR, NF, -2>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0], &keepL[0][0], acc0,
-                                                  acc1, &keepA[0][0], &l3r[0][0]);
+        stream_rows<TH, R, 4, false, NFR, NF, -2, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
+                                                            &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0]);
       }
     } else if (PASS == 41) {
       // pass 4 from Temp4 (ring 0..NF-1), then pass 1 of the next step from F' (ring NF..2NF-1)

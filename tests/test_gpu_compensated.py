@@ -114,6 +114,49 @@ def test_fp32c_tracks_fp64_for_every_theory(name, variants):
     c.close()
 
 
+@pytest.mark.parametrize("name", list(_CASES))
+def test_fp32c_paired_matches_unpaired(name):
+    """The compensated form's paired schedule (pass 1, pass 2 after the arrest
+    decision, then passes 3+4 with the compensated combine in one sweep)
+    against its 2+3 / 4 schedule in the same strips: 20 steps in calls of 7
+    and 13, then 2 steps from prev_energy = -inf (an arrested first step)."""
+    th = ssa.load_theory(name)
+    mkp, mkf = _CASES[name]
+    p = mkp(th)
+    p_i, p_f = H.packed(th, p)
+    F = mkf(th, p_i, p_f)
+    a = _sim(th, p, F, precision="fp32c")
+    b = _sim(th, p, F, precision="fp32c")
+    for s in (a, b):
+        s.set_tile_variants([6, 6, 6, 6])
+    b.set_fused_stages(True, False)
+    try:
+        a.set_fused_stages(False, False, pairs=True)
+    except ValueError:  # the two-field ring does not fit this theory
+        a.close()
+        b.close()
+        pytest.skip(f"no paired sweeps for {name}")
+    E0 = a.compute_energy()
+    Ea, Eb = E0, E0
+    for n in (7, 13):
+        Ea, _, _ = a.steps(Ea, n)
+        Eb, _, _ = b.steps(Eb, n)
+    scale = max(1.0, np.abs(F).max())
+    dF = np.abs(a.get_field() - b.get_field()).max() / scale
+    print(f"[fp32c paired {name}] relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e}")
+    assert H.rel(Ea, Eb) < 1e-7 and dF < 1e-5
+    Ea, _, _ = a.steps(-np.inf, 2)
+    Eb, _, _ = b.steps(-np.inf, 2)
+    dF = np.abs(a.get_field() - b.get_field()).max() / scale
+    dV = np.abs(a.get_velocity() - b.get_velocity()).max() / scale
+    print(f"[fp32c paired {name} arrested] relE={H.rel(Ea, Eb):.2e} max|dF|={dF:.2e} max|dV|={dV:.2e}")
+    assert H.rel(Ea, Eb) < 1e-7 and dF < 1e-5 and dV < 1e-5
+    # the returned energy is the compensated one
+    assert H.rel(Ea, a.compute_energy()) < 1e-12
+    a.close()
+    b.close()
+
+
 def test_sub_ulp_increments_accumulate():
     """dt = 2e-6: every increment of the field per step is below an FP32 ulp of
     its O(1) components, so plain FP32 storage rounds most of them away while

@@ -711,10 +711,6 @@ def test_paired_schedule_refusals():
     with pytest.raises(ValueError):
         s.set_fused_stages(True, False, pairs=True)  # the pairs replace the other sweeps
     s.close()
-    c = _sim(th, p, F, precision="fp32c")
-    with pytest.raises(ValueError):
-        c.set_fused_stages(False, False, pairs=True)
-    c.close()
     th = ssa.load_theory("Bose-Einstein condensate")
     s = ssa.Simulation(H.bec_params(th, 32, 32), th)
     with pytest.raises(ValueError):
