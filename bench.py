@@ -454,9 +454,12 @@ def batch_workload(args):
     tinfo = bs.autotune_tiles(reps=5)
     print(f"[bench] batch tile variant {tinfo['variant']} ({tinfo['rows']} rows), fused 2+3: {tinfo['fused23']}; "
           "ms per variant " + " ".join(f"{x:.3f}" for x in tinfo["ms"]) + "; with 2+3 fused "
-          + " ".join(f"{x:.3f}" for x in tinfo["ms_fused23"]), file=sys.stderr, flush=True)
+          + " ".join(f"{x:.3f}" for x in tinfo["ms_fused23"]) + "; paired sweeps "
+          + " ".join(f"{x:.3f}" for x in tinfo["ms_pairs"]) + f" (chosen: {tinfo['pairs']})",
+          file=sys.stderr, flush=True)
     v = tinfo["variant"]
-    pass_ms = float(tinfo["ms_fused23"][v] if tinfo["fused23"] else tinfo["ms"][v])
+    pass_ms = float(tinfo["ms_pairs"][v] if tinfo["pairs"] else
+                    tinfo["ms_fused23"][v] if tinfo["fused23"] else tinfo["ms"][v])
     bs.use_graphs(20)
     sites = np.array([int(n) * int(n) for n in sizes], dtype=np.float64)
     torch.cuda.synchronize()
@@ -484,7 +487,8 @@ def batch_workload(args):
         "config": {"workload": f"{ninst} baby Skyrme relaxations, sizes U{{64,128,256,512}}^2, seed {args.seed}, "
                                f"until max|EG| < {args.tol:g}", "instances": ninst,
                    "total_sites": int(sites.sum()), "tiles": bs.num_tiles, "tile_rows": tinfo["rows"],
-                   "shape_variant": v, "fused_passes_2_3": bool(tinfo["fused23"])},
+                   "shape_variant": v, "fused_passes_2_3": bool(tinfo["fused23"]),
+                   "paired_sweeps": bool(tinfo.get("pairs", False))},
         "passes_ms_all_active": pass_ms,
         "all_active_updates_per_s": float(sites.sum()) / (pass_ms * 1e-3),
         "time_to_converge_s": wall,

@@ -297,10 +297,12 @@ int ss_batch_memory_bytes(ss_batch* b, size_t* bytes);
  * 1, 2+3 (one sweep, SS_FUSE_23 as for a single lattice), 4 where the variant
  * has the fused sweep, and keeps the fastest variant and schedule; ms[0..8] =
  * mean time of the four passes per variant, ms[9..17] = the same with passes
- * 2+3 fused (0 where the variant has no fused sweep; ms holds 18 entries).
+ * 2+3 fused, ms[18..26] = the paired sweeps 1+2 and 3+4 (SS_FUSE_PAIRS as for
+ * a single lattice; strip variants, kept when 3 % faster than the best other
+ * schedule) (0 where the variant has no such sweep; ms holds 27 entries).
  * The relaxation state is unchanged; results do not depend on the variant or
- * schedule beyond FP64 summation order.  ss_batch_set_fused_stages (flags 0
- * or SS_FUSE_23) fails where the current variant has no fused sweep;
+ * schedule beyond FP64 summation order.  ss_batch_set_fused_stages (flags 0,
+ * SS_FUSE_23 or SS_FUSE_PAIRS) fails where the current variant has no such sweep;
  * ss_batch_get_fused_stages reports whether steps use it. */
 int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms);
 int ss_batch_set_tile_variant(ss_batch* b, int variant);

@@ -106,26 +106,30 @@ class BatchSimulation:
         lattice), and with passes 2+3 as one sweep where the variant has it, on
         the live buffers and keep the fastest; the relaxation state is unchanged."""
         chosen = C.c_int()
-        ms = np.zeros(18)
+        ms = np.zeros(27)
         self._check(_lib.lib().ss_batch_autotune_tiles(self._b, int(reps), C.byref(chosen), ms))
         info = self.tile_variant()
         info["ms"] = ms[:9]
-        info["ms_fused23"] = ms[9:]
+        info["ms_fused23"] = ms[9:18]
+        info["ms_pairs"] = ms[18:]
         return info
 
     def set_tile_variant(self, variant: int) -> None:
         self._check(_lib.lib().ss_batch_set_tile_variant(self._b, int(variant)))
 
-    def set_fused_stages(self, fuse23: bool) -> None:
-        """Run passes 2 and 3 of every instance as one sweep (fails where the
-        current shape variant has no such sweep)."""
-        self._check(_lib.lib().ss_batch_set_fused_stages(self._b, _lib.FUSE_23 if fuse23 else 0))
+    def set_fused_stages(self, fuse23: bool, pairs: bool = False) -> None:
+        """Run passes 2 and 3 of every instance as one sweep, or (``pairs``,
+        alone) a step as the paired sweeps 1+2 and 3+4 (fails where the current
+        shape variant has no such sweep)."""
+        flags = (_lib.FUSE_23 if fuse23 else 0) | (_lib.FUSE_PAIRS if pairs else 0)
+        self._check(_lib.lib().ss_batch_set_fused_stages(self._b, flags))
 
     def tile_variant(self) -> dict:
         v, rows, f = C.c_int(), C.c_int(), C.c_int()
         self._check(_lib.lib().ss_batch_tile_variant(self._b, C.byref(v), C.byref(rows)))
         self._check(_lib.lib().ss_batch_get_fused_stages(self._b, C.byref(f)))
-        return {"variant": v.value, "rows": rows.value, "fused23": bool(f.value & _lib.FUSE_23)}
+        return {"variant": v.value, "rows": rows.value, "fused23": bool(f.value & _lib.FUSE_23),
+                "pairs": bool(f.value & _lib.FUSE_PAIRS)}
 
     def use_graphs(self, steps_per_graph: int):
         self._check(_lib.lib().ss_batch_use_graphs(self._b, int(steps_per_graph)))

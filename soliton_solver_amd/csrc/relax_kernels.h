@@ -611,9 +611,10 @@ struct BatchBufs {
 template <int PASS>
 __device__ __forceinline__ void batch_outputs(PassBufs& b, const BatchBufs& B, long long off, size_t es) {
   b.lout = PASS == 23                ? (char*)B.l[1] + off * es
+           : PASS == 12             ? (char*)B.l[0] + off * es
            : PASS >= 1 && PASS <= 3 ? (char*)B.l[PASS - 1] + off * es
                                     : nullptr;
-  b.lout2 = PASS == 23 ? (char*)B.l[2] + off * es : nullptr;
+  b.lout2 = PASS == 23 ? (char*)B.l[2] + off * es : PASS == 12 ? (char*)B.l[1] + off * es : nullptr;
 }
 
 template <class TH, typename R, int PASS, int TXR = 0>
@@ -676,7 +677,8 @@ __global__ void __launch_bounds__(kThreads) batch_stream_pass(BatchBufs B) {
   b.Vout = (char*)B.V[cur ^ 1] + I.off * es;
   b.part0 = B.part0;
   b.part1 = B.part1;
-  const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
+  const bool kill = (PASS >= 2 && PASS != 12) ? (st.kill != 0) : false;  // 1+2 runs before the decision
+  if (PASS == 34 && kill) b.l2 = b.l1;                                      // l2 of an arrested V
   stream_strip<TH, R, PASS, CW>(q, red, b, I.tp, I.g, sxi * B.sx, B.sx, syi * TYW, kill, t);
 }
 
@@ -984,6 +986,16 @@ struct TheoryLaunchers {
     return fused_tile_fits<TH, R, r0>();
   }
   static int batch_fuse23(int prec, int v) { return prec ? batch_fuse23_r<float>(v) : batch_fuse23_r<double>(v); }
+  // The paired sweeps (batch pass ids 7 = 1+2, 8 = 3+4) in strip variant v.
+  template <typename R>
+  static int batch_pairs_r(int v) {
+    if (v < 3) return 0;
+    if constexpr (wide_fits<R>()) {
+      if (strip_cw(v) == 2) return pairs_fit<R, 2>();
+    }
+    return pairs_fit<R, 1>();
+  }
+  static int batch_pairs(int prec, int v) { return prec ? batch_pairs_r<float>(v) : batch_pairs_r<double>(v); }
 
   static int can_comp(int prec, const Geom& g) {
     return prec && StreamRing<TH, float, 1, 1, true>::shape::kFits && effective_shape<float>(3, g) == 3 &&
@@ -1175,6 +1187,20 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 7:
+        if constexpr (StreamRing<TH, R, 12, CW>::shape::kFits) {
+          hipLaunchKernelGGL((batch_stream_pass<TH, R, 12, CW>), grid, block, 0, st, B);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
+      case 8:
+        if constexpr (StreamRing<TH, R, 34, CW>::shape::kFits) {
+          hipLaunchKernelGGL((batch_stream_pass<TH, R, 34, CW>), grid, block, 0, st, B);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1195,7 +1221,8 @@ struct TheoryLaunchers {
   }
 
   // pass 1..4: fused RK4 passes; 5: energy partials of every instance; 6:
-  // passes 2 and 3 in one sweep (where batch_fuse23).  ``v``: shape variant
+  // passes 2 and 3 in one sweep (where batch_fuse23); 7, 8: the paired sweeps
+  // 1+2 and 3+4 (strips, where batch_pairs).  ``v``: shape variant
   // 0..8 (the tile table of the batch must be built for it).
   static hipError_t batch_pass(int prec, int pass, int v, const BatchBufs& B, hipStream_t st) {
     if (v >= 3) return prec ? batch_stream_v<float>(pass, v, B, st) : batch_stream_v<double>(pass, v, B, st);
@@ -1242,6 +1269,7 @@ struct TheoryOps {
   int (*fuse41_variant)(int, int, const Geom&);
   int (*pairs_variant)(int, int, const Geom&);
   int (*batch_fuse23)(int, int);
+  int (*batch_pairs)(int, int);
   int (*aux_components)(int);
   unsigned (*init_modes)();
   hipError_t (*init)(int, const InitArgs&, void*, const Geom&, hipStream_t);
@@ -1272,6 +1300,7 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::fuse41_variant,             \
                                       &ss::TheoryLaunchers<TH>::pairs_variant,              \
                                       &ss::TheoryLaunchers<TH>::batch_fuse23,               \
+                                      &ss::TheoryLaunchers<TH>::batch_pairs,                \
                                       &ss::TheoryLaunchers<TH>::aux_components,             \
                                       &ss::TheoryLaunchers<TH>::init_modes,                 \
                                       &ss::TheoryLaunchers<TH>::init,                       \

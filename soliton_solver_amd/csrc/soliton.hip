@@ -1692,6 +1692,7 @@ struct ss_batch {
   int ntcap = 0;        // capacity of the tile table and of each partials row (the most tiles over the variants)
   int tv = 0;           // shape variant (0..2 LDS tiles, 3..8 streaming strips, as for a single lattice)
   int fuse23 = 0;       // passes 2 and 3 in one sweep where the variant has it
+  int pairs = 0;        // the step as the paired sweeps 1+2 and 3+4 where the variant has them
   int sx = 0;           // streaming variants: rows per strip
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
@@ -1776,9 +1777,19 @@ static BatchBufs batch_bufs(ss_batch* b) {
 }
 
 static bool batch_use23(const ss_batch* b) { return b->fuse23 && b->ops->batch_fuse23(b->prec, b->tv); }
+static bool batch_use_pairs(const ss_batch* b) { return b->pairs && b->ops->batch_pairs(b->prec, b->tv); }
 
+// Paired batch step: passes 1+2 of every active instance (V before its arrest
+// decision) -> the decisions -> passes 3+4 (l2 = l1 where arrested) -> err.
 static int batch_launch_step(ss_batch* b, double tol) {
   const BatchBufs B = batch_bufs(b);
+  if (batch_use_pairs(b)) {
+    BHIP(b, b->ops->batch_pass(b->prec, 7, b->tv, B, b->stream));
+    BHIP(b, b->ops->batch_fin(0, b->ninst, B, 0, 0.0, b->stream));
+    BHIP(b, b->ops->batch_pass(b->prec, 8, b->tv, B, b->stream));
+    BHIP(b, b->ops->batch_fin(1, b->ninst, B, 0, tol, b->stream));
+    return 0;
+  }
   BHIP(b, b->ops->batch_pass(b->prec, 1, b->tv, B, b->stream));
   BHIP(b, b->ops->batch_fin(0, b->ninst, B, 0, 0.0, b->stream));
   if (batch_use23(b)) {
@@ -2056,13 +2067,14 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
     rc = batch_tiles(b, v);
     if (rc) break;
     const BatchBufs B = batch_bufs(b);
-    // f = 0: passes 1, 2, 3, 4; f = 1: passes 1, 2+3, 4 (where the variant has the sweep)
-    for (int f = 0; f < 2 && rc == 0; ++f) {
-      if (f == 1 && !b->ops->batch_fuse23(b->prec, v)) {
-        if (ms) ms[nv + v] = 0.0;
+    // f = 0: passes 1, 2, 3, 4; f = 1: passes 1, 2+3, 4 (where the variant has the sweep);
+    // f = 2: the paired sweeps 1+2, 3+4 (strip variants)
+    for (int f = 0; f < 3 && rc == 0; ++f) {
+      if ((f == 1 && !b->ops->batch_fuse23(b->prec, v)) || (f == 2 && !b->ops->batch_pairs(b->prec, v))) {
+        if (ms) ms[f * nv + v] = 0.0;
         continue;
       }
-      const int seq[2][4] = {{1, 2, 3, 4}, {1, 6, 4, 0}};
+      const int seq[3][4] = {{1, 2, 3, 4}, {1, 6, 4, 0}, {7, 8, 0, 0}};
       hipError_t e = hipSuccess;
       for (int k = 0; k < 4 && seq[f][k] && e == hipSuccess; ++k)  // warm-up
         e = b->ops->batch_pass(b->prec, seq[f][k], v, B, b->stream);
@@ -2081,7 +2093,8 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
       }
       t /= reps;
       if (ms) ms[f * nv + v] = t;
-      if ((v == 0 && f == 0) || t < best) {
+      // a paired schedule must win by 3 % (its passes 1+2 read V before the decision)
+      if ((v == 0 && f == 0) || (f == 2 ? t < 0.97f * best : t < best)) {
         best = t;
         best_v = v;
         best_f = f;
@@ -2092,7 +2105,8 @@ int ss_batch_autotune_tiles(ss_batch* b, int reps, int* chosen, double* ms) {
   hipEventDestroy(e1);
   if (rc) return rc;
   if (batch_tiles(b, best_v)) return -2;
-  b->fuse23 = best_f;
+  b->fuse23 = best_f == 1;
+  b->pairs = best_f == 2;
   if (chosen) *chosen = best_v;
   return 0;
 }
@@ -2106,11 +2120,17 @@ int ss_batch_set_tile_variant(ss_batch* b, int variant) {
 
 int ss_batch_set_fused_stages(ss_batch* b, int flags) {
   if (!b) return bfail(nullptr, -1, "null batch");
-  if (flags & ~SS_FUSE_23) return bfail(b, -1, "batches fuse passes 2+3 only (flags 0x%x)", flags);
+  if (flags & ~(SS_FUSE_23 | SS_FUSE_PAIRS)) return bfail(b, -1, "unknown batch fused-stage flags 0x%x", flags);
+  if ((flags & SS_FUSE_PAIRS) && (flags & SS_FUSE_23))
+    return bfail(b, -1, "the paired schedule replaces the 2+3 sweep: set it alone");
+  if ((flags & SS_FUSE_PAIRS) && !b->ops->batch_pairs(b->prec, b->tv))
+    return bfail(b, -1, "passes 1+2 and 3+4 of theory %s cannot run as paired sweeps in shape variant %d",
+                 b->ops->name, b->tv);
   if ((flags & SS_FUSE_23) && !b->ops->batch_fuse23(b->prec, b->tv))
     return bfail(b, -1, "passes 2 and 3 of theory %s cannot share one sweep in shape variant %d at this precision",
                  b->ops->name, b->tv);
   b->fuse23 = (flags & SS_FUSE_23) ? 1 : 0;
+  b->pairs = (flags & SS_FUSE_PAIRS) ? 1 : 0;
   if (b->graph) {
     hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
@@ -2120,7 +2140,7 @@ int ss_batch_set_fused_stages(ss_batch* b, int flags) {
 
 int ss_batch_get_fused_stages(ss_batch* b, int* flags) {
   if (!b) return bfail(nullptr, -1, "null batch");
-  if (flags) *flags = batch_use23(b) ? SS_FUSE_23 : 0;
+  if (flags) *flags = (batch_use23(b) ? SS_FUSE_23 : 0) | (batch_use_pairs(b) ? SS_FUSE_PAIRS : 0);
   return 0;
 }
 

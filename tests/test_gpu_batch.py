@@ -159,6 +159,7 @@ def test_batch_autotune_keeps_state_and_trajectory():
     info = a.autotune_tiles(reps=3)
     assert 0 <= info["variant"] < 9 and len(info["ms"]) == 9 and np.all(info["ms"] > 0)
     assert len(info["ms_fused23"]) == 9 and np.all(info["ms_fused23"][3:] > 0)  # every strip has the 2+3 sweep
+    assert len(info["ms_pairs"]) == 9 and np.all(info["ms_pairs"][:3] == 0) and np.all(info["ms_pairs"][3:] > 0)
     for i in range(len(params)):
         assert np.array_equal(a.get_field(i), F0[i])
     a.run(4, tol=0.0)
@@ -175,6 +176,44 @@ def test_batch_autotune_keeps_state_and_trajectory():
         assert sa["steps"][i] == sb["steps"][i] == 45
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_pairs_match_separate_passes(precision):
+    """The paired sweeps 1+2 and 3+4 over every instance (strips; passes 1+2
+    read V before each instance's arrest decision, passes 3+4 take l2 = l1
+    where it arrests) against the separate passes: the same energies, fields
+    and step counts after 12 steps; tile variants have no paired sweeps."""
+    th, params, cfgs = _bs_instances([48, 100, 64, 130], seed=11)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for v in (3, 6):
+        a = BatchSimulation(th, params, precision=precision)
+        b = BatchSimulation(th, params, precision=precision)
+        a.initialize(cfgs)
+        b.initialize(cfgs)
+        a.set_tile_variant(v)
+        b.set_tile_variant(v)
+        a.set_fused_stages(False, pairs=True)
+        assert a.tile_variant()["pairs"] and not b.tile_variant()["pairs"]
+        a.run(12, tol=0.0)
+        b.run(12, tol=0.0)
+        sa, sb = a.state(), b.state()
+        for i in range(len(params)):
+            dF = np.abs(a.get_field(i) - b.get_field(i)).max()
+            print(f"[batch pairs {precision} v={v} inst {i}] relE="
+                  f"{abs(sa['energy'][i] - sb['energy'][i]) / abs(sb['energy'][i]):.2e} max|dF|={dF:.2e}")
+            assert abs(sa["energy"][i] - sb["energy"][i]) <= tol * abs(sb["energy"][i])
+            assert dF <= tol
+            assert sa["steps"][i] == sb["steps"][i] == 12
+        a.close()
+        b.close()
+    with pytest.raises(ValueError):
+        c = BatchSimulation(th, params, precision=precision)
+        try:
+            c.set_tile_variant(1)
+            c.set_fused_stages(False, pairs=True)  # tiles: no paired sweeps
+        finally:
+            c.close()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
