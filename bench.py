@@ -8,11 +8,19 @@ radii uniform in [0.3, 0.6]) plus seeded Gaussian noise sigma = 1e-3,
 renormalised.  One step = one arrested-Newton RK4 step over every site.
 
 The headline ``value`` is grid-point updates/s of the FP32-stencil /
-FP64-reduction form; the plain FP64 form is timed beside it on the same
-initial field.  ``roofline`` prices the dominant kernel (the RK4 pass 4
-launch, 7 nf w bytes per site; with the fused 4+1 sweep that launch, 9 nf w
-bytes per site; with the paired schedule the 3+4 sweep, 6 nf w bytes per site)
-from HIP events on the context's stream.
+FP64-reduction form that converges to the FP64 minimum: the compensated form
+(FP32 fields and stencils, the field held as FP32 hi + lo words so that
+sub-ulp increments accumulate, FP64 energy of hi + lo, FP64 sums and arrest
+test).  The plain FP32 form and the plain FP64 form are timed beside it on the
+same initial field; the plain FP32 form alone does not reach Delta E / E <=
+1e-7 without a long FP64 tail.  ``hbm_frac_alg`` prices every form at the
+fixed algorithmic 16 nf w bytes per update (w = 4 for FP32 fields, 8 for
+FP64); ``hbm_frac_schedule`` at the words the chosen schedule actually moves.
+``roofline`` prices the dominant kernel (the launch that finishes a step:
+pass 4, the 4+1 sweep or the 3+4 sweep, at the words it reads and writes)
+from HIP events on the context's stream.  ``time_to_converge`` relaxes one
+4096^2 anti-skyrmion (demagnetisation off) to max |dE/dphi| < 1e-6 in the
+compensated form + FP64 polish and in the plain FP64 form.
 ``cpu_baseline`` times the oracle's C++/OpenMP FP64 restatement of the
 reference path on a bounded sample of the same workload.
 """
@@ -67,6 +75,8 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
         sim.use_graphs(graph_steps)
     E = sim.compute_energy()
     E, _, _ = sim.steps(E, warmup)
+    # the graphs the timed call replays are captured and uploaded here, not in the timed region
+    sim.prepare_graphs(steps)
     sim.synchronize()
     import torch
     torch.cuda.synchronize()
@@ -416,6 +426,75 @@ def measure_traffic(child_args, kernel):
             "read_bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024, "write_bytes": kib["WRITE_SIZE"] * WRITE_SCALE * 1024}
 
 
+def schedule(tiles, comp):
+    """Words per site and component of the launch that finishes a step (the
+    dominant one, timed as the pass-4 slot of ``time_passes``) and of a whole
+    step, for the schedule ``tiles`` describes (the dict of
+    ``Simulation.tile_variants``); ``comp``: the compensated FP32 form, whose
+    field carries a second word (lo) that its pass 1 reads and its combine
+    reads and writes."""
+    pairs, f41, f23 = bool(tiles["pairs"]), bool(tiles["fused41"]), bool(tiles["fused23"])
+    v4 = tiles["variants"][3]
+    cw = 2 if v4 >= 6 else 1
+    if comp:
+        if pairs:  # pass 1 (F, lo -> l1) + pass 2 (F, V -> l2) + 3+4 (F, lo, V, l1, l2 -> F', lo', V')
+            return {"name": "compensated pass 1, pass 2, paired 3+4", "dominant_words": 8, "step_words": 14,
+                    "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
+        kern = (f"stream_pass_comp<ChiralMagnet, float, 4, {cw}>" if v4 >= 3 else
+                f"stage_pass<ChiralMagnet, float, 4, {tiles['rows'][3]}, true>")
+        return {"name": "compensated pass 1, " + ("2+3, " if f23 else "2, 3, ") + "4",
+                "dominant_words": 9, "step_words": 3 + (5 if f23 else 7) + 9, "kernel": kern}
+    if pairs:  # 1+2 (F, V -> l1, l2) + 3+4 (F, V, l1, l2 -> F', V')
+        return {"name": "paired 1+2, 3+4", "dominant_words": 6, "step_words": 10,
+                "kernel": f"stream_pass<ChiralMagnet, float, 34, {cw}>"}
+    if f41:  # 2+3 or 2, 3, then 4+1 (F, V, l1, l2, l3 -> F', V', l1')
+        return {"name": ("2+3, " if f23 else "2, 3, ") + "4+1", "dominant_words": 9,
+                "step_words": (5 if f23 else 7) + 9, "kernel": f"stream_pass<ChiralMagnet, float, 41, {cw}>"}
+    kern = (f"stream_pass<ChiralMagnet, float, 4, {cw}>" if v4 >= 3 else
+            f"stage_pass<ChiralMagnet, float, 4, {tiles['rows'][3]}>")
+    return {"name": "1, " + ("2+3, " if f23 else "2, 3, ") + "4", "dominant_words": 7,
+            "step_words": 2 + (5 if f23 else 7) + 7, "kernel": kern}
+
+
+def cpu_baseline(args, n):
+    """The oracle's C++/OpenMP FP64 restatement of the reference path (unfused,
+    derivative scratch) on a bounded sample of the chiral workload, plus the
+    single-thread NumPy oracle on the first steps of the two small workloads."""
+    import soliton_solver_amd as ssa
+    from oracle import cpu_build
+    m = args.cpu_sample
+    thc, pc, cfgc = chiral_workload(m, seed=3, nsk=max(1, round(8 * (m / n) ** 2)))
+    simc = ssa.Simulation(pc, thc)
+    simc.initialize(cfgc)
+    Fc = simc.get_field().copy()
+    Vc = np.zeros_like(Fc)
+    p_i, p_f = simc.p_i_h, simc.p_f_h
+    simc.close()
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    t0 = time.perf_counter()
+    cpu_build.chiral_relax(p_i, p_f, Fc, Vc, np.inf, args.cpu_steps, nthreads=threads)
+    tc = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cpu = {"value": m * m * args.cpu_steps / tc, "unit": "grid-point updates/s", "cores": threads,
+           "kind": "port",
+           "sample": f"C++/OpenMP FP64 restatement of the reference path (unfused, derivative scratch), "
+                     f"chiral magnet {m}x{m} same parameters, {args.cpu_steps} steps, {tc:.1f} s; "
+                     f"host {model or platform.processor()}, os.cpu_count()={os.cpu_count()}, "
+                     f"{threads} threads (the process's CPU affinity)"}
+    gl_steps, cm_steps = (int(v) for v in args.numpy_steps.split(","))
+    cpu["numpy_oracle_single_thread"] = numpy_oracle_rates(gl_steps, cm_steps)
+    print(f"[bench] numpy oracle: {cpu['numpy_oracle_single_thread']}", file=sys.stderr, flush=True)
+    return cpu
+
+
 def _chunk(sim, E, nsteps):
     """``nsteps`` steps in one timed call; returns (E, err, err_absmax, steps, seconds)."""
     import torch
@@ -522,9 +601,11 @@ def main():
     ap.add_argument("--pairs", type=int, default=-1,
                     help="with --tiles: 1 runs a step as the two paired sweeps, passes 1+2 and 3+4")
     ap.add_argument("--profile-only", action="store_true", help="FP32 form only, no CPU leg (for rocprof)")
-    ap.add_argument("--reduced-form", choices=["fp32", "fp32c"], default="fp32",
-                    help="throughput workload: the FP32 form whose rate is the headline value (fp32c: with the "
-                         "compensated field update)")
+    ap.add_argument("--reduced-form", choices=["fp32", "fp32c"], default="fp32c",
+                    help="throughput workload: the FP32 form whose rate is the headline value (fp32c, the "
+                         "default: with the compensated field update, the form that converges; fp32: plain)")
+    ap.add_argument("--ttc-n", type=int, default=4096,
+                    help="throughput workload: lattice side of the time-to-converge run (one anti-skyrmion)")
     ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge"], default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
                          "batch: many baby-Skyrme relaxations of uneven sizes; converge: time-to-converge of the "
@@ -535,7 +616,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
     ap.add_argument("--no-precision", action="store_true",
-                    help="throughput workload: skip the 1024^2 time-to-converge check of the FP32C form")
+                    help="throughput workload: skip the time-to-converge run of the FP32C form")
     ap.add_argument("--converge-cap", type=int, default=0,
                     help="converge workload: stop each case after at most this many steps (0: run to the tolerance)")
     ap.add_argument("--start", choices=["fp64", "reduced"], default="reduced",
@@ -600,49 +681,34 @@ def main():
     print(f"[bench] building {n}x{n} chiral workload", file=sys.stderr, flush=True)
     form = args.reduced_form
     comp = form == "fp32c"
+    nf, w, N = 4, 4, n * n
+    tiles_arg = [int(v) for v in args.tiles.split(",")] if args.tiles else None
     sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, form, args.steps, args.warmup, args.graph_steps,
-                                             autotune=not args.no_autotune,
-                                             tiles=[int(v) for v in args.tiles.split(",")] if args.tiles else None,
+                                             autotune=not args.no_autotune, tiles=tiles_arg,
                                              fused23=args.fused23, fused41=args.fused41, pairs=args.pairs)
     tiles32 = sim32.tiles
-    print(f"[bench] fp32 tile rows per pass {tiles32['rows']}", file=sys.stderr, flush=True)
+    print(f"[bench] {form} tile rows per pass {tiles32['rows']}", file=sys.stderr, flush=True)
     if "ms" in tiles32:
         for k, row in enumerate(tiles32["ms"]):
             print(f"[bench]   pass {k + 1} ms by shape variant: " + " ".join(f"{x:.4f}" for x in row),
                   file=sys.stderr, flush=True)
-        print(f"[bench]   fused passes 2+3 ms by shape variant: "
-              + " ".join(f"{x:.4f}" for x in tiles32["ms_fused23"]) + f"; fused: {tiles32['fused23']}",
-              file=sys.stderr, flush=True)
-        print(f"[bench]   fused passes 4+1 ms by shape variant: "
-              + " ".join(f"{x:.4f}" for x in tiles32["ms_fused41"]) + f"; fused: {tiles32['fused41']}",
-              file=sys.stderr, flush=True)
-        for key, label in (("ms_pairs12", "1+2"), ("ms_pairs34", "3+4")):
-            print(f"[bench]   paired sweep {label} ms by shape variant: "
-                  + " ".join(f"{x:.4f}" for x in tiles32[key]) + f"; paired: {tiles32['pairs']}",
+        for key, label in (("ms_fused23", "fused passes 2+3"), ("ms_fused41", "fused passes 4+1"),
+                           ("ms_pairs12", "paired sweep 1+2"), ("ms_pairs34", "paired sweep 3+4")):
+            print(f"[bench]   {label} ms by shape variant: " + " ".join(f"{x:.4f}" for x in tiles32[key]),
                   file=sys.stderr, flush=True)
-    F0 = None
-    N = n * n
-    nf = 4
+        print(f"[bench]   chosen: fused23 {tiles32['fused23']}, fused41 {tiles32['fused41']}, "
+              f"paired {tiles32['pairs']}", file=sys.stderr, flush=True)
     ups32 = N * args.steps / t32
     print(f"[bench] {form} form: {t32 / args.steps * 1e3:.3f} ms/step, {ups32:.3e} updates/s", file=sys.stderr,
           flush=True)
 
-    # dominant-kernel roofline from HIP events on the context stream; with the
-    # 4+1 sweep the dominant launch is that sweep (pass 4's 7 words per site
-    # plus pass 1's 2) and pass 1 runs only in the first of the timed steps.
-    # The compensated form's pass 4 also reads and writes the low words (9
-    # words per site), its pass 1 reads them (3): 19 words per update.
-    # With the paired schedule the dominant launch is the 3+4 sweep (F, V, l1,
-    # l2 in; F', V' out: 6 words) and a step moves 10 words per update.
+    # dominant-kernel roofline from HIP events on the context stream
     passes = sim32.time_passes(E32, 20)
-    w = 4
-    f41 = bool(tiles32.get("fused41", False))
-    pairs = bool(tiles32.get("pairs", False))
-    alg_pass4 = (6 if pairs else 9 if (f41 or comp) else 7) * nf * w * N
-    pass4_gbs = alg_pass4 / (passes[4] * 1e-3) / 1e9
+    sched = schedule(tiles32, comp)
+    alg_dom = sched["dominant_words"] * nf * w * N
+    dom_gbs = alg_dom / (passes[4] * 1e-3) / 1e9
     step_ms_kernels = float(passes.sum())
-    words_step = 19 if comp else 10 if pairs else 16
-    alg_step = words_step * nf * w * N
+    alg_step = sched["step_words"] * nf * w * N
     step_gbs = alg_step / (step_ms_kernels * 1e-3) / 1e9
     sim32.close()
 
@@ -654,102 +720,74 @@ def main():
         ups64 = N * args.steps / t64
         relE = abs(E32 - E64) / abs(E64)
         sim64.close()
+        s64 = schedule(tiles64, False)
         result64 = {"updates_per_s": ups64, "ms_per_step": t64 / args.steps * 1e3,
                     "hbm_frac_alg": ups64 * 16 * nf * 8 / (HBM_PEAK_GBS * 1e9),
-                    "energy": E64, "fp32_vs_fp64_relE_after_steps": relE,
+                    "hbm_frac_schedule": ups64 * s64["step_words"] * nf * 8 / (HBM_PEAK_GBS * 1e9),
+                    "words_per_update": s64["step_words"],
+                    "energy": E64, f"{form}_vs_fp64_relE_after_steps": relE,
                     "relE_note": "both forms from the same field after the same timed steps, mid-relaxation; "
-                                 "relaxed energies agree within 4e-9 (bench.py --workload precision)",
+                                 "relaxed energies: time_to_converge",
                     "tile_rows_per_pass": tiles64["rows"]}
         print(f"[bench] fp64 form: {t64 / args.steps * 1e3:.3f} ms/step, {ups64:.3e} updates/s, "
-              f"relE(fp32 vs fp64)={relE:.2e}", file=sys.stderr, flush=True)
+              f"relE({form} vs fp64)={relE:.2e}", file=sys.stderr, flush=True)
 
-    resultc = None
-    if not args.profile_only and not comp:
-        simc_, Ecc, _, _, tcc = time_form(th, p, cfg, "fp32c", args.steps, args.warmup, args.graph_steps,
-                                          autotune=not args.no_autotune)
-        upsc = N * args.steps / tcc
-        resultc = {"updates_per_s": upsc, "ms_per_step": tcc / args.steps * 1e3,
-                   "hbm_frac_alg": upsc * 19 * nf * 4 / (HBM_PEAK_GBS * 1e9),
-                   "speedup_vs_fp64": (upsc / result64["updates_per_s"]) if result64 else None,
-                   "tile_rows_per_pass": simc_.tiles["rows"],
-                   "note": "compensated field update (FP32 hi + lo words, FP64 combine and FP64 energy of hi + lo),"
-                           " 19 words per update; the form the precision workload converges with"}
-        simc_.close()
-        print(f"[bench] fp32c form: {tcc / args.steps * 1e3:.3f} ms/step, {upsc:.3e} updates/s",
+    other = None
+    if not args.profile_only:
+        oform = "fp32" if comp else "fp32c"
+        simo, Eo, _, _, to = time_form(th, p, cfg, oform, args.steps, args.warmup, args.graph_steps,
+                                       autotune=not args.no_autotune)
+        upso = N * args.steps / to
+        so = schedule(simo.tiles, oform == "fp32c")
+        other = {"form": oform, "updates_per_s": upso, "ms_per_step": to / args.steps * 1e3,
+                 "hbm_frac_alg": upso * 16 * nf * 4 / (HBM_PEAK_GBS * 1e9),
+                 "hbm_frac_schedule": upso * so["step_words"] * nf * 4 / (HBM_PEAK_GBS * 1e9),
+                 "words_per_update": so["step_words"],
+                 "speedup_vs_fp64": (upso / result64["updates_per_s"]) if result64 else None,
+                 "tile_rows_per_pass": simo.tiles["rows"],
+                 "note": ("plain FP32 storage: its steps round away increments below an FP32 ulp, so alone it "
+                          "stalls above the FP64 minimum and does not reach Delta E / E <= 1e-7 without a long "
+                          "FP64 tail" if oform == "fp32" else
+                          "compensated field update (FP32 hi + lo words), the form that converges")}
+        simo.close()
+        print(f"[bench] {oform} form: {to / args.steps * 1e3:.3f} ms/step, {upso:.3e} updates/s",
               file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and not args.profile_only:
-        from oracle import cpu_build
-        m = args.cpu_sample
-        thc, pc, cfgc = chiral_workload(m, seed=3, nsk=max(1, round(8 * (m / n) ** 2)))
-        simc = ssa.Simulation(pc, thc)
-        simc.initialize(cfgc)
-        Fc = simc.get_field().copy()
-        Vc = np.zeros_like(Fc)
-        p_i, p_f = simc.p_i_h, simc.p_f_h
-        simc.close()
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(threads, 16))
-        t0 = time.perf_counter()
-        cpu_build.chiral_relax(p_i, p_f, Fc, Vc, np.inf, args.cpu_steps, nthreads=threads)
-        tc = time.perf_counter() - t0
-        model = ""
-        try:
-            with open("/proc/cpuinfo") as fh:
-                for line in fh:
-                    if line.startswith("model name"):
-                        model = line.split(":", 1)[1].strip()
-                        break
-        except OSError:
-            pass
-        cpu = {"value": m * m * args.cpu_steps / tc, "unit": "grid-point updates/s", "cores": threads,
-               "kind": "port",
-               "sample": f"C++/OpenMP FP64 restatement of the reference path (unfused, derivative scratch), "
-                         f"chiral magnet {m}x{m} same parameters, {args.cpu_steps} steps, {tc:.1f} s; "
-                         f"host {model or platform.processor()}, os.cpu_count()={os.cpu_count()}"}
-        gl_steps, cm_steps = (int(v) for v in args.numpy_steps.split(","))
-        cpu["numpy_oracle_single_thread"] = numpy_oracle_rates(gl_steps, cm_steps)
-        print(f"[bench] numpy oracle: {cpu['numpy_oracle_single_thread']}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args, n)
 
     # time to converge of the compensated FP32 form against the FP64 form, to
-    # max|EG| < 1e-6 from one seeded field: the precision workload at 1024^2
-    # (the 4096^2 run, ~2.5 min, is `--workload precision --n 4096`)
+    # max|EG| < 1e-6 from one seeded field: one anti-skyrmion at 4096^2
+    # (demagnetisation off; with it on, the saddle field psi keeps the flow
+    # killing its velocity nearly every step and no form converges in a bounded run)
     ttc = None
     if not args.no_precision and not args.profile_only:
         import copy
         pa = copy.copy(args)
-        pa.n, pa.nsk, pa.demag, pa.tol, pa.form, pa.seed = 1024, 1, 0, 1e-6, "fp32c", 4
+        pa.n, pa.nsk, pa.demag, pa.tol, pa.form, pa.seed = args.ttc_n, 1, 0, 1e-6, "fp32c", 4
         pr = precision_workload(pa, emit=False)
         ttc = {"workload": pr["config"]["workload"], "speedup_vs_fp64": pr["value"], "converged": pr["converged"],
                "fp64_seconds": pr["fp64_form"]["seconds"], "fp64_steps": pr["fp64_form"]["steps"],
                "fp32c_seconds": pr["fp32_phase"]["seconds"] + pr["fp64_polish"]["seconds"],
                "fp32c_steps": pr["fp32_phase"]["steps"], "fp64_polish_steps": pr["fp64_polish"]["steps_total"],
-               "relE_vs_fp64_form": pr["relE_vs_fp64_form"],
+               "relE_vs_fp64_form": pr["relE_vs_fp64_form"], "max_abs_dm": pr["max_dm_vs_fp64_form"],
+               "skyrmion_number": pr["skyrmion_number"],
                "skyrmion_number_identical": pr["skyrmion_number"]["identical_rounded"]}
-        print(f"[bench] time to converge (1024^2, fp32c vs fp64): {ttc['speedup_vs_fp64']:.2f}x, "
+        print(f"[bench] time to converge ({args.ttc_n}^2, fp32c vs fp64): {ttc['speedup_vs_fp64']:.2f}x, "
               f"relE {ttc['relE_vs_fp64_form']:.1e}", file=sys.stderr, flush=True)
 
-    v4 = tiles32["variants"][3]
-    if f41 and v4 < 3:
-        v4 = 3
-    if comp:
-        kernel = (f"stream_pass_comp<ChiralMagnet, float, 4, {2 if v4 >= 6 else 1}>" if v4 >= 3 else
-                  f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}, true>")
-    else:
-        kernel = (f"stream_pass<ChiralMagnet, float, {34 if pairs else 41 if f41 else 4}, {2 if v4 >= 6 else 1}>"
-                  if v4 >= 3 else
-                  f"stage_pass<ChiralMagnet, float, 4, {tiles32['rows'][3]}>")
+    kernel = sched["kernel"]
     traffic = None
     if not args.no_traffic and not args.profile_only:
         child = ["--tiles", ",".join(str(v) for v in tiles32["variants"]),
-                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(f41)),
-                 "--pairs", str(int(pairs)), "--n", str(n),
+                 "--fused23", str(int(tiles32.get("fused23", False))), "--fused41", str(int(tiles32["fused41"])),
+                 "--pairs", str(int(tiles32["pairs"])), "--n", str(n),
                  "--reduced-form", form]
         traffic = measure_traffic(child, kernel)
         if traffic:
             print(f"[bench] traffic of {kernel}: {traffic['bytes'] / 1e9:.3f} GB per launch "
-                  f"({traffic['bytes'] / alg_pass4:.2f}x algorithmic)", file=sys.stderr, flush=True)
+                  f"({traffic['bytes'] / alg_dom:.2f}x algorithmic)", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -764,29 +802,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "form": form,
         "tile_variants": tiles32["variants"],
         "fused23": int(tiles32.get("fused23", False)),
-        "fused41": int(f41),
-        "paired_sweeps": int(pairs),
+        "fused41": int(tiles32["fused41"]),
+        "paired_sweeps": int(tiles32["pairs"]),
         "config": {"workload": f"chiral magnet {n}x{n}, h=10/319, Heusler DMI, demag, 8 seeded anti-skyrmions; "
                                "FP32 fields+stencils, FP64 energy sums and arrest test"
                                + ("; compensated field update (FP32 hi + lo)" if comp else ""),
                    "lattice": [n, n], "fields": nf, "graph_steps": args.graph_steps,
                    "tile_rows_per_pass": tiles32["rows"], "tile_shape": "64 y x rows x",
-                   "fused_passes_2_3": tiles32.get("fused23", False),
-                   "fused_passes_4_1": f41,
-                   "paired_sweeps_1_2_and_3_4": pairs, "words_per_update": words_step,
+                   "schedule": sched["name"], "words_per_update": sched["step_words"],
                    "parallelism": "single GPU"},
-        "hbm_frac_alg": ups32 * words_step * nf * w / (HBM_PEAK_GBS * 1e9),
-        "roofline": {"bound": "hbm", "achieved": pass4_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": pass4_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
+        "hbm_frac_alg": ups32 * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
+        "hbm_frac_alg_note": "updates/s x 16 nf w bytes (the fixed algorithmic bytes of one update) / 8 TB/s",
+        "hbm_frac_schedule": ups32 * sched["step_words"] * nf * w / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "traffic_read_write": [traffic["read_bytes"], traffic["write_bytes"]] if traffic else None,
-                     "kernel": kernel + (f" (strips of {tiles32['rows'][3]} rows)" if v4 >= 3 else ""),
-                     "alg_bytes_per_launch": alg_pass4, "avg_launch_ms": passes[4],
-                     "step_kernels_gbs": step_gbs, "pass_ms": [float(x) for x in passes]},
+                     "kernel": kernel, "alg_words_per_site": sched["dominant_words"] * nf,
+                     "alg_bytes_per_launch": alg_dom, "avg_launch_ms": passes[4],
+                     "step_kernels_ms": step_ms_kernels, "step_kernels_gbs": step_gbs,
+                     "pass_ms": [float(x) for x in passes]},
         "fp64_form": result64,
-        "speedup_fp32_vs_fp64": (ups32 / result64["updates_per_s"]) if result64 else None,
-        "fp32c_form": resultc,
+        "speedup_vs_fp64": (ups32 / result64["updates_per_s"]) if result64 else None,
+        ("fp32_form" if comp else "fp32c_form"): other,
         "time_to_converge": ttc,
         "cpu_baseline": cpu,
     }

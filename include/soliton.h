@@ -94,7 +94,12 @@ int ss_download(ss_ctx* ctx, int buf, double* host, size_t n);
  * previous step's (the first with prev_energy) using strict '>', and kills the
  * velocity when killkinen is set.  Returns the last step's energy, the signed
  * max of its stage-4 energy gradient (the reference's err, utils.py:397-420)
- * and max |stage-4 energy gradient|.  Any output pointer may be NULL. */
+ * and max |stage-4 energy gradient|.  Any output pointer may be NULL.
+ * Non-finite values are not dropped: a NaN or infinite energy gradient at any
+ * site makes err_abs_max NaN or +inf (the error reductions propagate NaN, where
+ * the reference's tree of '<' comparisons and numpy max usually do), so a
+ * blown-up relaxation never compares below a tolerance; a non-finite field
+ * also gives a non-finite e_new. */
 int ss_step(ss_ctx* ctx, double prev_energy, int nsteps, double* e_new, double* err_signed_max,
             double* err_abs_max);
 
@@ -105,7 +110,16 @@ int ss_step_history(ss_ctx* ctx, double* energies, double* errs, int n);
 int ss_energy(ss_ctx* ctx, double* e);
 
 /* Charges of the theory: kind indexes the theory's charge list (chiral magnet:
- * 0 skyrmion number, 1 magnetic charge; Ginzburg-Landau: 0 vortex number ...). */
+ * 0 skyrmion number, 1 magnetic charge; Ginzburg-Landau: 0 vortex number ...).
+ * The reference's compute_vortex_number(..., which, ...) has no ``which``
+ * here: each flux component it selects is a charge kind of its own.
+ * Ferromagnetic superconductor (ferromagnetic_superconductor/kernels.py:291-367)
+ * and spin-triplet superconducting ferromagnet
+ * (spin_triplet_superconducting_magnet/kernels.py:301-377): kind 0 skyrmion
+ * number, kind 1 = which 2 (d_x A2 - d_y A1), kind 2 = which 0 (d_y A3),
+ * kind 3 = which 1 (-d_x A3).  Ginzburg-Landau, anisotropic superconductor and
+ * Maxwell-Chern-Simons-Higgs ignore ``which`` in the reference (one flux
+ * component): their vortex number is kind 0. */
 int ss_charge(ss_ctx* ctx, int kind, double* q);
 int ss_num_charges(ss_ctx* ctx);
 
@@ -162,10 +176,13 @@ int ss_aux_density(ss_ctx* ctx, int kind, int comp, double* host_out, size_t n);
 int ss_variation(ss_ctx* ctx, int mode, double* host_eg, size_t n);
 
 /* Device-pointer variation for the autograd boundary: field_dev and grad_dev
- * hold nf*xlen*ylen values of the context's precision in reference layout;
- * runs on ``stream`` (a hipStream_t; NULL is the null stream, as for any HIP
- * call).  These calls share the context's scratch and partial buffers: they
- * must not overlap with each other or with ss_step on the same context. */
+ * hold nf*xlen*ylen FP64 values in reference layout (the context must be in
+ * SS_PREC_FP64; other precisions fail); runs on ``stream`` (a hipStream_t;
+ * NULL is the null stream, as for any HIP call).  These calls use the
+ * context's scratch and partial buffers, so each one first makes ``stream``
+ * wait for the context's stream and then makes the context's stream wait for
+ * the call: they are ordered against ss_step and against each other, also
+ * when issued from different streams. */
 int ss_variation_dev(ss_ctx* ctx, const void* field_dev, void* grad_dev, int mode, void* stream);
 
 /* Device-pointer energy: writes the FP64 total into energy_dev[0]. */
@@ -174,6 +191,14 @@ int ss_energy_dev(ss_ctx* ctx, const void* field_dev, double* energy_dev, void* 
 /* Capture steps_per_graph steps (rounded up to even) into one hipGraph and
  * replay it inside ss_step; 0 disables graphs. */
 int ss_use_graphs(ss_ctx* ctx, int steps_per_graph);
+
+/* Set up ahead of time what the first ss_step(ctx, _, nsteps) would otherwise
+ * build inside the call: the step-history buffer for nsteps, the scratch of
+ * the exact-adjoint and L2-normalised flows, and the step graphs of both
+ * buffer parities (captured, instantiated and uploaded, not run).  Runs no
+ * step; the relaxation state is unchanged.  (No counterpart in the
+ * reference, which launches every kernel eagerly.) */
+int ss_prepare_graphs(ss_ctx* ctx, int nsteps);
 
 /* Wait for all work of the context. */
 int ss_synchronize(ss_ctx* ctx);
@@ -275,7 +300,8 @@ int ss_batch_begin(ss_batch* b);
 
 /* Up to nsteps arrested-Newton steps of every active instance.  An instance
  * retires (stops updating) after the first step whose max |stage-4 energy
- * gradient| is below tol.  The host reads the activity flags every
+ * gradient| is below tol, or, as failed, whose max |stage-4 energy gradient|
+ * is NaN or infinite.  The host reads the activity flags every
  * check_every steps and returns early when none is active. */
 int ss_batch_step(ss_batch* b, int nsteps, double tol, int check_every, int* n_active);
 
@@ -284,8 +310,8 @@ int ss_batch_step(ss_batch* b, int nsteps, double tol, int check_every, int* n_a
 int ss_batch_use_graphs(ss_batch* b, int steps_per_graph);
 
 /* Per-instance energy of the current field, last max |EG|, steps taken and
- * the step at which it converged (-1 if not yet); arrays of ninst, any may be
- * NULL. */
+ * the step at which it converged (-1 if not yet, -2 if it failed with a
+ * non-finite error); arrays of ninst, any may be NULL. */
 int ss_batch_state(ss_batch* b, double* energies, double* err_absmax, int* steps, int* converged_step);
 int ss_batch_memory_bytes(ss_batch* b, size_t* bytes);
 

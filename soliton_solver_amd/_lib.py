@@ -80,6 +80,7 @@ SIGNATURES = {
     "ss_variation_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ss_energy_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ss_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "ss_prepare_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_synchronize": (C.c_int, [C.c_void_p]),
     "ss_time_passes": (C.c_int, [C.c_void_p, C.c_double, C.c_int, _f64p]),
     "ss_set_tile_variants": (C.c_int, [C.c_void_p, _i32p]),

@@ -135,7 +135,9 @@ class BatchSimulation:
         self._check(_lib.lib().ss_batch_use_graphs(self._b, int(steps_per_graph)))
 
     def run(self, nsteps: int, tol: float, check_every: int = 100) -> int:
-        """Advance up to ``nsteps``; returns how many instances are still active."""
+        """Advance up to ``nsteps``; returns how many instances are still active.
+        An instance whose max |EG| turns NaN or infinite retires as failed
+        (``state()["failed"]``), never as converged."""
         n = C.c_int()
         self._check(_lib.lib().ss_batch_step(self._b, int(nsteps), float(tol), int(check_every), C.byref(n)))
         return n.value
@@ -146,7 +148,9 @@ class BatchSimulation:
         steps = np.empty(self.ninst, dtype=np.int32)
         conv = np.empty(self.ninst, dtype=np.int32)
         self._check(_lib.lib().ss_batch_state(self._b, E, err, steps, conv))
-        return {"energy": E, "err_absmax": err, "steps": steps, "converged_step": conv}
+        # converged_step: the step an instance converged at, -1 still running,
+        # -2 retired with a non-finite max |EG|
+        return {"energy": E, "err_absmax": err, "steps": steps, "converged_step": conv, "failed": conv == -2}
 
     @property
     def num_tiles(self) -> int:

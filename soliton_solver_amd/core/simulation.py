@@ -88,6 +88,12 @@ def which_argument(args, default: int = 2) -> int:
     return default
 
 
+def _check_finite(E: float, err: float, steps: int) -> None:
+    """Raise when a relaxation has produced a non-finite energy or error."""
+    if not (np.isfinite(E) and np.isfinite(err)):
+        raise FloatingPointError(f"relaxation diverged: energy {E!r}, max|EG| {err!r} after {steps} steps")
+
+
 class Simulation:
     """Arrested-Newton-flow relaxation of one theory on one GPU."""
 
@@ -167,6 +173,12 @@ class Simulation:
     def use_graphs(self, steps_per_graph: int) -> None:
         """Replay ``steps_per_graph`` captured steps per hipGraph launch (0: off)."""
         self._check(_lib.lib().ss_use_graphs(self._ctx, int(steps_per_graph)))
+
+    def prepare_graphs(self, nsteps: int) -> None:
+        """Build now what the first ``steps(E, nsteps)`` would build inside the call
+        (history buffer, flow scratch, the step graphs of both buffer parities),
+        without running a step."""
+        self._check(_lib.lib().ss_prepare_graphs(self._ctx, int(nsteps)))
 
     # -- state --------------------------------------------------------------
     @property
@@ -285,7 +297,10 @@ class Simulation:
                  prev_energy: float | None = None):
         """Relax until max |EG| < tol (checked every ``check_every`` steps).
 
-        Returns ``(energy, err_absmax, steps_taken)``.
+        Returns ``(energy, err_absmax, steps_taken)``.  Raises
+        ``FloatingPointError`` as soon as a check finds a non-finite energy or
+        max |EG| (the device reductions propagate NaN, so a blown-up field is
+        never taken for a converged one).
         """
         E = self.compute_energy() if prev_energy is None else float(prev_energy)
         done = 0
@@ -294,6 +309,7 @@ class Simulation:
             n = min(check_every, max_steps - done)
             E, _, ea = self.steps(E, n)
             done += n
+            _check_finite(E, ea, done)
             if ea < tol:
                 break
         return E, ea, done
@@ -328,7 +344,8 @@ class Simulation:
         def switch(prec, info):
             self.set_precision(prec)
             self.set_tile_variants(info["variants"])
-            self.set_fused_stages(info["fused23"], info["fused41"] and prec == "fp64")
+            # the compensated form has no 4+1 sweep; every form keeps the paired schedule
+            self.set_fused_stages(info["fused23"], info["fused41"] and prec != "fp32c", pairs=info["pairs"])
 
         first = self.precision
         self.set_precision("fp64")
@@ -345,8 +362,9 @@ class Simulation:
         while True:
             n = min(size, max_steps - n32 - n64)
             if n > 0:
-                E, _, _ = self.steps(E, n)
+                E, _, ea = self.steps(E, n)
                 n32 += n
+                _check_finite(E, ea, n32 + n64)
             switch("fp64", tiles64)
             E = self.compute_energy()
             E, ea, npol = self.minimize(tol, polish, check_every, prev_energy=E)

@@ -92,9 +92,53 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// NaN-propagating maximum (IEEE 754-2019 maximum): NaN in either operand gives
+// NaN.  Every error reduction uses it, so a non-finite energy gradient cannot
+// vanish from max |EG| the way it does under fmax (maxNum drops a NaN operand),
+// and a blown-up field never reads as converged.
+__device__ __forceinline__ double nmax(double a, double b) { return __builtin_elementwise_maximum(a, b); }
+__device__ __forceinline__ float nmax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
+// Per-thread signed and absolute maxima of the stage-4 energy gradient over
+// the sites one thread visits, folded into the thread's FP64 accumulators at
+// the end.  FP32 values are compared in FP32 (exact, and gfx950's
+// v_maximum3_f32 propagates NaN in one instruction, |e| as an operand
+// modifier); FP64 values with fmax plus an unordered-compare flag (one compare
+// per value, where a NaN-propagating FP64 maximum costs three more
+// instructions).
+template <typename R>
+struct ErrMax;
+template <>
+struct ErrMax<float> {
+  float m0 = -__builtin_inff(), m1 = 0.f;
+  __device__ __forceinline__ void add(float e) {
+    m0 = nmax(m0, e);
+    m1 = nmax(m1, __builtin_fabsf(e));
+  }
+  __device__ __forceinline__ void fold(double& acc0, double& acc1) const {
+    acc0 = nmax(acc0, (double)m0);
+    acc1 = nmax(acc1, (double)m1);
+  }
+};
+template <>
+struct ErrMax<double> {
+  double m0 = -1.0e300, m1 = 0.0;
+  bool bad = false;
+  __device__ __forceinline__ void add(double e) {
+    m0 = fmax(m0, e);
+    m1 = fmax(m1, __builtin_fabs(e));
+    bad |= (e != e);
+  }
+  __device__ __forceinline__ void fold(double& acc0, double& acc1) const {
+    const double nan = __builtin_nan("");
+    acc0 = nmax(acc0, bad ? nan : m0);
+    acc1 = nmax(acc1, bad ? nan : m1);
+  }
+};
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, 64));
   return v;
 }
 
@@ -123,7 +167,7 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   double s = -1.0e300;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NT / 64; ++i) s = fmax(s, red[i]);
+    for (int i = 0; i < NT / 64; ++i) s = nmax(s, red[i]);
   }
   return s;  // valid in thread 0
 }

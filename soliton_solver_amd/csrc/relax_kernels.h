@@ -231,6 +231,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
   const int cy = lane + 2;
   double acc0 = kP4 ? -1.0e300 : 0.0;
   double acc1 = 0.0, acc2 = 0.0;
+  ErrMax<R> em;  // pass 4: the stage-4 error of this thread's sites
 #pragma unroll
   for (int stg = 0; stg < NFT / NF; ++stg) {
   const int ao = stg * NF;
@@ -300,9 +301,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
             Fn[a] = tile.v[NF + a][cx][cy];
           else
             Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
-          const double e = (double)eg[a];
-          acc0 = fmax(acc0, e);
-          acc1 = fmax(acc1, fabs(e));
+          em.add(eg[a]);
         }
         if constexpr (COMP) {
           static_assert(sizeof(R) == 4, "compensated combine: FP32 only");
@@ -331,6 +330,7 @@ __device__ __forceinline__ void stage_tile(Tile<R, tile_nf<TH, PASS>(), TXR>& ti
     }
   }
   }  // stage fields
+  if (kP4) em.fold(acc0, acc1);
   if (PASS == 1) {
     const double s = block_sum<kThreads>(acc0, red);
     if (threadIdx.x == 0) b.part0[t] = s;
@@ -536,6 +536,7 @@ combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, cons
   const bool kill = st->kill != 0;
   const R dt = R(g.dt), half = R(0.5), sixth = R(1.0 / 6.0);
   double m0 = -1.0e300, m1 = 0.0;
+  ErrMax<double> em;
   const long long n = g.N;
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) {
@@ -550,8 +551,7 @@ combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, cons
       const R k1 = dt * Vc, k2 = dt * (Vc + half * a1), k3 = dt * (Vc + half * a2), k4 = dt * (Vc + a3);
       Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
       Fn[a] = F[o] + sixth * (k1 + R(2) * k2 + R(2) * k3 + k4);
-      m0 = fmax(m0, e);
-      m1 = fmax(m1, fabs(e));
+      em.add(e);
     }
     if (TH::kUnitConstraint && g.unit_mag) {
       TH::template normalize<R>(tp, Fn);
@@ -563,6 +563,7 @@ combine_pass(const R* F, const R* V, const R* l1, const R* l2, const R* l3, cons
       Vo[i + (long long)a * n] = Vn[a];
     }
   }
+  em.fold(m0, m1);
   const double b0 = block_max<kThreads>(m0, red);
   if (threadIdx.x == 0) part0[blockIdx.x] = b0;
   const double b1 = block_max<kThreads>(m1, red);
@@ -755,7 +756,8 @@ __global__ void __launch_bounds__(256) batch_fin_energy(BatchBufs B, int report)
 }
 
 // Per-instance finalize of the stage-4 error; flips the buffer parity and
-// retires instances whose max |EG| fell below ``tol``.
+// retires instances whose max |EG| fell below ``tol`` (converged_step = the
+// step) or is not finite (converged_step = -2: failed, never converged).
 template <class TH>
 __global__ void __launch_bounds__(256) batch_fin_err(BatchBufs B, double tol) {
   __shared__ double red[4];
@@ -765,8 +767,8 @@ __global__ void __launch_bounds__(256) batch_fin_err(BatchBufs B, double tol) {
   const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
   double m0 = -1.0e300, m1 = 0.0;
   for (int k = threadIdx.x; k < I.ntiles; k += 256) {
-    m0 = fmax(m0, B.part0[I.tile_begin + k]);
-    m1 = fmax(m1, B.part1[I.tile_begin + k]);
+    m0 = nmax(m0, B.part0[I.tile_begin + k]);
+    m1 = nmax(m1, B.part1[I.tile_begin + k]);
   }
   const double a = block_max<256>(m0, red);
   const double b = block_max<256>(m1, red);
@@ -778,6 +780,9 @@ __global__ void __launch_bounds__(256) batch_fin_err(BatchBufs B, double tol) {
     if (b < tol) {
       st.active = 0;
       st.converged_step = st.step;
+    } else if (!(b <= 1.7976931348623157e308)) {  // NaN or +inf: the instance has blown up
+      st.active = 0;
+      st.converged_step = -2;
     }
   }
 }

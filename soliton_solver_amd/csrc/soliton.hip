@@ -133,8 +133,8 @@ k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histE
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0;
   for (int i = threadIdx.x; i < n; i += kFinThreads) {
-    m0 = fmax(m0, p0[i]);
-    m1 = fmax(m1, p1[i]);
+    m0 = nmax(m0, p0[i]);
+    m1 = nmax(m1, p1[i]);
   }
   const double a = block_max<kFinThreads>(m0, red);
   const double b = block_max<kFinThreads>(m1, red);
@@ -156,8 +156,8 @@ k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, De
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < n; i += kFinThreads) {
-    m0 = fmax(m0, p0[i]);
-    m1 = fmax(m1, p1[i]);
+    m0 = nmax(m0, p0[i]);
+    m1 = nmax(m1, p1[i]);
     const double y = p2[i] - c;
     const double t = s + y;
     c = (t - s) - y;
@@ -302,6 +302,7 @@ struct ss_ctx {
   int xpart_n = 0;
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // indexed by starting parity
+  hipEvent_t ev_dev = nullptr;  // orders the device-pointer calls against the context's stream
   std::string err;
 };
 
@@ -871,6 +872,8 @@ static int build_graph(ss_ctx* c, int parity) {
   e = hipGraphInstantiate(&c->graph[parity], gr, nullptr, nullptr, 0);
   hipGraphDestroy(gr);
   if (e != hipSuccess) return hipfail(c, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  // move the executable graph to the device now rather than at its first launch
+  HIPCHK(c, hipGraphUpload(c->graph[parity], c->stream));
   return 0;
 }
 
@@ -973,6 +976,7 @@ int ss_ctx_destroy(ss_ctx* c) {
   if (c->Px) hipFree(c->Px);
   if (c->Py) hipFree(c->Py);
   if (c->xpart) hipFree(c->xpart);
+  if (c->ev_dev) hipEventDestroy(c->ev_dev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1132,6 +1136,18 @@ int ss_use_graphs(ss_ctx* c, int steps) {
   if (steps < 0) return fail(c, "steps_per_graph must be >= 0");
   free_graphs(c);
   c->graph_steps = steps ? ((steps + 1) & ~1) : 0;
+  return 0;
+}
+
+int ss_prepare_graphs(ss_ctx* c, int nsteps) {
+  if (!c) return fail(nullptr, "null context");
+  if (nsteps < 0) return fail(c, "nsteps must be >= 0");
+  if (ensure_hist(c, nsteps + 1)) return -1;
+  if ((c->var_mode == SS_VAR_EXACT_ADJOINT || c->ops->l2_normalize) && ensure_exact(c)) return -1;
+  if (c->graph_steps <= 0) return 0;
+  for (int p = 0; p < 2; ++p)
+    if (!c->graph[p] && build_graph(c, p)) return -1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -1640,21 +1656,48 @@ int ss_variation(ss_ctx* c, int mode, double* host_eg, size_t n) {
   return 0;
 }
 
+// The device-pointer calls run on the caller's stream but use the context's
+// scratch (partials, exact-adjoint buffers): the caller's stream first waits
+// for the context's stream, and the context's stream then waits for the call,
+// so they are ordered against ss_step and against each other even when issued
+// from different streams.
+static int dev_call_begin(ss_ctx* c, hipStream_t st) {
+  if (c->prec != SS_PREC_FP64)
+    return fail(c, "device-pointer calls take FP64 fields: the context is in precision %d", c->prec);
+  if (!c->ev_dev) HIPCHK(c, hipEventCreateWithFlags(&c->ev_dev, hipEventDisableTiming));
+  if (st != c->stream) {
+    HIPCHK(c, hipEventRecord(c->ev_dev, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(st, c->ev_dev, 0));
+  }
+  return 0;
+}
+static int dev_call_end(ss_ctx* c, hipStream_t st) {
+  if (st != c->stream) {
+    HIPCHK(c, hipEventRecord(c->ev_dev, st));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_dev, 0));
+  }
+  return 0;
+}
+
 int ss_variation_dev(ss_ctx* c, const void* field_dev, void* grad_dev, int mode, void* stream) {
   if (!c) return fail(nullptr, "null context");
   if (!field_dev || !grad_dev) return fail(c, "null device pointer");
   hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
-  return variation_into(c, field_dev, grad_dev, mode, st);
+  if (mode == SS_VAR_EXACT_ADJOINT && ensure_exact(c)) return -1;  // allocates before any ordering
+  if (dev_call_begin(c, st)) return -1;
+  if (variation_into(c, field_dev, grad_dev, mode, st)) return -1;
+  return dev_call_end(c, st);
 }
 
 int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* stream) {
   if (!c) return fail(nullptr, "null context");
   if (!field_dev || !energy_dev) return fail(c, "null device pointer");
   hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
+  if (dev_call_begin(c, st)) return -1;
   HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, c->part, c->tp, c->g, st));
   hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->nt0, energy_dev);
   HIPCHK(c, hipGetLastError());
-  return 0;
+  return dev_call_end(c, st);
 }
 
 int ss_synchronize(ss_ctx* c) {
@@ -1821,8 +1864,6 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
   if (ops->l2_normalize)
     return bfail(nullptr, -1, "theory %s L2-normalises its stage fields; the batched form does not", ops->name);
   if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32) return bfail(nullptr, -1, "bad precision %d", precision);
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bfail(nullptr, -2, "no HIP device available");
   ss_batch* b = new ss_batch();
   b->theory = theory_id;
   b->ops = ops;
@@ -1849,6 +1890,13 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
       bfail(b, -1, "instance %d: theory %s is built for %d fields", i, ops->name, ops->nf);
       return bail(-1);
     }
+    // the batched strip kernels address an instance's planes with 32-bit
+    // byte offsets from its own base, as the single-lattice strips do
+    if (!((long long)g.nf * g.N < (1LL << 31) && g.N * (long long)sizeof(double) < (1LL << 32))) {
+      bfail(b, -1, "instance %d: %d x %d x %d values exceed the batched form's limit (nf*xlen*ylen < 2^31)", i,
+            g.nf, g.X, g.Y);
+      return bail(-1);
+    }
     char msg[256] = {0};
     if (ops->make_tp(pi, n_i, pf, n_f, b->tps.data() + (size_t)512 * i, msg, sizeof(msg))) {
       bfail(b, -1, "instance %d: %s", i, msg);
@@ -1857,6 +1905,11 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
     b->geoms.push_back(g);
     b->offs.push_back(b->total);
     b->total += (long long)g.nf * g.N;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    bfail(b, -2, "no HIP device available");
+    return bail(-2);
   }
   for (int v = 0; v < ops->num_shapes; ++v) {
     int sx = 0, tyw = TY;

@@ -61,8 +61,8 @@ template <int NT>
 __device__ __forceinline__ bool fold_finalize(const PassBufs& b, bool writer, double* red) {
   double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
   for (int i = threadIdx.x; i < b.fin_n; i += NT) {
-    m0 = fmax(m0, b.fin_p0[i]);
-    m1 = fmax(m1, b.fin_p1[i]);
+    m0 = nmax(m0, b.fin_p0[i]);
+    m1 = nmax(m1, b.fin_p1[i]);
     const double y = b.fin_p2[i] - c;
     const double t = s + y;
     c = (t - s) - y;
@@ -523,6 +523,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       }
     }
   }
+  ErrMax<R> em;  // pass 4: the stage-4 error of this call's sites
 #pragma unroll
   for (int k = 0; k < RPW; ++k) {
     const int x = c0 + w * RPW + k;
@@ -602,9 +603,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
             Fn[a] = q.v[FOFF + a][sl[2]][cy];
           else
             Fn[a] = f[a] + dt6 * (a1 - R(2) * a2 + a3);
-          const double e = (double)eg[a];
-          acc0 = fmax(acc0, e);
-          acc1 = fmax(acc1, fabs(e));
+          em.add(eg[a]);
         }
         if constexpr (COMP) {
           static_assert(sizeof(R) == 4 && FOFF < 0, "compensated combine: FP32 pass 4 only");
@@ -625,6 +624,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       }
     }
   }
+  if (PASS == 4) em.fold(acc0, acc1);
 }
 
 // One strip of one pass (PASS 0..4 as in stage_tile; 23 = passes 2 and 3 in
