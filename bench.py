@@ -437,6 +437,9 @@ def schedule(tiles, comp):
     v4 = tiles["variants"][3]
     cw = 2 if v4 >= 6 else 1
     if comp:
+        if pairs and tiles.get("comp12"):  # 1+2 (F, lo, V -> l1, l2) + 3+4 (F, lo, V, l1, l2 -> F', lo', V')
+            return {"name": "compensated paired 1+2, 3+4", "dominant_words": 8, "step_words": 13,
+                    "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
         if pairs:  # pass 1 (F, lo -> l1) + pass 2 (F, V -> l2) + 3+4 (F, lo, V, l1, l2 -> F', lo', V')
             return {"name": "compensated pass 1, pass 2, paired 3+4", "dominant_words": 8, "step_words": 14,
                     "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}

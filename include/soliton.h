@@ -255,16 +255,21 @@ int ss_autotune_tiles(ss_ctx* ctx, int reps, int* chosen, double* ms);
  * instead of 14.  Passes 1+2 read V before the arrest decision; after an
  * arrest passes 3+4 take l2 = l1, the l2 of V = 0.  Pass 1+2 uses the strip
  * variant of pass 1, pass 3+4 that of pass 4 (variant 3 where the one set
- * does not run the sweep).  In the compensated form (whose pass 1 carries
- * the low words in the second ring half) only passes 3+4 pair, as the
- * compensated combine after a separate pass 2: pass 1 (3 words) + pass 2
- * (3) + 3+4 (8) = 14 words instead of 17.  ss_autotune_tiles times both sweeps in every
+ * does not run the sweep).  In the compensated form the 1+2 sweep also
+ * carries the low words (a three-part ring: F, Temp2, lo; hi, lo, V in; l1
+ * with the energy of hi + lo, and l2 out) and the 3+4 sweep ends with the
+ * compensated combine: 5 + 8 = 13 words instead of 17; where the three-part
+ * ring does not fit the theory, passes 1 and 2 run separately before the 3+4
+ * sweep (3 + 3 + 8 = 14 words), and ss_get_fused_stages reports
+ * SS_FUSE_COMP12 only when the single 1+2 sweep runs (setting it has no
+ * effect).  ss_autotune_tiles times both sweeps in every
  * strip variant (ms[54..62] for 1+2, ms[63..71] for 3+4) and switches the
  * paired schedule on when its two sweeps beat the schedule it chose
  * otherwise by 3 %. */
 #define SS_FUSE_23 1
 #define SS_FUSE_41 2
 #define SS_FUSE_PAIRS 4
+#define SS_FUSE_COMP12 8
 int ss_set_fused_stages(ss_ctx* ctx, int flags);
 int ss_get_fused_stages(ss_ctx* ctx, int* flags);
 

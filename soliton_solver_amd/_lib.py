@@ -32,6 +32,7 @@ AUX_HIGGS, AUX_HIGGS1, AUX_HIGGS2, AUX_PHASE_DIFFERENCE, AUX_SUPERCURRENT, AUX_M
 FUSE_23 = 1
 FUSE_41 = 2
 FUSE_PAIRS = 4
+FUSE_COMP12 = 8
 BUF_FIELD = 0
 BUF_VELOCITY = 1
 

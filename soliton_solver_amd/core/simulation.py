@@ -550,7 +550,8 @@ class Simulation:
         on = C.c_int()
         self._check(_lib.lib().ss_get_fused_stages(self._ctx, C.byref(on)))
         return {"variants": v.tolist(), "rows": rows.tolist(), "fused23": bool(on.value & _lib.FUSE_23),
-                "fused41": bool(on.value & _lib.FUSE_41), "pairs": bool(on.value & _lib.FUSE_PAIRS)}
+                "fused41": bool(on.value & _lib.FUSE_41), "pairs": bool(on.value & _lib.FUSE_PAIRS),
+                "comp12": bool(on.value & _lib.FUSE_COMP12)}
 
     # -- output bundle and resume (core/simulation.py:411-510, core/io.py) ---
     def save_output(self, output_dir: str | None = None, precision: int = 32) -> str:

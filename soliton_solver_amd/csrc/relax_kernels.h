@@ -906,6 +906,13 @@ struct TheoryLaunchers {
         } else {
           return hipErrorInvalidValue;
         }
+      case 12:
+        if constexpr (sizeof(R) == 4 && StreamRing<TH, R, 12, CW, true>::shape::kFits) {
+          hipLaunchKernelGGL((stream_pass_comp<TH, R, 12, CW>), grid, block, 0, st, b, tp, g, L);
+          break;
+        } else {
+          return hipErrorInvalidValue;
+        }
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -976,7 +983,27 @@ struct TheoryLaunchers {
     }
     return pairs_fit<R, 1>();
   }
+  // The compensated form (prec 2): bit 0 = its 3+4 sweep (id 11) runs in v,
+  // bit 1 = its 1+2 sweep (id 12, a three-part ring: F, Temp2 and the low
+  // words) runs there too; the larger ring fitting implies the smaller.
+  template <int CW>
+  static constexpr int comp_pairs_fit() {
+    return (StreamRing<TH, float, 34, CW, true>::shape::kFits ? 1 : 0) |
+           (StreamRing<TH, float, 12, CW, true>::shape::kFits && StreamRing<TH, float, 34, CW, true>::shape::kFits
+                ? 2
+                : 0);
+  }
+  static int comp_pairs_variant(int v, const Geom& g) {
+    if (TH::kL2Normalize || v < 3) return 0;
+    v = effective_shape<float>(v, g);
+    if (v < 3) return 0;
+    if constexpr (wide_fits<float>()) {
+      if (strip_cw(v) == 2) return comp_pairs_fit<2>();
+    }
+    return comp_pairs_fit<1>();
+  }
   static int pairs_variant(int prec, int v, const Geom& g) {
+    if (prec == 2) return comp_pairs_variant(v, g);
     return prec ? pairs_variant_r<float>(v, g) : pairs_variant_r<double>(v, g);
   }
 

@@ -445,6 +445,15 @@ static bool use_comp_pairs(const ss_ctx* c) {
 }
 // Strip variant of a paired sweep: v where both sweeps run in it, else 3.
 static int pairs_v(const ss_ctx* c, int v) { return c->ops->pairs_variant(c->prec, v, c->g) ? v : 3; }
+// The compensated form's 1+2 sweep (id 12: F, Temp2 and the low words in a
+// three-part ring) runs in strip variant v.
+static bool comp12_in(const ss_ctx* c, int v) {
+  return c->prec == SS_PREC_FP32C && (c->ops->pairs_variant(c->prec, v, c->g) & 2) != 0;
+}
+// The compensated paired schedule runs passes 1+2 as that sweep (in the first
+// strip variant at least), not as pass 1 and pass 2.
+static bool comp12_ok(const ss_ctx* c) { return use_comp_pairs(c) && comp12_in(c, 3); }
+static int comp12_v(const ss_ctx* c, int v) { return comp12_in(c, v) ? v : 3; }
 
 // Shape variant of the fused 2+3 sweep: tv[2] where the sweep exists in it,
 // else the first strip variant.
@@ -484,7 +493,7 @@ static int energy_partials(ss_ctx* c, const void* F, int* nparts) {
 // stored variants are set to what runs, so ss_get_tile_variants reports it.
 static void settle_variants(ss_ctx* c) {
   if (c->pairs) {
-    c->tv[1] = pairs_v(c, c->tv[1]);
+    c->tv[1] = c->prec == SS_PREC_FP32C ? comp12_v(c, c->tv[1]) : pairs_v(c, c->tv[1]);
     c->tv[4] = pairs_v(c, c->tv[4]);
   }
   if (c->fuse23) c->tv[2] = fused23_v(c);
@@ -615,8 +624,12 @@ static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
   return 0;
 }
 
-// Compensated form, paired: pass 1 (low words, energy of hi + lo) -> the
-// arrest decision -> pass 2 (V arrested) -> compensated 3+4 -> err.
+// Compensated form, paired.  Where the three-part ring fits: the
+// compensated 1+2 sweep (F, its low words and V in; l1 with the energy of
+// hi + lo, and l2 from Temp2 = F + dt V/2 with V read before the arrest
+// decision) -> the decision -> compensated 3+4 (after an arrest with
+// l2 = l1) -> err.  Elsewhere: pass 1 (low words, energy of hi + lo) -> the
+// decision -> pass 2 (V arrested) -> compensated 3+4 -> err.
 static int launch_step_comp_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
@@ -624,6 +637,29 @@ static int launch_step_comp_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
   const int v1 = pass1_variant(c), v34 = pairs_v(c, c->tv[4]);
   int e = 0;
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+  if (comp12_ok(c)) {
+    const int v12 = comp12_v(c, c->tv[1]);
+    b.lout = l1_buf(c, parity);
+    b.lout2 = c->l[1];
+    b.part0 = p0;
+    HIPCHK(c, c->ops->pass(c->prec, 12, v12, b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
+                       c->ops->ntiles(c->prec, v12, c->g), c->st, c->hist, c->hist_cap, c->g.killkinen, parity);
+    HIPCHK(c, hipGetLastError());
+    if (ev)
+      for (int k = 0; k < 3; ++k) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate passes 2, 3
+    b.lout = nullptr;
+    b.lout2 = nullptr;
+    b.part1 = p1;
+    HIPCHK(c, c->ops->pass(c->prec, 11, v34, b, c->tp, c->g, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
+                       c->ops->ntiles(c->prec, v34, c->g), c->st, c->hist + c->hist_cap, c->hist_cap);
+    HIPCHK(c, hipGetLastError());
+    if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
+    return 0;
+  }
   b.lout = l1_buf(c, parity);
   b.part0 = p0;
   HIPCHK(c, c->ops->pass(c->prec, 8, v1, b, c->tp, c->g, c->stream));
@@ -1382,19 +1418,16 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   // parity's field, which the state does not hold
   float best12 = 0.f, best34 = 0.f;
   int pick12 = -1, pick34 = -1;
-  // (compensated form: only its 3+4 sweep, id 11, after a separate pass 2)
+  // (compensated form: its 1+2 sweep, id 12, where the three-part ring fits,
+  // and its 3+4 sweep, id 11)
   const bool compf = c->prec == SS_PREC_FP32C;
   const bool any_pairs = c->ops->pairs_variant(c->prec, 3, c->g);
   for (int k = 0; k < 2 && rc == 0; ++k) {
-    const int pid = k == 0 ? 9 : compf ? 11 : 10;
-    if (compf && k == 0) {
-      for (int v = 0; v < c->ops->num_shapes && ms; ++v) ms[6 * c->ops->num_shapes + v] = 0.0;
-      continue;
-    }
+    const int pid = k == 0 ? (compf ? 12 : 9) : compf ? 11 : 10;
     b.lout = k == 0 ? c->l[0] : nullptr;
     b.lout2 = k == 0 ? c->l[1] : nullptr;
     for (int v = 0; v < c->ops->num_shapes && rc == 0; ++v) {
-      if (!any_pairs || !c->ops->pairs_variant(c->prec, v, c->g)) {
+      if (!any_pairs || !c->ops->pairs_variant(c->prec, v, c->g) || (compf && k == 0 && !comp12_in(c, v))) {
         if (ms) ms[(6 + k) * c->ops->num_shapes + v] = 0.0;
         continue;
       }
@@ -1444,7 +1477,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
   }
   // the paired schedule replaces all of these when at least 3 % faster per step
   const float sched = (c->fuse41 ? best41 : sep41) + (c->fuse23 ? best23 : sep23);
-  if (pick12 >= 0 && pick34 >= 0 && best12 + best34 < 0.97f * sched) {
+  if (!compf && pick12 >= 0 && pick34 >= 0 && best12 + best34 < 0.97f * sched) {
     c->fuse23 = c->fuse41 = 0;
     c->pairs = 1;
     c->tv[1] = pick12;
@@ -1454,14 +1487,23 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
       chosen[3] = pick34;
     }
   }
-  if (compf && pick34 >= 0 && bestp[2] + best34 < 0.97f * ((c->fuse23 ? best23 : sep23) + bestp[4])) {
-    c->fuse23 = 0;
-    c->pairs = 1;
-    c->tv[2] = pick[2];
-    c->tv[4] = pick34;
-    if (chosen) {
-      chosen[1] = pick[2];
-      chosen[3] = pick34;
+  // compensated form: pass 1 + (2+3 or 2, 3) + pass 4, against the paired
+  // schedule, which is the 1+2 sweep + 3+4 where the three-part ring fits
+  // (else pass 1 + pass 2 + 3+4)
+  if (compf && pick34 >= 0) {
+    const float sepc = bestp[1] + (c->fuse23 ? best23 : sep23) + bestp[4];
+    const float pairc = pick12 >= 0 ? best12 + best34 : bestp[1] + bestp[2] + best34;
+    if (pairc < 0.97f * sepc) {
+      c->fuse23 = 0;
+      c->pairs = 1;
+      c->tv[2] = pick[2];
+      c->tv[4] = pick34;
+      if (pick12 >= 0) c->tv[1] = pick12;
+      if (chosen) {
+        if (pick12 >= 0) chosen[0] = pick12;
+        chosen[1] = pick[2];
+        chosen[3] = pick34;
+      }
     }
   }
   settle_variants(c);
@@ -1471,6 +1513,7 @@ int ss_autotune_tiles(ss_ctx* c, int reps, int* chosen, double* ms) {
 
 int ss_set_fused_stages(ss_ctx* c, int flags) {
   if (!c) return fail(nullptr, "null context");
+  flags &= ~SS_FUSE_COMP12;  // reported by ss_get_fused_stages, implied by SS_FUSE_PAIRS
   if (flags & ~(SS_FUSE_23 | SS_FUSE_41 | SS_FUSE_PAIRS)) return fail(c, "unknown fused-stage flags 0x%x", flags);
   if ((flags & SS_FUSE_PAIRS) && (flags & (SS_FUSE_23 | SS_FUSE_41)))
     return fail(c, "the paired schedule replaces the 2+3 and 4+1 sweeps: set it alone");
@@ -1493,7 +1536,9 @@ int ss_set_fused_stages(ss_ctx* c, int flags) {
 
 int ss_get_fused_stages(ss_ctx* c, int* flags) {
   if (!c) return fail(nullptr, "null context");
-  if (flags) *flags = (c->fuse23 ? SS_FUSE_23 : 0) | (c->fuse41 ? SS_FUSE_41 : 0) | (c->pairs ? SS_FUSE_PAIRS : 0);
+  if (flags)
+    *flags = (c->fuse23 ? SS_FUSE_23 : 0) | (c->fuse41 ? SS_FUSE_41 : 0) | (c->pairs ? SS_FUSE_PAIRS : 0) |
+             (comp12_ok(c) ? SS_FUSE_COMP12 : 0);
   return 0;
 }
 

@@ -305,7 +305,8 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
 // element 0 and is replaced by 0, so no load sits behind a branch.  PASS
 // selects the arrays: 0/1 F; 2 and 12 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 34
 // F, V, l1, l2 (into la); 41 F, V, l2, l1, l3; pass 1 of the compensated form (PASS 1 with a two-half ring) F
-// and its low words (into l).
+// and its low words (into l); passes 1+2 of the compensated form (PASS 12 with a three-part ring) F, V and the
+// low words (into l).
 template <int PASS, typename R, int NF, int NFR, int SC>
 __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, const PassBufs& b,
                                            const Geom& g, int rb, int y0, bool kill) {
@@ -334,7 +335,7 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
       if constexpr (PASS >= 3 && PASS != 12) {
         const R lv = ld<R>(plane<R>(L, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
-      } else if constexpr (PASS == 1 && NFR == 2 * NF) {
+      } else if constexpr ((PASS == 1 && NFR == 2 * NF) || (PASS == 12 && NFR == 3 * NF)) {
         const R lv = ld<R>(plane<R>(b.Flo, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
       } else {
@@ -424,6 +425,7 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
       } else if (PASS == 12) {
         q.v[a][slot][c] = p.f[k][a];
         q.v[NF + a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], R(0), dt);
+        if constexpr (NFR == 3 * NF) q.v[2 * NF + a][slot][c] = p.l[k][a];  // compensated: low words
       } else if (PASS == 34) {
         q.v[a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
         q.v[NF + a][slot][c] = stage_value<4, R>(p.f[k][a], p.v[k][a], p.la[k][a], dt);
@@ -564,11 +566,12 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       }
       if (PASS == 1) {
         if constexpr (COMP) {
-          // compensated form: the energy of hi + lo (low words in ring
-          // components NF..2NF-1), read after the variation so that the FP64
+          // compensated form: the energy of hi + lo (low words in the last
+          // NF ring components), read after the variation so that the FP64
           // values are not held across it
 #pragma unroll
-          for (int a = 0; a < NF; ++a) ring_d1_hilo<INTERIOR, R, NFR>(q, AOFF + a, NF + a, sl, cy, Kd, s, fd[a], d1d[a]);
+          for (int a = 0; a < NF; ++a)
+            ring_d1_hilo<INTERIOR, R, NFR>(q, AOFF + a, NFR - NF + a, sl, cy, Kd, s, fd[a], d1d[a]);
         }
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
@@ -645,7 +648,9 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 template <class TH, typename R, int PASS, int CW = 1, bool COMP = false>
 struct StreamRing {
   static constexpr int NFR =
-      (PASS == 23 || PASS == 41 || PASS == 12 || PASS == 34 || (PASS == 1 && COMP)) ? 2 * TH::NF : TH::NF;
+      (PASS == 12 && COMP) ? 3 * TH::NF
+      : (PASS == 23 || PASS == 41 || PASS == 12 || PASS == 34 || (PASS == 1 && COMP)) ? 2 * TH::NF
+                                                                                     : TH::NF;
   // the 4+1 and 3+4 sweeps prefetch five and four arrays: one row per wave and iteration
   static constexpr int SC = (PASS == 41 || PASS == 34 || PASS == 12 ? 1 : 2) | (CW == 2 ? 4 : 0);
   using shape = StreamShape<R, NFR, SC>;
@@ -719,9 +724,9 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       } else {
         q.v[a][r][c] = in ? stage_value<PASS, R>(fo, v, l, dt) : R(0);
       }
-      if constexpr (PASS == 1 && COMP) {
+      if constexpr ((PASS == 1 || PASS == 12) && COMP) {  // low words in the last ring part
         const R lo = ld<R>(plane<R>(b.Flo, a, NP), boff);
-        q.v[NF + a][r][c] = in ? lo : R(0);
+        q.v[NFR - NF + a][r][c] = in ? lo : R(0);
       }
     }
   }
@@ -785,11 +790,14 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       }
     } else if (PASS == 12) {
       // pass 1 from F (ring 0..NF-1: l1 and the energy), pass 2 from Temp2 (ring NF..2NF-1)
+      // (compensated form: pass 1 takes the energy of F + its low words, ring 2NF..3NF-1)
       if (interior) {
-        stream_rows<TH, R, 1, true, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 1, true, NFR, 0, -1, SC, COMP>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0,
+                                                          acc1);
         stream_rows<TH, R, 2, true, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused);
       } else {
-        stream_rows<TH, R, 1, false, NFR, 0, -1>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0, acc1);
+        stream_rows<TH, R, 1, false, NFR, 0, -1, SC, COMP>(q, b2, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, acc0,
+                                                           acc1);
         stream_rows<TH, R, 2, false, NFR, NF, -1>(q, b3, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused,
                                                   unused);
       }
