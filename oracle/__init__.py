@@ -16,5 +16,7 @@ shape ``(nf, xlen, ylen)`` so that ``F.ravel()`` is the reference's flat
 1 for d/dy); second derivatives are ``D2[a, k, x, y]`` with k = 0 (xx), 1 (yy),
 2 (the mixed entry).
 
-Parity pinning of each function is listed in ``oracle/PINNING.md``.
+The checks that pin each function to the reference (closed forms, identities,
+invariants) are in ``tests/test_oracle_pinning.py`` and
+``tests/test_oracle_pinning_charges.py``.
 """

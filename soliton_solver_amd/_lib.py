@@ -120,11 +120,28 @@ class LibraryMissing(RuntimeError):
     """libsoliton.so is not built (or failed to load)."""
 
 
+def _share_torch_hip_runtime() -> None:
+    """Load torch's HIP runtime first, when torch is installed.
+
+    torch ships its own libamdhip64 (soname libamdhip64.so.7, the library's
+    own dependency).  Loaded first, it satisfies the library's dependency and
+    the process has one HIP runtime; loaded after /opt/rocm's copy (which the
+    library would pull in otherwise), torch starts a second runtime that finds
+    no GPU.  The autograd boundary hands torch tensors to the library, so both
+    must share one runtime.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load the library once; raise ``LibraryMissing`` if it is not there."""
     global _lib
     if _lib is not None:
         return _lib
+    _share_torch_hip_runtime()
     if not LIB_PATH.exists():
         raise LibraryMissing(
             f"{LIB_PATH} is not built; run `python -m soliton_solver_amd.build` "

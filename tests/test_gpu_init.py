@@ -190,3 +190,31 @@ def test_device_init_rejects_unavailable_modes():
     d.mode = _lib.INIT_SKYRMIONS
     assert L.ss_init_field(sim._ctx, C.byref(d)) != 0
     sim.close()
+
+
+_ORACLE_CASES = [c for c in _CASES
+                 if c[0] in ("Chiral magnet", "Baby Skyrme model", "Ginzburg-Landau superconductor")
+                 and c[1].get("mode") in ("ground", "initial", "skyrmions")
+                 and "skyrmion_number" not in c[1]]
+
+
+@pytest.mark.parametrize("name,cfg", _ORACLE_CASES, ids=[f"{n}-{c['mode']}-{i}" for i, (n, c) in enumerate(_ORACLE_CASES)])
+def test_device_initial_conditions_match_the_oracle(name, cfg):
+    """ss_init_field against oracle/initial.py, the restatement of the
+    reference's initial-condition kernels (chiral_magnet/initial_config.py:
+    16-386, baby_skyrme/initial_config.py:18-383,
+    ginzburg_landau_superconductor/initial_config.py:14-313), not against the
+    product's own host generators."""
+    from tests.test_oracle_pinning_charges import oracle_initial_field
+    th = ssa.load_theory(name)
+    sim = ssa.Simulation(_PARAMS[name](th), th)
+    try:
+        sim.initialize({**cfg, "device": True})
+        F_dev = sim.get_field()
+        F_or = oracle_initial_field(name, cfg, sim.p_i_h, sim.p_f_h)
+        err = np.abs(F_dev - F_or).max()
+        print(f"[init vs oracle] {name} {cfg}: max|dF| = {err:.2e}", flush=True)
+        assert err <= 1e-11 * max(1.0, np.abs(F_or).max())
+    finally:
+        sim.close()
+
