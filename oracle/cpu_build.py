@@ -42,6 +42,9 @@ def lib():
         f64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
         L.cref_chiral_relax.restype = C.c_int
         L.cref_chiral_relax.argtypes = [i32, C.c_int, f64, C.c_int, f64, f64, C.c_double, C.c_int, f64, f64, C.c_int]
+        L.cref_chiral_relax_tol.restype = C.c_int
+        L.cref_chiral_relax_tol.argtypes = [i32, C.c_int, f64, C.c_int, f64, f64, C.c_double, C.c_int, C.c_double,
+                                            f64, f64, f64, C.c_int]
         L.cref_max_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -60,6 +63,22 @@ def chiral_relax(p_i, p_f, F, V, prev_energy, nsteps, nthreads=0):
     if rc != 0:
         raise ValueError("cref_chiral_relax rejected its parameters")
     return Es[:nsteps], errs[:nsteps]
+
+
+def chiral_relax_to(p_i, p_f, F, V, prev_energy, max_steps, tol, nthreads=0):
+    """Steps in place on F, V until max |stage-4 EG| < tol (at most
+    ``max_steps``); returns (energies, errs, errs_abs) of the steps taken."""
+    L = lib()
+    p_i = np.ascontiguousarray(p_i, dtype=np.int32)
+    p_f = np.ascontiguousarray(p_f, dtype=np.float64)
+    assert F.flags.c_contiguous and V.flags.c_contiguous and F.dtype == np.float64
+    n = max(max_steps, 1)
+    Es, errs, ea = np.empty(n), np.empty(n), np.empty(n)
+    done = L.cref_chiral_relax_tol(p_i, p_i.size, p_f, p_f.size, F.reshape(-1), V.reshape(-1), float(prev_energy),
+                                   int(max_steps), float(tol), Es, errs, ea, int(nthreads))
+    if done < 0:
+        raise ValueError("cref_chiral_relax_tol rejected its parameters")
+    return Es[:done], errs[:done], ea[:done]
 
 
 def max_threads() -> int:

@@ -167,8 +167,13 @@ double energy(const P& p, const double* F, double* d1, double* d2, std::vector<d
 
 }  // namespace
 
-extern "C" int cref_chiral_relax(const int32_t* pi, int ni, const double* pf, int nfl, double* F, double* V,
-                                 double prev_energy, int nsteps, double* energies, double* errs, int nthreads) {
+// nsteps reference steps (or fewer: with tol > 0 the run stops after the
+// first step whose max |stage-4 EG| is below tol); energies, errs (signed
+// max, the reference's err) and errs_abs (max |EG|) get one entry per step
+// taken (any may be null).  Returns the number of steps taken, or -1.
+static int chiral_relax(const int32_t* pi, int ni, const double* pf, int nfl, double* F, double* V,
+                        double prev_energy, int nsteps, double tol, double* energies, double* errs, double* errs_abs,
+                        int nthreads) {
   if (ni < 16 || nfl < 9 || pi[4] != 4 || pi[10] != 3 || pi[2] < 2) return -1;
   if (nthreads > 0) omp_set_num_threads(nthreads);
   P p;
@@ -223,14 +228,32 @@ extern "C" int cref_chiral_relax(const int32_t* pi, int ni, const double* pf, in
     }
     const double En = energy(p, F, d1.data(), d2.data(), en);
     if (p.killkinen && En > E) memset(V, 0, sizeof(double) * M);
-    double mx = -1.0e300;
-    for (long long i = 0; i < M; ++i)
+    double mx = -1.0e300, ma = 0.0;
+    for (long long i = 0; i < M; ++i) {
       if (mx < EG[i]) mx = EG[i];
+      if (!(fabs(EG[i]) <= ma)) ma = fabs(EG[i]);  // NaN stays
+    }
     E = En;
     if (energies) energies[step] = En;
     if (errs) errs[step] = mx;
+    if (errs_abs) errs_abs[step] = ma;
+    if (tol > 0.0 && ma < tol) return step + 1;
   }
-  return 0;
+  return nsteps;
+}
+
+extern "C" int cref_chiral_relax(const int32_t* pi, int ni, const double* pf, int nfl, double* F, double* V,
+                                 double prev_energy, int nsteps, double* energies, double* errs, int nthreads) {
+  return chiral_relax(pi, ni, pf, nfl, F, V, prev_energy, nsteps, 0.0, energies, errs, nullptr, nthreads) < 0 ? -1
+                                                                                                             : 0;
+}
+
+// As cref_chiral_relax, stopping after the first step whose max |stage-4 EG|
+// is below tol; returns the steps taken (or -1).
+extern "C" int cref_chiral_relax_tol(const int32_t* pi, int ni, const double* pf, int nfl, double* F, double* V,
+                                     double prev_energy, int nsteps, double tol, double* energies, double* errs,
+                                     double* errs_abs, int nthreads) {
+  return chiral_relax(pi, ni, pf, nfl, F, V, prev_energy, nsteps, tol, energies, errs, errs_abs, nthreads);
 }
 
 extern "C" int cref_max_threads(void) { return omp_get_max_threads(); }
