@@ -188,6 +188,22 @@ int ss_variation_dev(ss_ctx* ctx, const void* field_dev, void* grad_dev, int mod
 /* Device-pointer energy: writes the FP64 total into energy_dev[0]. */
 int ss_energy_dev(ss_ctx* ctx, const void* field_dev, double* energy_dev, void* stream);
 
+/* ss_energy_dev without any allocation and without the context's scratch:
+ * same result (the same kernels and summation order), with the per-tile
+ * energy partials in a library-held static device array of 2 MiB (262144
+ * doubles per GPU, held until the process ends) instead of the context's
+ * partials.  No counterpart in the reference; it is for callers that cannot
+ * allocate: code under HIP graph capture and allocation-free serving loops.
+ * Calls no hipMalloc / hipFree and creates no event; the array's device
+ * address is resolved once per GPU by ss_ctx_create.  Fails (-1) for a
+ * context not in SS_PREC_FP64 and for lattices that need more than 262144
+ * partials (about 8192 x 8192 sites in FP64 and beyond).  Every call on one
+ * GPU shares that GPU's array and it keeps the last call's partials: calls
+ * that overlap on one GPU (concurrent streams, threads) race; calls ordered on
+ * one stream do not.  It does not touch the context's device memory, so it
+ * may overlap with ss_step on the same context. */
+int ss_energy_dev_noalloc(ss_ctx* ctx, const void* field_dev, double* energy_dev, void* stream);
+
 /* Capture steps_per_graph steps (rounded up to even) into one hipGraph and
  * replay it inside ss_step; 0 disables graphs. */
 int ss_use_graphs(ss_ctx* ctx, int steps_per_graph);

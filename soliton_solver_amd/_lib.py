@@ -80,6 +80,7 @@ SIGNATURES = {
     "ss_variation": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t]),
     "ss_variation_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ss_energy_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ss_energy_dev_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ss_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_prepare_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_synchronize": (C.c_int, [C.c_void_p]),

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -259,12 +260,54 @@ __global__ void k_scale_into(double* dst, const double* src, long long n, double
 }
 
 // ---------------------------------------------------------------------------
+// Library-held scratch of ss_energy_dev_noalloc: the per-tile energy partials
+// live in this file-scope device array instead of memory the library
+// allocates, so the call works where hipMalloc may not run (inside HIP graph
+// capture, allocation-free serving loops).  Each GPU holds its own copy for
+// the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host-side
+// stand-in, not a device address).
+constexpr int kScratchDoubles = 1 << 18;  // 2 MiB: energy partials of up to 262144 tiles
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+// Resolves the scratch array's address on device ``dev`` (once per device,
+// with that device current); 0 on success.
+static int scratch_setup(int dev, std::string* err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    if (err) *err = "device id out of range for the static scratch table";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) {
+    if (err) *err = "hipGetDevice failed";
+    return -2;
+  }
+  if (cur != dev && hipSetDevice(dev) != hipSuccess) {
+    if (err) *err = "hipSetDevice failed";
+    return -2;
+  }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != dev) hipSetDevice(cur);
+  if (e != hipSuccess || !ptr) {
+    if (err) *err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    return -2;
+  }
+  g_scratch[dev] = (double*)ptr;
+  return 0;
+}
 
 struct ss_ctx {
   int theory = 0;
   const TheoryOps* ops = nullptr;
   int prec = 0;
   int var_mode = 0;
+  int dev = 0;  // the device current at creation
   int32_t pi[kMaxPI] = {0};
   int ni = 0;
   double pf[kMaxPF] = {0};
@@ -951,6 +994,18 @@ int ss_ctx_create(int theory_id, const int32_t* p_i, int n_i, const double* p_f,
     ss_ctx_destroy(c);
     return rc;
   };
+  if (hipGetDevice(&c->dev) != hipSuccess) {
+    hipfail(c, "hipGetDevice failed");
+    return bail(-2);
+  }
+  {  // set-up of the allocation-free entry point's scratch on this device
+    std::string e;
+    const int rc = scratch_setup(c->dev, &e);
+    if (rc) {
+      fail(c, "%s", e.c_str());
+      return bail(rc);
+    }
+  }
   if (resolve_geom(c, p_i, n_i, p_f, n_f, &c->g)) return bail(-1);
   if (c->g.nf != ops->nf) {
     fail(c, "theory %s is built for %d fields (number_total_fields=%d)", ops->name, ops->nf, c->g.nf);
@@ -1743,6 +1798,29 @@ int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* st
   hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->nt0, energy_dev);
   HIPCHK(c, hipGetLastError());
   return dev_call_end(c, st);
+}
+
+int ss_energy_dev_noalloc(ss_ctx* c, const void* field_dev, double* energy_dev, void* stream) {
+  if (!c) return fail(nullptr, "null context");
+  if (!field_dev || !energy_dev) return fail(c, "null device pointer");
+  if (c->prec != SS_PREC_FP64)
+    return fail(c, "device-pointer calls take FP64 fields: the context is in precision %d", c->prec);
+  if (c->nt0 > kScratchDoubles)
+    return fail(c, "ss_energy_dev_noalloc: the lattice needs %d energy partials, the static scratch holds %d "
+                   "(%d x %d sites is too large for it; use ss_energy_dev)",
+                c->nt0, kScratchDoubles, c->g.X, c->g.Y);
+  double* part = (c->dev >= 0 && c->dev < kMaxDevices) ? g_scratch[c->dev] : nullptr;
+  if (!part) {  // resolved by ss_ctx_create; kept as a check
+    std::string e;
+    const int rc = scratch_setup(c->dev, &e);
+    if (rc) return fail(c, "%s", e.c_str());
+    part = g_scratch[c->dev];
+  }
+  hipStream_t st = (hipStream_t)stream;  // NULL is the null stream, as for any HIP call
+  HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, part, c->tp, c->g, st));
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)part, c->nt0, energy_dev);
+  HIPCHK(c, hipGetLastError());
+  return 0;
 }
 
 int ss_synchronize(ss_ctx* c) {
