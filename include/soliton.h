@@ -188,6 +188,21 @@ int ss_variation_dev(ss_ctx* ctx, const void* field_dev, void* grad_dev, int mod
 /* Device-pointer energy: writes the FP64 total into energy_dev[0]. */
 int ss_energy_dev(ss_ctx* ctx, const void* field_dev, double* energy_dev, void* stream);
 
+/* Parameter derivatives of the discrete energy at a fixed FP64 field (device
+ * pointer, reference layout): grad_dev[j] = dE_h/dp_f[j] for the theory's
+ * packed coupling constants p_f[6 ..] (n = the packed p_f length; entries 0..5,
+ * the lattice geometry, are written as 0).  Exact: forward-mode dual numbers
+ * seeded at p_f[j] run through the theory's own packed-parameter derivation
+ * and energy density (anisotropic_superconductor/kernels.py:96-170 with the
+ * vacuum constant of :129-168), summed in FP64.  The derivative with respect
+ * to a user parameter chains through the host's ResolvedParams derivation
+ * (e.g. anisotropic_superconductor/params.py:190-243: u1, u2, theta_diff,
+ * beta3tilde).  Available for the anisotropic superconductor; other theories
+ * fail (-1).  FP64 contexts only; ordered against the context's stream like
+ * ss_energy_dev; allocates its partials on first use.  (The reference has no
+ * parameter derivatives: it is the backward of the differentiable energy.) */
+int ss_energy_param_grad(ss_ctx* ctx, const void* field_dev, double* grad_dev, int n, void* stream);
+
 /* ss_energy_dev without any allocation and without the context's scratch:
  * same result (the same kernels and summation order), with the per-tile
  * energy partials in a library-held static device array of 2 MiB (262144

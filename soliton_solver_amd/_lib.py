@@ -81,6 +81,7 @@ SIGNATURES = {
     "ss_variation_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ss_energy_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ss_energy_dev_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ss_energy_param_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ss_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_prepare_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_synchronize": (C.c_int, [C.c_void_p]),

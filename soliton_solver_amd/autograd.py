@@ -12,8 +12,16 @@ PyTorch is used only at this boundary: tensors are FP64 CUDA tensors in the
 reference layout ``(nf, xlen, ylen)``; the kernels run on torch's current
 stream through ``ss_energy_dev`` / ``ss_variation_dev``.
 
-Parameter sensitivities at a relaxed minimum phi* (dE*/dtheta = dE_h/dtheta
-at phi*, by stationarity) are provided by ``energy_parameter_gradient``.
+Parameters can be inputs of the same Function: ``discrete_energy(field, sim,
+theta, names)`` takes a float64 tensor of named user parameters (e.g.
+``["beta1", "beta2"]``), and its backward returns dE_h/dtheta as well.  That
+gradient is exact: the ``ss_energy_param_grad`` kernel differentiates the
+energy density with respect to every packed coupling constant p_f[j] with
+forward-mode dual numbers (through the theory's own derivation of its
+couplings from p_f), and the chain to the user parameters runs through the
+host's ResolvedParams derivation (``packed_parameter_jacobian``).  At a
+relaxed minimum phi* (dE_h/dphi = 0) it is dE*/dtheta (envelope theorem):
+``energy_parameter_gradient``.
 """
 from __future__ import annotations
 
@@ -66,50 +74,88 @@ def reference_variation(sim, field: torch.Tensor) -> torch.Tensor:
     return g
 
 
+def packed_gradient(sim, field: torch.Tensor) -> torch.Tensor:
+    """dE_h/dp_f (one entry per packed p_f value; the lattice-geometry entries
+    0..5 are 0) at ``field``, from ``ss_energy_param_grad``."""
+    f = _check_field(sim, field)
+    n = int(sim.p_f_h.size)
+    out = torch.empty(n, dtype=torch.float64, device=f.device)
+    _lib.check(_lib.lib().ss_energy_param_grad(sim._ctx, C.c_void_p(f.data_ptr()), C.c_void_p(out.data_ptr()), n,
+                                               C.c_void_p(_stream_ptr(f.device))), sim._ctx)
+    return out
+
+
+def packed_parameter_jacobian(sim, names, rel_step: float = 1e-6) -> np.ndarray:
+    """J[j, k] = dp_f[j] / dtheta_k of the theory's packing
+    (``pack_device_params(params.resolved())``) at the simulation's parameters,
+    host-side: entries that copy a parameter are exactly 1 (the central
+    difference of an identity), derived ones (e.g. the anisotropic theory's u1,
+    u2 from alpha, beta: anisotropic_superconductor/params.py:190-243) are the
+    central difference of that closed-form algebra at step rel_step |theta_k|
+    (truncation ~ step^2, rounding ~ 1e-16/step: ~1e-10 relative)."""
+    base = sim.p
+    th = sim.theory
+    J = np.zeros((int(sim.p_f_h.size), len(names)))
+    for k, name in enumerate(names):
+        v = float(getattr(base, name))
+        h = rel_step * max(abs(v), 1.0)
+        _, pf_p = th.params.pack_device_params(base.with_(**{name: v + h}).resolved())
+        _, pf_m = th.params.pack_device_params(base.with_(**{name: v - h}).resolved())
+        J[:, k] = (np.asarray(pf_p, dtype=np.float64) - np.asarray(pf_m, dtype=np.float64)) / (2.0 * h)
+    J[:6, :] = 0.0  # the lattice geometry does not depend on the couplings
+    return J
+
+
 class DiscreteEnergy(torch.autograd.Function):
-    """``E = DiscreteEnergy.apply(field, sim)``; ``E.backward()`` gives dE_h/dfield."""
+    """``E = DiscreteEnergy.apply(field, sim, theta, names)``; ``E.backward()``
+    gives dE_h/dfield and, when ``theta`` (a float64 tensor of the values of
+    the user parameters ``names``, equal to the simulation's own) requires
+    grad, dE_h/dtheta."""
 
     @staticmethod
-    def forward(ctx, field, sim):
+    def forward(ctx, field, sim, theta=None, names=None):
         ctx.sim = sim
+        ctx.names = list(names) if names is not None else []
+        if theta is not None:
+            cur = [float(getattr(sim.p, n)) for n in ctx.names]
+            if theta.numel() != len(cur) or any(abs(a - b) > 1e-15 * max(1.0, abs(b))
+                                                for a, b in zip(theta.detach().cpu().tolist(), cur)):
+                raise ValueError("theta must hold the simulation's current values of the named parameters "
+                                 "(set them with sim.set_params)")
         ctx.save_for_backward(field)
         return energy_of(sim, field)
 
     @staticmethod
     def backward(ctx, grad_out):
         (field,) = ctx.saved_tensors
-        g = exact_gradient(ctx.sim, field.detach())
-        return grad_out * g, None
+        g_field = None
+        if ctx.needs_input_grad[0]:
+            g_field = grad_out * exact_gradient(ctx.sim, field.detach())
+        g_theta = None
+        if len(ctx.needs_input_grad) > 2 and ctx.needs_input_grad[2]:
+            dpf = packed_gradient(ctx.sim, field.detach()).cpu().numpy()
+            J = packed_parameter_jacobian(ctx.sim, ctx.names)
+            g_theta = grad_out * torch.tensor(J.T @ dpf, dtype=torch.float64, device=grad_out.device)
+        return g_field, None, g_theta, None
 
 
-def discrete_energy(field: torch.Tensor, sim) -> torch.Tensor:
-    return DiscreteEnergy.apply(field, sim)
+def discrete_energy(field: torch.Tensor, sim, theta: torch.Tensor | None = None, names=None) -> torch.Tensor:
+    return DiscreteEnergy.apply(field, sim, theta, names)
 
 
-def energy_parameter_gradient(sim, field: torch.Tensor, names, rel_step: float = 1e-3):
-    """dE_h/dtheta at fixed ``field`` for named theory parameters.
-
-    Dependent packed values (e.g. the anisotropic theory's u1, u2, theta_diff,
-    beta3tilde) are re-derived by the theory's own ``ResolvedParams``, so the
-    derivative includes their dependence on theta.  It is the 5-point central
-    difference in theta of the FP64 GPU energy at fixed field, step
-    ``rel_step*|theta|`` (truncation O(step^4), about 1e-12 relative for the
-    smooth theta-dependence of these energies).  At a relaxed minimum phi*,
-    dE_h/dphi = 0, so this is dE*/dtheta (envelope theorem).
-    """
-    base = sim.p
-    out = {}
-    for name in names:
-        v = float(getattr(base, name))
-        h = rel_step * max(abs(v), 1e-3)
-        vals = []
-        for k in (-2, -1, 1, 2):
-            sim.set_params(base.with_(**{name: v + k * h}))
-            vals.append(float(energy_of(sim, field)))
-        sim.set_params(base)
-        out[name] = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
-    return out
+def energy_parameter_gradient(sim, field: torch.Tensor, names):
+    """dE_h/dtheta at fixed ``field`` for the named user parameters, by the
+    backward of ``DiscreteEnergy``.  Dependent packed values (e.g. the
+    anisotropic theory's u1, u2, theta_diff, beta3tilde) are chained through
+    the theory's own ``ResolvedParams``, so the derivative includes their
+    dependence on theta.  At a relaxed minimum phi*, dE_h/dphi = 0, so this is
+    dE*/dtheta (envelope theorem).  Returns {name: value}."""
+    theta = torch.tensor([float(getattr(sim.p, n)) for n in names], dtype=torch.float64, device=field.device,
+                         requires_grad=True)
+    E = discrete_energy(field.detach(), sim, theta, names)
+    (g,) = torch.autograd.grad(E, theta)
+    return {n: float(v) for n, v in zip(names, g.cpu().tolist())}
 
 
 __all__ = ["DiscreteEnergy", "discrete_energy", "energy_of", "exact_gradient", "reference_variation",
-           "energy_parameter_gradient"]
+           "packed_gradient", "packed_parameter_jacobian", "energy_parameter_gradient"]

@@ -409,6 +409,92 @@ density_pass(const void* Fv, double* dens, double* part, int kind, typename TH::
   if (threadIdx.x == 0) part[t] = s;
 }
 
+// Packed parameters by value (kernel argument).
+struct PackedPF {
+  double v[kMaxPF];
+};
+
+// Theories whose coupling constants are built by fill_tp in any scalar type
+// (kParamGrad): the energy can be differentiated with respect to p_f.
+template <class T, class = void>
+struct HasParamGrad {
+  static constexpr bool value = false;
+};
+template <class T>
+struct HasParamGrad<T, decltype((void)T::kParamGrad)> {
+  static constexpr bool value = T::kParamGrad;
+};
+
+// dE_h/dp_f[j] for j = j0 .. j0+nj-1 at the FP64 field Fv: forward-mode dual
+// numbers seeded at p_f[j] through the theory's own parameter derivation
+// (fill_tp) and energy density, field fixed.  Per tile and parameter one FP64
+// partial: part[jj * ntiles + t].
+template <class TH>
+__global__ void __launch_bounds__(kThreads)
+param_grad_pass(const double* Fv, double* part, PackedPF pf, int j0, int nj, Geom g, Launch L) {
+  if constexpr (HasParamGrad<TH>::value) {
+    using S = TileShape<double, TH::NF>;
+    constexpr int NF = TH::NF, NP = TH::kNumPF;
+    __shared__ Tile<double, NF> tile;
+    __shared__ double red[kWaves];
+    const int t = xcd_tile(blockIdx.x, L.ntiles, L.per_xcd);
+    if (t < 0) return;
+    const int tx = t / L.ntiles_y, ty = t - tx * L.ntiles_y;
+    const int x0 = tx * S::TX, y0 = ty * TY;
+    load_tile<double, NF>(tile, g, x0, y0, [&](int a, long long off) -> double { return Fv[off]; });
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = j0 + jj;
+      Dual<1> pd[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        pd[k] = Dual<1>(pf.v[k]);
+        pd[k].d[0] = (k == j) ? 1.0 : 0.0;
+      }
+      typename TH::template TPT<Dual<1>> tpd;
+      TH::template fill_tp<Dual<1>>(pd, tpd);
+      double acc = 0.0;
+      for (int k = 0; k < S::kRowsPerWave; ++k) {
+        const int lx = w * S::kRowsPerWave + k, ly = lane;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x < g.X && y < g.Y) {
+          const SiteInfo s = site_info(g, x, y);
+          double fd[NF], d1d[NF][2];
+          tile_d1<double, double, NF>(tile, lx, ly, g, s, fd, d1d);
+          Dual<1> f[NF], d1[NF][2];
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            f[a] = Dual<1>(fd[a]);
+            d1[a][0] = Dual<1>(d1d[a][0]);
+            d1[a][1] = Dual<1>(d1d[a][1]);
+          }
+          acc += TH::template energy<Dual<1>>(tpd, g, s, f, d1).d[0] * g.gv;
+        }
+      }
+      const double sum = block_sum<kThreads>(acc, red);
+      if (threadIdx.x == 0) part[(long long)jj * L.ntiles + t] = sum;
+    }
+  }
+}
+
+// out[jj] = Kahan sum of the nt partials of parameter jj (one block of NT
+// threads each; a template so that every translation unit may instantiate it).
+template <int NT>
+__global__ void __launch_bounds__(NT) param_grad_fin(const double* part, int nt, double* out) {
+  __shared__ double red[NT / 64];
+  const double* p = part + (long long)blockIdx.x * nt;
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nt; i += NT) {
+    const double y = p[i] - c;
+    const double tt = s + y;
+    c = (tt - s) - y;
+    s = tt;
+  }
+  const double r = block_sum<NT>(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 // ---------------------------------------------------------------------------
 // Exact adjoint of the discrete energy E_h = sum_s gv e(f_s, D1 f_s).
 //
@@ -1105,6 +1191,30 @@ struct TheoryLaunchers {
                 : density_r<double>(kind, F, dens, part, *(const TP*)tp, g, st);
   }
 
+  // dE_h/dp_f[j] for j in [j0, j0 + nj) into out[0 .. nj) (FP64 field; the
+  // theory must provide fill_tp).  part holds nj * param_grad_tiles(g) doubles.
+  static int param_grad_tiles(const Geom& g) { return make_launch<double, TH::NF>(g).ntiles; }
+  static int param_grad_count() {
+    if constexpr (HasParamGrad<TH>::value) return TH::kNumPF;
+    return 0;
+  }
+  static hipError_t param_grad(const double* F, double* part, double* out, const double* pf, int j0, int nj,
+                               const Geom& g, hipStream_t st) {
+    if constexpr (HasParamGrad<TH>::value) {
+      if (j0 < 0 || nj < 1 || j0 + nj > TH::kNumPF) return hipErrorInvalidValue;
+      const Launch L = make_launch<double, TH::NF>(g);
+      PackedPF P{};
+      for (int k = 0; k < TH::kNumPF; ++k) P.v[k] = pf[k];
+      hipLaunchKernelGGL((param_grad_pass<TH>), dim3(L.grid), dim3(kThreads), 0, st, F, part, P, j0, nj, g, L);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(param_grad_fin<256>, dim3(nj), dim3(256), 0, st, (const double*)part, L.ntiles, out);
+      return hipGetLastError();
+    } else {
+      return hipErrorNotSupported;
+    }
+  }
+
   // Exact adjoint of the energy of Fv (FP64 storage) into G, scratch Px, Py
   // (nf*N doubles each).  With ``flow`` (storage precision ``prec``) also the
   // flow convention against Tv and l = -dt EG into lout.
@@ -1309,6 +1419,9 @@ struct TheoryOps {
   hipError_t (*batch_pass)(int, int, int, const BatchBufs&, hipStream_t);
   void (*batch_shape)(int, int, int*, int*);
   hipError_t (*batch_fin)(int, int, const BatchBufs&, int, double, hipStream_t);
+  int (*param_grad_tiles)(const Geom&);
+  int (*param_grad_count)();
+  hipError_t (*param_grad)(const double*, double*, double*, const double*, int, int, const Geom&, hipStream_t);
   int l2_normalize;  // the flow L2-normalises stage fields (unfused normalised path)
 };
 
@@ -1340,6 +1453,9 @@ struct TheoryOps {
                                       &ss::TheoryLaunchers<TH>::batch_pass,                 \
                                       &ss::TheoryLaunchers<TH>::batch_shape,                \
                                       &ss::TheoryLaunchers<TH>::batch_fin,                  \
+                                      &ss::TheoryLaunchers<TH>::param_grad_tiles,           \
+                                      &ss::TheoryLaunchers<TH>::param_grad_count,           \
+                                      &ss::TheoryLaunchers<TH>::param_grad,                 \
                                       TH::kL2Normalize ? 1 : 0};                            \
     return &ops;                                                                            \
   }

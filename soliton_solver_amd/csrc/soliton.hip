@@ -275,6 +275,10 @@ static std::mutex g_scratch_mu;
 
 // Resolves the scratch array's address on device ``dev`` (once per device,
 // with that device current); 0 on success.
+// Packed p_f entries 0..5 are the lattice geometry (core/params.py:247-248);
+// the theories' coupling constants follow.
+constexpr int kCorePF = 6;
+
 static int scratch_setup(int dev, std::string* err) {
   if (dev < 0 || dev >= kMaxDevices) {
     if (err) *err = "device id out of range for the static scratch table";
@@ -346,6 +350,8 @@ struct ss_ctx {
   int graph_steps = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // indexed by starting parity
   hipEvent_t ev_dev = nullptr;  // orders the device-pointer calls against the context's stream
+  double* pgpart = nullptr;     // partials of ss_energy_param_grad (allocated on first use)
+  size_t pgpart_n = 0;
   std::string err;
 };
 
@@ -1068,6 +1074,7 @@ int ss_ctx_destroy(ss_ctx* c) {
   if (c->Py) hipFree(c->Py);
   if (c->xpart) hipFree(c->xpart);
   if (c->ev_dev) hipEventDestroy(c->ev_dev);
+  if (c->pgpart) hipFree(c->pgpart);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1797,6 +1804,37 @@ int ss_energy_dev(ss_ctx* c, const void* field_dev, double* energy_dev, void* st
   HIPCHK(c, c->ops->density(c->prec, -1, field_dev, nullptr, c->part, c->tp, c->g, st));
   hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, st, (const double*)c->part, c->nt0, energy_dev);
   HIPCHK(c, hipGetLastError());
+  return dev_call_end(c, st);
+}
+
+__global__ void k_fill_zero(double* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+int ss_energy_param_grad(ss_ctx* c, const void* field_dev, double* grad_dev, int n, void* stream) {
+  if (!c) return fail(nullptr, "null context");
+  if (!field_dev || !grad_dev) return fail(c, "null device pointer");
+  if (n != c->nfl) return fail(c, "ss_energy_param_grad: n must be the packed p_f length %d (got %d)", c->nfl, n);
+  const int np = c->ops->param_grad_count();
+  if (np <= kCorePF) return fail(c, "parameter derivatives of the energy are not available for theory %s", c->ops->name);
+  if (np > c->nfl) return fail(c, "theory %s packs %d p_f entries, the context holds %d", c->ops->name, np, c->nfl);
+  const size_t need = (size_t)(np - kCorePF) * (size_t)c->ops->param_grad_tiles(c->g);
+  if (c->pgpart_n < need) {
+    if (c->pgpart) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      hipFree(c->pgpart);
+    }
+    c->pgpart = nullptr;
+    c->pgpart_n = 0;
+    HIPCHK(c, hipMalloc(&c->pgpart, need * sizeof(double)));
+    c->pgpart_n = need;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (dev_call_begin(c, st)) return -1;
+  hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, st, grad_dev, n);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, c->ops->param_grad((const double*)field_dev, c->pgpart, grad_dev + kCorePF, c->pf, kCorePF,
+                               np - kCorePF, c->g, st));
   return dev_call_end(c, st);
 }
 

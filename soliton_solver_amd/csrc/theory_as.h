@@ -19,13 +19,46 @@ struct AnisotropicSC {
   static constexpr int kNumCharges = 1;  // 0 vortex number
   static constexpr const char* kName = "anisotropic_superconductor";
 
-  struct TP {
-    double kappa, Q, a1, a2, b1, b2, b3, b4;
-    double G[2][2][2];  // G[i][a][b] = gamma_entry(i, i, a, b)
-    double vac;         // vacuum constant subtracted per site (kernels.py:168)
+  // The coupling constants in scalar type P: double for the flow, dual
+  // numbers for the derivatives of the energy with respect to the packed
+  // parameters (energy_param_grad).
+  template <typename P>
+  struct TPT {
+    P kappa, Q, a1, a2, b1, b2, b3, b4;
+    P G[2][2][2];  // G[i][a][b] = gamma_entry(i, i, a, b)
+    P vac;         // vacuum constant subtracted per site (kernels.py:168)
   };
+  using TP = TPT<double>;
+  static constexpr int kNumPF = 25;  // packed p_f entries (anisotropic_superconductor/params.py:287-308)
+  static constexpr bool kParamGrad = true;
 
-  // Packing: anisotropic_superconductor/params.py:266-313.
+  // Packed parameters -> coupling constants (anisotropic_superconductor/
+  // params.py:266-313; gamma_entry kernels.py:45-94; vacuum constant
+  // kernels.py:140-168).
+  template <typename P>
+  __host__ __device__ static void fill_tp(const P* pf, TPT<P>& tp) {
+    tp.kappa = pf[6];
+    tp.Q = pf[7];
+    tp.a1 = pf[8];
+    tp.a2 = pf[9];
+    tp.b1 = pf[10];
+    tp.b2 = pf[11];
+    tp.b3 = pf[12];
+    tp.b4 = pf[13];
+    const P g1 = pf[14], g2 = pf[15], g12 = pf[16];
+    tp.G[0][0][0] = P(2.0) * g1;
+    tp.G[0][0][1] = -(P(2.0) * g12);
+    tp.G[0][1][0] = -(P(2.0) * g12);
+    tp.G[0][1][1] = P(2.0) * g2;
+    tp.G[1][0][0] = P(2.0) * g1;
+    tp.G[1][0][1] = P(2.0) * g12;
+    tp.G[1][1][0] = P(2.0) * g12;
+    tp.G[1][1][1] = P(2.0) * g2;
+    const P u1 = pf[17], u2 = pf[18], th = pf[19];
+    tp.vac = tp.a1 * u1 * u1 + tp.a2 * u2 * u2 + tp.b1 * u1 * u1 * u1 * u1 + tp.b2 * u2 * u2 * u2 * u2 +
+             tp.b3 * u1 * u1 * u2 * u2 + P(2.0) * tp.b4 * u1 * u1 * u2 * u2 * cos(P(2.0) * th);
+  }
+
   static int make(const int32_t* pi, int ni, const double* pf, int nfl, TP* tp, char* msg, int cap) {
     if (ni < 14 || nfl < 25) {
       snprintf(msg, cap, "anisotropic superconductor needs p_i[0..13] and p_f[0..24] (got %d, %d)", ni, nfl);
@@ -35,26 +68,7 @@ struct AnisotropicSC {
       snprintf(msg, cap, "anisotropic superconductor kernels are built for 6 fields (got %d)", pi[4]);
       return -1;
     }
-    tp->kappa = pf[6];
-    tp->Q = pf[7];
-    tp->a1 = pf[8];
-    tp->a2 = pf[9];
-    tp->b1 = pf[10];
-    tp->b2 = pf[11];
-    tp->b3 = pf[12];
-    tp->b4 = pf[13];
-    const double g1 = pf[14], g2 = pf[15], g12 = pf[16];
-    tp->G[0][0][0] = 2.0 * g1;
-    tp->G[0][0][1] = -2.0 * g12;
-    tp->G[0][1][0] = -2.0 * g12;
-    tp->G[0][1][1] = 2.0 * g2;
-    tp->G[1][0][0] = 2.0 * g1;
-    tp->G[1][0][1] = 2.0 * g12;
-    tp->G[1][1][0] = 2.0 * g12;
-    tp->G[1][1][1] = 2.0 * g2;
-    const double u1 = pf[17], u2 = pf[18], th = pf[19];
-    tp->vac = tp->a1 * u1 * u1 + tp->a2 * u2 * u2 + tp->b1 * u1 * u1 * u1 * u1 + tp->b2 * u2 * u2 * u2 * u2 +
-              tp->b3 * u1 * u1 * u2 * u2 + 2.0 * tp->b4 * u1 * u1 * u2 * u2 * cos(2.0 * th);
+    fill_tp<double>(pf, *tp);
     return 0;
   }
 
@@ -114,9 +128,10 @@ struct AnisotropicSC {
     }
   }
 
-  // kernels.py:96-170, without the gv factor.
-  template <typename T>
-  __device__ __forceinline__ static T energy(const TP& tp, const Geom& g, const SiteInfo& s,
+  // kernels.py:96-170, without the gv factor.  TPX: TP, or TPT<T> for
+  // derivatives with respect to the coupling constants.
+  template <typename T, class TPX = TP>
+  __device__ __forceinline__ static T energy(const TPX& tp, const Geom& g, const SiteInfo& s,
                                              const T f[NF], const T d1[NF][2]) {
     const T k = T(tp.kappa), Q = T(tp.Q);
     const T p1r = f[2], p1i = f[3], p2r = f[4], p2i = f[5];

@@ -1,0 +1,176 @@
+"""Parity of the full-size workloads (GPU).
+
+* The drop-in FP64 workload, chiral magnet 320 x 320 (xsize 10, Heusler DMI,
+  demag, centred anti-skyrmion ansatz + N(0, 1e-3) on m, renormalised),
+  relaxed to max |EG| < 1e-8 on the GPU against the same relaxation by the
+  oracle's C++ restatement of the reference path (tests/golden/
+  chiral320_relaxed.json, made by tests/golden/make_chiral320.py): final
+  energy within 1e-10 relative, identical rounded skyrmion number, and the
+  largest difference of m over a 20 x 20 sample of sites stated.
+* The precision form (compensated FP32 fields and stencils, FP64 energy and
+  arrest, FP64 polish) relaxing a 256 x 256 lattice at the precision
+  workloads' spacing to max |EG| < 1e-6 against the C++ restatement's FP64
+  relaxation (tests/golden/chiral256_relaxed.json): energy within 1e-7
+  relative, identical rounded skyrmion number.
+* Simulation.minimize_fast at 2048 x 2048 (one anti-skyrmion, demag off)
+  against the plain FP64 GPU form from the same field, both to 1e-6: energy
+  within 1e-7 relative, identical rounded skyrmion number.
+* The 4096 x 4096 throughput workload itself (8 seeded anti-skyrmions, demag
+  on): the FP32 and compensated forms against the FP64 form over the same 400
+  steps from the same field (a mid-relaxation comparison; that workload's
+  demagnetising saddle field keeps any form from converging in a bounded run).
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import soliton_solver_amd as ssa
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+def _sample(F, rc):
+    s = slice(*rc)
+    return F[:3, s, s]
+
+
+def test_chiral_320_matches_cpu_restatement_relaxed_to_1e_8():
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_chiral320 as g
+    ref = _golden("chiral320_relaxed.json")
+    th, p, p_i, p_f = g.params()
+    F0 = g.initial_field(p_i, p_f)
+    sim = ssa.Simulation(p, th)
+    try:
+        sim.initialize({"mode": "array", "field": F0})
+        E0 = sim.compute_energy()
+        assert H.rel(E0, ref["initial"]["energy"]) < 1e-12
+        sim.use_graphs(20)
+        E, ea, n = sim.minimize(tol=1e-8, max_steps=2 * ref["steps"] + 100000, check_every=5000, prev_energy=E0)
+        Q = sim.observables()["skyrmion_number"]
+        F = sim.get_field()
+        dm = float(np.abs(_sample(F, ref["sample_rows_cols"]) - np.array(ref["sample_m"])).max())
+        print(f"[chiral 320 vs C++ restatement] GPU: {n} steps E={E:.15f} Q={Q:.9f} max|EG|={ea:.2e}; "
+              f"C++: {ref['steps']} steps E={ref['energy']:.15f} Q={ref['skyrmion_number']:.9f}; "
+              f"relE={H.rel(E, ref['energy']):.2e} max|dm| (400 sites)={dm:.2e}", flush=True)
+        assert ea < 1e-8
+        assert H.rel(E, ref["energy"]) < 1e-10
+        assert round(Q) == round(ref["skyrmion_number"])
+        assert dm < 1e-4
+    finally:
+        sim.close()
+
+
+def test_fp32c_relaxation_matches_cpu_restatement_at_256():
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_chiral256 as g
+    ref = _golden("chiral256_relaxed.json")
+    th, p, p_i, p_f = g.params()
+    F0 = g.initial_field(p_i, p_f)
+    sim = ssa.Simulation(p, th, precision="fp32c")
+    try:
+        sim.initialize({"mode": "array", "field": F0})
+        sim.set_tile_variants([3, 3, 3, 3])
+        E, ea, n32, n64 = sim.minimize_fast(tol=1e-6, max_steps=4 * ref["steps"] + 200000, chunk=10000)
+        Q = sim.observables()["skyrmion_number"]
+        dm = float(np.abs(_sample(sim.get_field(), ref["sample_rows_cols"]) - np.array(ref["sample_m"])).max())
+        print(f"[fp32c 256 vs C++ restatement] GPU: {n32} fp32c + {n64} fp64 steps E={E:.15f} Q={Q:.9f} "
+              f"max|EG|={ea:.2e}; C++ FP64: {ref['steps']} steps E={ref['energy']:.15f}; "
+              f"relE={H.rel(E, ref['energy']):.2e} max|dm| (256 sites)={dm:.2e}", flush=True)
+        assert ea < 1e-6
+        assert H.rel(E, ref["energy"]) < 1e-7
+        assert round(Q) == round(ref["skyrmion_number"])
+    finally:
+        sim.close()
+
+
+def _workload(n, nsk, demag, seed):
+    """The precision workloads' recipe: h = 10/319, Heusler DMI, nsk
+    anti-skyrmions with centres uniform in the central 80 % and radii uniform
+    in [0.3, 0.6], N(0, 1e-3) on m, renormalised."""
+    th = ssa.load_theory("Chiral magnet")
+    h = 10.0 / 319.0
+    L = h * (n - 1)
+    p = th.params.default_params(xlen=n, ylen=n, xsize=L, ysize=L, J=40e-12, K=0.8e6, D=4e-3, M=580e3, B=0.0,
+                                 dmi_term="Heusler", demag=demag, unit_magnetization=True)
+    rng = np.random.default_rng(seed)
+    sk = [(rng.uniform(0.1 * L, 0.9 * L), rng.uniform(0.1 * L, 0.9 * L), rng.uniform(0.3, 0.6)) for _ in range(nsk)]
+    return th, p, {"mode": "skyrmions", "ansatz": "anti", "skyrmions": sk, "noise": 1e-3, "seed": seed + 1}
+
+
+def test_minimize_fast_2048_matches_fp64_form():
+    th, p, cfg = _workload(2048, 1, False, 4)
+    a = ssa.Simulation(p, th)
+    try:
+        a.initialize(cfg)
+        F0 = a.get_field()
+        a.autotune_tiles(reps=2)
+        a.use_graphs(20)
+        Ea, eaa, na = a.minimize(tol=1e-6, max_steps=300000, check_every=500)
+        Fa = a.get_field()
+        Qa = a.observables()["skyrmion_number"]
+    finally:
+        a.close()
+    b = ssa.Simulation(p, th, precision="fp32c")
+    try:
+        b.initialize({"mode": "array", "field": F0})
+        Eb, eab, n32, n64 = b.minimize_fast(tol=1e-6, max_steps=300000, autotune=True)
+        Qb = b.observables()["skyrmion_number"]
+        dm = float(np.abs(b.get_field()[:3] - Fa[:3]).max())
+        print(f"[2048 minimize_fast] fp64 form: {na} steps E={Ea:.13f} Q={Qa:.7f}; fp32c: {n32} + {n64} fp64 "
+              f"steps E={Eb:.13f} Q={Qb:.7f}; relE={H.rel(Eb, Ea):.2e} max|dm|={dm:.2e}", flush=True)
+        assert eaa < 1e-6 and eab < 1e-6
+        assert H.rel(Eb, Ea) < 1e-7
+        assert round(Qa) == round(Qb)
+    finally:
+        b.close()
+
+
+def test_precision_forms_track_fp64_on_the_4096_throughput_workload():
+    th, p, cfg = _workload(4096, 8, True, 0)
+    sims = {}
+    out = {}
+    try:
+        s64 = ssa.Simulation(p, th)
+        sims["fp64"] = s64
+        s64.initialize(cfg)
+        F0 = s64.get_field()
+        Q0 = s64.observables()["skyrmion_number"]
+        for form in ("fp32", "fp32c"):
+            s = ssa.Simulation(p, th, precision=form)
+            sims[form] = s
+            s.initialize({"mode": "array", "field": F0})
+        del F0
+        for form, s in sims.items():
+            s.set_tile_variants([3, 3, 3, 3])
+            s.use_graphs(20)
+            E0 = s.compute_energy()
+            E, _, ea = s.steps(E0, 400)
+            out[form] = (E, ea, s.observables()["skyrmion_number"])
+        F64 = sims["fp64"].get_field()
+        for form in ("fp32", "fp32c"):
+            E, ea, Q = out[form]
+            dm = float(np.abs(sims[form].get_field()[:3] - F64[:3]).max())
+            print(f"[4096 workload, 400 steps] {form}: E={E:.12f} relE vs fp64={H.rel(E, out['fp64'][0]):.2e} "
+                  f"max|EG|={ea:.2e} Q={Q:.6f} (fp64 Q={out['fp64'][2]:.6f}, initial {Q0:.6f}) max|dm|={dm:.2e}",
+                  flush=True)
+            assert round(Q) == round(out["fp64"][2]) == round(Q0)
+        assert H.rel(out["fp32c"][0], out["fp64"][0]) < 1e-7
+        assert H.rel(out["fp32"][0], out["fp64"][0]) < 1e-5
+    finally:
+        for s in sims.values():
+            s.close()
