@@ -329,6 +329,15 @@ int ss_batch_num_tiles(ss_batch* b);
 int ss_batch_upload(ss_batch* b, int inst, int buf, const double* host, size_t n);
 int ss_batch_download(ss_batch* b, int inst, int buf, double* host, size_t n);
 
+/* Switch the batch's storage precision in place (SS_PREC_FP32 <-> SS_PREC_FP64):
+ * fields and velocities of every instance are converted, the per-instance
+ * arrest state is kept; the shape variant and schedule stay (a fused sweep the
+ * new precision lacks falls back to separate passes).  For an FP64 retire
+ * phase after an FP32 relaxation: switch, then ss_batch_begin to reactivate
+ * every instance for the tighter tolerance.  (No counterpart in the reference,
+ * which runs FP64 only.) */
+int ss_batch_set_precision(ss_batch* b, int precision);
+
 /* Start a relaxation: energy of every instance's field becomes its first
  * prev_energy (the reference's viewer loop seeds its E the same way,
  * chiral_magnet/render_gl.py:663-666); all instances active. */

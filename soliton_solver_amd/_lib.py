@@ -104,6 +104,7 @@ SIGNATURES = {
     "ss_batch_upload": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
     "ss_batch_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_size_t]),
     "ss_batch_begin": (C.c_int, [C.c_void_p]),
+    "ss_batch_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_batch_step": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]),
     "ss_batch_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "ss_batch_state": (C.c_int, [C.c_void_p, _f64p, _f64p, _i32p, _i32p]),

@@ -98,7 +98,59 @@ class BatchSimulation:
         self.begin()
 
     def begin(self):
+        """Start (or restart) a relaxation: every instance active, its energy the
+        first prev_energy, step counters at 0."""
         self._check(_lib.lib().ss_batch_begin(self._b))
+
+    def set_precision(self, precision: str) -> None:
+        """Convert every instance's field and velocity to ``precision`` in place
+        ("fp32" <-> "fp64"); the arrest state is kept."""
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        self._check(_lib.lib().ss_batch_set_precision(self._b, _PREC[precision]))
+        self.precision = precision
+
+    def relax(self, tol: float, max_steps: int, check_every: int = 500, reduced_tol: float = 1e-3,
+              reduced_steps: int | None = None, on_chunk=None) -> dict:
+        """Relax every instance to max |EG| < ``tol``: an FP32 phase (when the
+        batch is FP32) to ``reduced_tol`` -- the FP32 floor of max |EG|, set by
+        the field's rounding -- for at most ``reduced_steps`` steps, then an
+        FP64 retire phase (fields converted in place, every instance
+        reactivated) until each reaches ``tol``; at most ``max_steps`` steps in
+        all.  Returns per-instance arrays: steps of each phase, the total step
+        at which the instance converged (-1: not within the budget, -2: failed
+        with a non-finite error) and its last max |EG|."""
+        n = self.ninst
+        steps32 = np.zeros(n, dtype=np.int64)
+        done = 0
+        if self.precision == "fp32" and tol < reduced_tol:
+            cap = max_steps if reduced_steps is None else min(reduced_steps, max_steps)
+            self.begin()
+            active = n
+            while done < cap and active > 0:
+                k = min(10 * check_every, cap - done)
+                active = self.run(k, reduced_tol, check_every=check_every)
+                done += k
+                if on_chunk:
+                    on_chunk("fp32", done, active, self.state())
+            st = self.state()
+            steps32 = st["steps"].astype(np.int64)
+            self.set_precision("fp64")
+            done = int(steps32.max()) if n else 0
+        self.begin()
+        active = n
+        done64 = 0
+        while done + done64 < max_steps and active > 0:
+            k = min(10 * check_every, max_steps - done - done64)
+            active = self.run(k, tol, check_every=check_every)
+            done64 += k
+            if on_chunk:
+                on_chunk(self.precision, done + done64, active, self.state())
+        st = self.state()
+        conv = np.where(st["converged_step"] >= 0, st["converged_step"] + steps32, st["converged_step"])
+        return {"steps_reduced": steps32, "steps_fp64": st["steps"].astype(np.int64), "converged_step": conv,
+                "err_absmax": st["err_absmax"], "energy": st["energy"], "converged": int((conv >= 0).sum()),
+                "failed": int((conv == -2).sum())}
 
     def autotune_tiles(self, reps: int = 10) -> dict:
         """Time the four RK4 passes in each shape variant (0-2 LDS tiles of the

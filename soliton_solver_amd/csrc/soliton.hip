@@ -2072,14 +2072,17 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
     bfail(b, -2, "no HIP device available");
     return bail(-2);
   }
-  for (int v = 0; v < ops->num_shapes; ++v) {
-    int sx = 0, tyw = TY;
-    ops->batch_shape(precision, v, &sx, &tyw);
-    const int tx = sx ? sx : ops->shape_rows(precision, v);
-    int nt = 0;
-    for (const Geom& g : b->geoms) nt += ((g.X + tx - 1) / tx) * ((g.Y + tyw - 1) / tyw);
-    b->ntcap = nt > b->ntcap ? nt : b->ntcap;
-  }
+  // tile-table capacity over every variant in both storage precisions, so
+  // that ss_batch_set_precision needs no new table
+  for (int pr = SS_PREC_FP64; pr <= SS_PREC_FP32; ++pr)
+    for (int v = 0; v < ops->num_shapes; ++v) {
+      int sx = 0, tyw = TY;
+      ops->batch_shape(pr, v, &sx, &tyw);
+      const int tx = sx ? sx : ops->shape_rows(pr, v);
+      int nt = 0;
+      for (const Geom& g : b->geoms) nt += ((g.X + tx - 1) / tx) * ((g.Y + tyw - 1) / tyw);
+      b->ntcap = nt > b->ntcap ? nt : b->ntcap;
+    }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     bfail(b, -2, "hipStreamCreate failed");
     return bail(-2);
@@ -2176,6 +2179,59 @@ int ss_batch_upload(ss_batch* b, int inst, int buf, const double* host, size_t n
 }
 int ss_batch_download(ss_batch* b, int inst, int buf, double* host, size_t n) {
   return batch_copy(b, inst, buf, host, n, false);
+}
+
+int ss_batch_set_precision(ss_batch* b, int precision) {
+  if (!b) return bfail(nullptr, -1, "null batch");
+  if (precision != SS_PREC_FP64 && precision != SS_PREC_FP32)
+    return bfail(b, -1, "the batched form stores FP64 or FP32 fields (got precision %d)", precision);
+  if (precision == b->prec) return 0;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  const size_t n = (size_t)b->total;
+  const size_t bytes = n * elem_size(precision);
+  void* nf[2] = {nullptr, nullptr};
+  void* nv[2] = {nullptr, nullptr};
+  void* nl[3] = {nullptr, nullptr, nullptr};
+  bool ok = true;
+  for (int k = 0; k < 2 && ok; ++k) ok = hipMalloc(&nf[k], bytes) == hipSuccess && hipMalloc(&nv[k], bytes) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k) ok = hipMalloc(&nl[k], bytes) == hipSuccess;
+  if (!ok) {
+    for (int k = 0; k < 2; ++k) {
+      if (nf[k]) hipFree(nf[k]);
+      if (nv[k]) hipFree(nv[k]);
+    }
+    for (int k = 0; k < 3; ++k)
+      if (nl[k]) hipFree(nl[k]);
+    return bfail(b, -2, "device allocation for the precision switch failed (%zu bytes per buffer)", bytes);
+  }
+  const dim3 grid(4096), blk(256);
+  for (int k = 0; k < 2; ++k) {  // both parities: every instance keeps its own buffer parity
+    if (precision == SS_PREC_FP64) {
+      hipLaunchKernelGGL((k_convert<float, double>), grid, blk, 0, b->stream, (const float*)b->F[k], (double*)nf[k],
+                         (long long)n);
+      hipLaunchKernelGGL((k_convert<float, double>), grid, blk, 0, b->stream, (const float*)b->V[k], (double*)nv[k],
+                         (long long)n);
+    } else {
+      hipLaunchKernelGGL((k_convert<double, float>), grid, blk, 0, b->stream, (const double*)b->F[k], (float*)nf[k],
+                         (long long)n);
+      hipLaunchKernelGGL((k_convert<double, float>), grid, blk, 0, b->stream, (const double*)b->V[k], (float*)nv[k],
+                         (long long)n);
+    }
+  }
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int k = 0; k < 2; ++k) {
+    hipFree(b->F[k]);
+    hipFree(b->V[k]);
+    b->F[k] = nf[k];
+    b->V[k] = nv[k];
+  }
+  for (int k = 0; k < 3; ++k) {  // the stage scratch is recomputed by every step
+    hipFree(b->l[k]);
+    b->l[k] = nl[k];
+  }
+  b->prec = precision;
+  return batch_tiles(b, b->tv);  // rows per tile depend on the precision; drops the graph
 }
 
 int ss_batch_begin(ss_batch* b) {

@@ -9,10 +9,16 @@ Workload: 320 x 320, xsize = ysize = 10, J, K, D, M, B = 40e-12, 0.8e6,
 reference's centred anti-skyrmion ansatz (chiral_magnet/initial_config.py:
 183-284) plus a seeded Gaussian perturbation sigma = 1e-3 on m (NumPy
 default_rng(seed).normal, components 1..3 in order), renormalised; psi = 0;
-relaxed until max |stage-4 energy gradient| < 1e-8.
+run for 150 000 arrested-Newton steps.  (The flow crosses max |stage-4
+energy gradient| = 1e-6 near step 80 000 and then sits on a plateau: from step
+140 000 the energy falls by ~4e-13 per 1000 steps while max |EG| stays at
+7.2-7.3e-7 -- the noise-displaced anti-skyrmion slides along a nearly flat
+direction -- so 1e-8 is out of reach of the reference algorithm here; the
+fixture records the state after a fixed step count, which a faithful
+implementation reproduces to the plateau's energy.)
 
 Run from the repository root: ``python tests/golden/make_chiral320.py``
-(about 20-40 minutes on 8 cores).  Writes tests/golden/chiral320_relaxed.json.
+(about 8 minutes on 12 cores of an EPYC host).  Writes tests/golden/chiral320_relaxed.json.
 """
 from __future__ import annotations
 
@@ -27,7 +33,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-N, XSIZE, SEED, SIGMA, TOL = 320, 10.0, 11, 1e-3, 1e-8
+N, XSIZE, SEED, SIGMA, NSTEPS = 320, 10.0, 11, 1e-3, 150000
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "chiral320_relaxed.json")
 SAMPLE = slice(8, N, 16)  # 20 x 20 sample sites
 
@@ -73,14 +79,14 @@ def main():
     init = {"energy": E0, "sum_m": [float(F[a].sum()) for a in range(3)]}
     t0 = time.perf_counter()
     E, steps, ea = E0, 0, np.inf
-    while steps < 2_000_000:
-        Es, errs, eas = cpu_build.chiral_relax_to(p_i, p_f, F, V, E, 20000, TOL)
+    trace = []
+    while steps < NSTEPS:
+        Es, errs, eas = cpu_build.chiral_relax_to(p_i, p_f, F, V, E, min(20000, NSTEPS - steps), 0.0)
         steps += len(Es)
         E, ea = float(Es[-1]), float(eas[-1])
+        trace.append([steps, E, ea])
         print(f"{steps} steps, {time.perf_counter() - t0:.0f} s: E={E:.15f} max|EG|={ea:.3e}", flush=True)
-        if ea < TOL:
-            break
-    out = {"workload": __doc__.split("\n\n")[1].replace("\n", " "), "seed": SEED, "sigma": SIGMA, "tol": TOL,
+    out = {"workload": __doc__.split("\n\n")[1].replace("\n", " "), "seed": SEED, "sigma": SIGMA, "trace": trace,
            "lattice": N, "initial": init, "steps": steps, "energy": E, "err_absmax": ea,
            "sample_rows_cols": [SAMPLE.start, SAMPLE.stop, SAMPLE.step], **summary(F, p_i, p_f),
            "seconds": time.perf_counter() - t0}

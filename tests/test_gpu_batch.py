@@ -250,3 +250,55 @@ def test_batch_fused23_matches_separate_passes(precision):
         b.close()
     print(f"[batch fused23 {precision}] variants run fused: {ran}")
     assert 3 in ran and 6 in ran
+
+
+def test_batch_precision_switch_keeps_the_state():
+    """ss_batch_set_precision: FP32 -> FP64 converts every instance's field and
+    velocity exactly (FP32 values are FP64 values), and FP64 -> FP32 rounds them."""
+    th, params, cfgs = _bs_instances([40, 56, 48], seed=4)
+    batch = BatchSimulation(th, params, precision="fp32")
+    try:
+        batch.initialize(cfgs)
+        batch.run(20, tol=0.0)
+        F32 = [batch.get_field(i) for i in range(3)]
+        batch.set_precision("fp64")
+        assert batch.precision == "fp64"
+        for i in range(3):
+            assert np.array_equal(batch.get_field(i), F32[i])
+        batch.run(10, tol=0.0)  # steps run in FP64 after the switch
+        F64 = [batch.get_field(i) for i in range(3)]
+        batch.set_precision("fp32")
+        for i in range(3):
+            assert np.abs(batch.get_field(i) - F64[i]).max() <= 1e-7 * max(1.0, np.abs(F64[i]).max())
+    finally:
+        batch.close()
+
+
+def test_batch_fp64_retire_phase_reaches_1e_6():
+    """BatchSimulation.relax: FP32 to the FP32 floor (1e-3), then the FP64
+    retire phase to max |EG| < 1e-6, against single-lattice FP64 relaxations
+    of the same charge-1 fields to the same tolerance: same energy to 1e-8
+    relative (both are at the minimum; the FP32 phase only changes the path)."""
+    th = ssa.load_theory("Baby Skyrme model")
+    params = [H.bs_params(th, n, n, mpi=0.8, kappa=1.0) for n in (64, 80, 72)]
+    cfgs = [{"mode": "initial", "ansatz": "neel", "skyrmion_number": 1, "noise": 1e-3, "seed": 30 + k}
+            for k in range(3)]
+    batch = BatchSimulation(th, params, precision="fp32")
+    try:
+        batch.initialize(cfgs)
+        res = batch.relax(1e-6, 120000, check_every=500, reduced_steps=20000)
+        print(f"[batch retire] converged {res['converged']}/3 at steps {res['converged_step'].tolist()} "
+              f"(fp32 {res['steps_reduced'].tolist()}, fp64 {res['steps_fp64'].tolist()}) "
+              f"err {res['err_absmax'].tolist()}", flush=True)
+        assert res["converged"] == 3 and np.all(res["err_absmax"] < 1e-6)
+        for i, (p, cfg) in enumerate(zip(params, cfgs)):
+            sim = ssa.Simulation(p, th)
+            sim.initialize(cfg)
+            E, ea, n = sim.minimize(tol=1e-6, max_steps=200000, check_every=500)
+            print(f"[batch retire inst {i}] batch E={res['energy'][i]:.13f} single FP64 E={E:.13f} ({n} steps) "
+                  f"relE={H.rel(res['energy'][i], E):.2e}", flush=True)
+            assert ea < 1e-6
+            assert H.rel(res["energy"][i], E) < 1e-8
+            sim.close()
+    finally:
+        batch.close()
