@@ -595,6 +595,102 @@ def batch_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def differentiation_workload(args):
+    """Anisotropic s+id Ginzburg-Landau, 512 x 512, FP64, the exact-adjoint
+    flow (the variation is dE_h/dphi of the discrete energy): alpha1 = -1,
+    alpha2 = -0.5, beta1..beta4 ~ U[0.2, 1] from seed 1 (redrawn until the
+    quartic is bounded below), gamma1 = gamma2 = 1, gamma12 = 0.3, kappa =
+    0.673274, Q = 1; vortex ansatz (N1 = N2 = 1) plus U(-1e-2, 1e-2).  Relax to
+    max |dE_h/dphi|/gv < --tol, then dE*/dbeta_i by the backward of the
+    discrete energy (exact: dual numbers in the packed couplings, chained
+    through ResolvedParams) against central re-relaxations at beta_i +- 1e-3,
+    and torch.autograd.gradcheck of the energy on a 32 x 32 patch."""
+    import torch
+    import soliton_solver_amd as ssa
+    from soliton_solver_amd import autograd as sag
+    th = ssa.load_theory("Anisotropic superconductor")
+    n, xs = 512, args.diff_xsize
+    h = xs / (n - 1)
+    rng = np.random.default_rng(1)
+    while True:  # beta1..beta4 ~ U[0.2, 1], redrawn until the quartic is bounded below with real vacua
+        b1, b2, b3, b4 = rng.uniform(0.2, 1.0, size=4)
+        b3t = b3 - 2.0 * b4
+        den = b3t * b3t - 4.0 * b1 * b2
+        n1, n2 = 2.0 * b2 * -1.0 - b3t * -0.5, 2.0 * b1 * -0.5 - b3t * -1.0
+        if b3t > -2.0 * np.sqrt(b1 * b2) + 0.1 and den != 0 and n1 / den > 0 and n2 / den > 0:
+            break
+    p = th.params.default_params(xlen=n, ylen=n, xsize=xs, ysize=xs, alpha1=-1.0, alpha2=-0.5, beta1=b1, beta2=b2,
+                                 beta3=b3, beta4=b4, gamma1=1.0, gamma2=1.0, gamma12=0.3, Q=1.0, kappa=0.673274,
+                                 time_step=args.diff_dt * h)
+    sim = ssa.Simulation(p, th, variation="exact_adjoint")
+    sim.initialize({"mode": "initial", "noise": 1e-2, "seed": 1})
+    p_f = sim.p_f_h
+    sim.use_graphs(20)
+    t0 = time.perf_counter()
+    E, ea, nrel = sim.minimize(tol=args.tol, max_steps=args.max_steps, check_every=5000)
+    t_relax = time.perf_counter() - t0
+    Fstar = sim.get_field()
+    field = torch.tensor(Fstar, device="cuda")
+    G = sag.exact_gradient(sim, field)
+    gmax = float(G.abs().max()) / float(p_f[4])
+    print(f"[bench] 512^2 exact flow (xsize {xs}, dt {args.diff_dt} h): {nrel} steps, {t_relax:.1f} s, E*={E:.13f}, "
+          f"max|EG|={ea:.2e}, max|dE/dphi|/gv={gmax:.2e}", file=sys.stderr, flush=True)
+    names = ["beta1", "beta2", "beta3", "beta4"]
+    t0 = time.perf_counter()
+    grads = sag.energy_parameter_gradient(sim, field, names)
+    t_back = time.perf_counter() - t0
+    sim.close()
+    sens = {}
+    for name in names:
+        v = float(getattr(p, name))
+        d = 1e-3 * max(abs(v), 1.0)
+        Es = []
+        steps = []
+        for sgn in (+1, -1):
+            s2 = ssa.Simulation(p.with_(**{name: v + sgn * d}), th, variation="exact_adjoint")
+            s2.initialize({"mode": "array", "field": Fstar})
+            s2.use_graphs(20)
+            E2, ea2, n2 = s2.minimize(tol=args.tol, max_steps=args.max_steps, check_every=5000)
+            Es.append(E2)
+            steps.append([n2, ea2])
+            s2.close()
+        fd = (Es[0] - Es[1]) / (2 * d)
+        rel = abs(grads[name] - fd) / abs(fd)
+        sens[name] = {"backward": grads[name], "rerelaxation": fd, "rel_diff": rel, "rerelax_steps_err": steps}
+        print(f"[bench] dE*/d{name}: backward {grads[name]:.10e} re-relaxation {fd:.10e} rel {rel:.2e}",
+              file=sys.stderr, flush=True)
+    # gradcheck of the energy's backward against central differences on a 32 x 32 patch
+    sim = ssa.Simulation(p, th)
+    base = torch.tensor(Fstar, device="cuda")
+    x0, y0 = 240, 250
+    patch = base[:, x0:x0 + 32, y0:y0 + 32].clone().requires_grad_(True)
+
+    def f(pt):
+        full = base.clone()
+        full[:, x0:x0 + 32, y0:y0 + 32] = pt
+        return sag.discrete_energy(full, sim)
+
+    ok = bool(torch.autograd.gradcheck(f, (patch,), eps=1e-6, atol=1e-6, rtol=1e-4, nondet_tol=0.0,
+                                       fast_mode=False))
+    sim.close()
+    out = {
+        "metric": "dE*/dbeta_i at a relaxed minimum: backward of the discrete energy vs re-relaxation",
+        "value": max(v["rel_diff"] for v in sens.values()),
+        "unit": "max relative difference",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"anisotropic s+id 512x512, xsize {xs}, exact-adjoint flow dt {args.diff_dt} h, "
+                               f"betas from seed 1, vortex ansatz + U(-1e-2, 1e-2), relaxed to {args.tol:g}"},
+        "relaxation": {"steps": nrel, "seconds": t_relax, "energy": E, "err_absmax": ea, "max_dEdphi_over_gv": gmax},
+        "backward_seconds": t_back,
+        "sensitivities": sens,
+        "gradcheck_32x32_patch": ok,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -622,10 +718,16 @@ def main():
                          "default: with the compensated field update, the form that converges; fp32: plain)")
     ap.add_argument("--ttc-n", type=int, default=4096,
                     help="throughput workload: lattice side of the time-to-converge run (one anti-skyrmion)")
-    ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge"], default="throughput",
+    ap.add_argument("--diff-xsize", type=float, default=100.0,
+                    help="differentiation workload: box side of the 512^2 lattice")
+    ap.add_argument("--diff-dt", type=float, default=0.3,
+                    help="differentiation workload: time step in units of the spacing")
+    ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge", "differentiation"],
+                    default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
                          "batch: many baby-Skyrme relaxations of uneven sizes; converge: time-to-converge of the "
-                         "Abelian-Higgs 128^2 and chiral 320^2 FP64 workloads")
+                         "Abelian-Higgs 128^2 and chiral 320^2 FP64 workloads; differentiation: dE*/dbeta by "
+                         "backward at a relaxed 512^2 anisotropic minimum")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--nsk", type=int, default=1, help="precision workload: seeded anti-skyrmions")
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
@@ -694,6 +796,10 @@ def main():
         return
     if args.workload == "converge":
         converge_workload(args)
+        return
+    if args.workload == "differentiation":
+        args.tol = 1e-6 if args.tol is None else args.tol
+        differentiation_workload(args)
         return
 
     n = args.n
