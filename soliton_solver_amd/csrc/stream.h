@@ -127,11 +127,28 @@ __device__ __forceinline__ int row_wave() {
 }
 
 template <typename R, int NFR, int SC = 2>
-struct Ring {
+struct alignas(16) Ring {
   using S = StreamShape<R, NFR, SC>;
   static_assert(S::kFits, "ring does not fit in 64 KiB of LDS");
   R v[NFR][S::NR][S::RY];
 };
+
+// The compensated 1+2 sweep's ring, where four parts fit, holds the field
+// joined in FP64, (double)hi + (double)lo, formed once per site when a row
+// enters the ring: NF planes of doubles over float components 2NF .. 4NF-1
+// (after F and Temp2), plane a at [a][slot][column].
+template <typename R, int NFR, int SC>
+__device__ __forceinline__ double* joined_plane(Ring<R, NFR, SC>& q, int a) {
+  using S = StreamShape<R, NFR, SC>;
+  static_assert(sizeof(R) == 4 && NFR % 4 == 0, "joined planes: FP32 ring of four parts");
+  return reinterpret_cast<double*>(&q.v[NFR / 2][0][0]) + a * S::NR * S::RY;
+}
+template <typename R, int NFR, int SC>
+__device__ __forceinline__ const double* joined_plane(const Ring<R, NFR, SC>& q, int a) {
+  using S = StreamShape<R, NFR, SC>;
+  static_assert(sizeof(R) == 4 && NFR % 4 == 0, "joined planes: FP32 ring of four parts");
+  return reinterpret_cast<const double*>(&q.v[NFR / 2][0][0]) + a * S::NR * S::RY;
+}
 
 struct SLaunch {
   int sx;       // rows per strip (a multiple of RB)
@@ -256,13 +273,11 @@ __device__ __forceinline__ void ring_derivs_d1(const Ring<R, NFR, SC>& q, int a,
   d1d[1] = (INTERIOR || s.iny) ? ty : 0.0;
 }
 
-// FP64 value and first derivatives of hi + lo, hi in ring component ah and lo
-// in al (the compensated form's field), from the FP64 sums of the ring values.
-template <bool INTERIOR, typename R, int NFR, int SC>
-__device__ __forceinline__ void ring_d1_hilo(const Ring<R, NFR, SC>& q, int ah, int al, const int sl[5], int cy,
-                                             const StencilK<double>& Kd, const SiteInfo& s, double& f,
-                                             double d1[2]) {
-  auto v = [&](int r, int c) { return (double)q.v[ah][r][c] + (double)q.v[al][r][c]; };
+// FP64 value and first derivatives from v(slot, column), the compensated
+// form's field hi + lo in FP64.
+template <bool INTERIOR, class VF>
+__device__ __forceinline__ void d1_fp64(VF v, const int sl[5], int cy, const StencilK<double>& Kd,
+                                        const SiteInfo& s, double& f, double d1[2]) {
   const double c12 = SS_C12, c23 = SS_C23;
   const double xm2 = v(sl[0], cy), xm1 = v(sl[1], cy), xp1 = v(sl[3], cy), xp2 = v(sl[4], cy);
   const double ym2 = v(sl[2], cy - 2), ym1 = v(sl[2], cy - 1), yp1 = v(sl[2], cy + 1), yp2 = v(sl[2], cy + 2);
@@ -271,6 +286,25 @@ __device__ __forceinline__ void ring_d1_hilo(const Ring<R, NFR, SC>& q, int ah, 
   const double sy1 = (c12 * (ym2 - yp2) + c23 * (yp1 - ym1)) * Kd.ily;
   d1[0] = (INTERIOR || s.inx) ? sx1 : 0.0;
   d1[1] = (INTERIOR || s.iny) ? sy1 : 0.0;
+}
+
+// hi in ring component ah and lo in al, joined at every read.
+template <bool INTERIOR, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_d1_hilo(const Ring<R, NFR, SC>& q, int ah, int al, const int sl[5], int cy,
+                                             const StencilK<double>& Kd, const SiteInfo& s, double& f,
+                                             double d1[2]) {
+  d1_fp64<INTERIOR>([&](int r, int c) { return (double)q.v[ah][r][c] + (double)q.v[al][r][c]; }, sl, cy, Kd, s, f,
+                    d1);
+}
+
+// hi + lo from the joined FP64 plane a (the same values ring_d1_hilo forms).
+template <bool INTERIOR, typename R, int NFR, int SC>
+__device__ __forceinline__ void ring_d1_joined(const Ring<R, NFR, SC>& q, int a, const int sl[5], int cy,
+                                               const StencilK<double>& Kd, const SiteInfo& s, double& f,
+                                               double d1[2]) {
+  using S = StreamShape<R, NFR, SC>;
+  const double* pl = joined_plane(q, a);
+  d1_fp64<INTERIOR>([&](int r, int c) { return pl[r * S::RY + c]; }, sl, cy, Kd, s, f, d1);
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
@@ -335,7 +369,7 @@ __device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, 
       if constexpr (PASS >= 3 && PASS != 12) {
         const R lv = ld<R>(plane<R>(L, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
-      } else if constexpr ((PASS == 1 && NFR == 2 * NF) || (PASS == 12 && NFR == 3 * NF)) {
+      } else if constexpr ((PASS == 1 && NFR == 2 * NF) || (PASS == 12 && NFR >= 3 * NF)) {
         const R lv = ld<R>(plane<R>(b.Flo, a, NP), boff);
         p.l[k][a] = in ? lv : R(0);
       } else {
@@ -426,6 +460,8 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
         q.v[a][slot][c] = p.f[k][a];
         q.v[NF + a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], R(0), dt);
         if constexpr (NFR == 3 * NF) q.v[2 * NF + a][slot][c] = p.l[k][a];  // compensated: low words
+        if constexpr (NFR == 4 * NF)  // compensated: hi + lo joined in FP64
+          joined_plane(q, a)[slot * S::RY + c] = (double)p.f[k][a] + (double)p.l[k][a];
       } else if (PASS == 34) {
         q.v[a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
         q.v[NF + a][slot][c] = stage_value<4, R>(p.f[k][a], p.v[k][a], p.la[k][a], dt);
@@ -570,8 +606,12 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           // NF ring components), read after the variation so that the FP64
           // values are not held across it
 #pragma unroll
-          for (int a = 0; a < NF; ++a)
-            ring_d1_hilo<INTERIOR, R, NFR>(q, AOFF + a, NFR - NF + a, sl, cy, Kd, s, fd[a], d1d[a]);
+          for (int a = 0; a < NF; ++a) {
+            if constexpr (NFR == 4 * NF)
+              ring_d1_joined<INTERIOR, R, NFR>(q, a, sl, cy, Kd, s, fd[a], d1d[a]);
+            else
+              ring_d1_hilo<INTERIOR, R, NFR>(q, AOFF + a, NFR - NF + a, sl, cy, Kd, s, fd[a], d1d[a]);
+          }
         }
         acc0 += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
       }
@@ -647,12 +687,19 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
 // Partials: one per strip.
 template <class TH, typename R, int PASS, int CW = 1, bool COMP = false>
 struct StreamRing {
+  // the compensated 1+2 sweep: F, Temp2 and hi + lo joined in FP64 (two
+  // float parts) where that fits 64 KiB, else F, Temp2 and the low words
+  static constexpr int kSC = (PASS == 41 || PASS == 34 || PASS == 12 ? 1 : 2) | (CW == 2 ? 4 : 0);
+  static constexpr bool kJoined =
+      PASS == 12 && COMP && sizeof(R) == 4 &&
+      StreamShape<R, 4 * TH::NF, kSC>::NR * StreamShape<R, 4 * TH::NF, kSC>::kBytesRow <= 65536;
   static constexpr int NFR =
-      (PASS == 12 && COMP) ? 3 * TH::NF
+      kJoined ? 4 * TH::NF
+      : (PASS == 12 && COMP) ? 3 * TH::NF
       : (PASS == 23 || PASS == 41 || PASS == 12 || PASS == 34 || (PASS == 1 && COMP)) ? 2 * TH::NF
                                                                                      : TH::NF;
   // the 4+1 and 3+4 sweeps prefetch five and four arrays: one row per wave and iteration
-  static constexpr int SC = (PASS == 41 || PASS == 34 || PASS == 12 ? 1 : 2) | (CW == 2 ? 4 : 0);
+  static constexpr int SC = kSC;
   using shape = StreamShape<R, NFR, SC>;
   using type = Ring<R, NFR, SC>;
 };
@@ -724,9 +771,12 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       } else {
         q.v[a][r][c] = in ? stage_value<PASS, R>(fo, v, l, dt) : R(0);
       }
-      if constexpr ((PASS == 1 || PASS == 12) && COMP) {  // low words in the last ring part
+      if constexpr ((PASS == 1 || PASS == 12) && COMP) {
         const R lo = ld<R>(plane<R>(b.Flo, a, NP), boff);
-        q.v[NFR - NF + a][r][c] = in ? lo : R(0);
+        if constexpr (PASS == 12 && NFR == 4 * NF)  // hi + lo joined in FP64
+          joined_plane(q, a)[r * S::RY + c] = in ? (double)fo + (double)lo : 0.0;
+        else  // low words in the last ring part
+          q.v[NFR - NF + a][r][c] = in ? lo : R(0);
       }
     }
   }
