@@ -148,21 +148,29 @@ k_fin_err(const double* p0, const double* p1, int n, DevState* st, double* histE
   }
 }
 
-// The two finalizes of the 4+1 schedule in one launch: err of step n (as
-// k_fin_err), then the energy of the new field and the arrest decision (as
-// k_fin_energy with the step counter already advanced).
+// Two finalizes in one launch: err of step n (as k_fin_err, from the n_err
+// partials p0, p1), then the energy of the field after it and the arrest
+// decision (as k_fin_energy with the step counter already advanced, from the
+// n_en partials p2; the per-thread Kahan order is k_fin_energy's, so the energy
+// is the same to the bit).  The 4+1 schedule calls it after its sweep; the
+// paired schedules after the 1+2 sweep of step n + 1.
 __global__ void __launch_bounds__(kFinThreads)
-k_fin_err_energy(const double* p0, const double* p1, const double* p2, int n, DevState* st, double* histErr,
-                 double* histE, int cap, int killkinen, int parity) {
+k_fin_err_energy(const double* p0, const double* p1, int n_err, const double* p2, int n_en, DevState* st,
+                 double* histErr, double* histE, int cap, int killkinen, int parity) {
   __shared__ double red[16];
   double m0 = -1.0e300, m1 = 0.0, s = 0.0, c = 0.0;
+  const int n = n_err > n_en ? n_err : n_en;
   for (int i = threadIdx.x; i < n; i += kFinThreads) {
-    m0 = nmax(m0, p0[i]);
-    m1 = nmax(m1, p1[i]);
-    const double y = p2[i] - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
+    if (i < n_err) {
+      m0 = nmax(m0, p0[i]);
+      m1 = nmax(m1, p1[i]);
+    }
+    if (i < n_en) {
+      const double y = p2[i] - c;
+      const double t = s + y;
+      c = (t - s) - y;
+      s = t;
+    }
   }
   const double a = block_max<kFinThreads>(m0, red);
   const double b = block_max<kFinThreads>(m1, red);
@@ -504,6 +512,15 @@ static bool comp12_in(const ss_ctx* c, int v) {
 static bool comp12_ok(const ss_ctx* c) { return use_comp_pairs(c) && comp12_in(c, 3); }
 static int comp12_v(const ss_ctx* c, int v) { return comp12_in(c, v) ? v : 3; }
 
+// Paired schedules (plain, or compensated with its 1+2 sweep) finalize the err
+// of a step together with the next step's energy: three launches per step
+// instead of four.  Not in timed runs, which finalize every step on its own.
+static bool defer_err(const ss_ctx* c, bool timed) { return !timed && (use_pairs(c) || comp12_ok(c)); }
+// Calls whose first step runs outside the graphs: the 4+1 schedule (its first
+// step carries pass 1) and the deferred paired schedules (no err pending in
+// their first step).
+static bool lead_step(const ss_ctx* c, bool timed) { return use41(c) || defer_err(c, timed); }
+
 // Shape variant of the fused 2+3 sweep: tv[2] where the sweep exists in it,
 // else the first strip variant.
 static int fused23_v(const ss_ctx* c) {
@@ -625,7 +642,7 @@ static int launch_step_l2(ss_ctx* c, int parity);
 static int launch_fin41(ss_ctx* c, int parity) {
   const int nt = c->ops->ntiles(c->prec, fused41_v(c), c->g);
   hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part,
-                     (const double*)(c->part + c->ntiles), (const double*)(c->part + 2 * c->ntiles), nt, c->st,
+                     (const double*)(c->part + c->ntiles), nt, (const double*)(c->part + 2 * c->ntiles), nt, c->st,
                      c->hist + c->hist_cap, c->hist, c->hist_cap, c->g.killkinen, parity);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -640,35 +657,62 @@ static int launch_fin41(ss_ctx* c, int parity) {
 // runs only in the first step of a call (``first``); every step ends with the
 // 4+1 sweep, which also gives l1 and the energy of the new field, so the
 // arrest decision follows it directly.
+// The finalize that follows a 1+2 sweep whose energy partials are in p2 (n_en
+// of them): with the err of the previous step still pending (deferred
+// schedule, not the first step of a call) both in one launch, else the
+// energy's alone.
+static int launch_fin_after12(ss_ctx* c, bool pending, int n_err, const double* p2, int n_en, int parity) {
+  if (pending)
+    hipLaunchKernelGGL(k_fin_err_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part,
+                       (const double*)(c->part + c->ntiles), n_err, p2, n_en, c->st, c->hist + c->hist_cap, c->hist,
+                       c->hist_cap, c->g.killkinen, parity);
+  else
+    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, p2, n_en, c->st, c->hist,
+                       c->hist_cap, c->g.killkinen, parity);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// The err finalize of a 3+4 sweep (partials p0, p1; n of them).
+static int launch_fin_err(ss_ctx* c, int n) {
+  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)c->part,
+                     (const double*)(c->part + c->ntiles), n, c->st, c->hist + c->hist_cap, c->hist_cap);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
 // Paired schedule: passes 1+2 (l1 and the energy of F; l2 from Temp2 = F +
 // dt V/2, V read before the arrest decision) -> the decision -> passes 3+4
 // (after an arrest with l2 = l1, which is the l2 of V = 0) -> err.  Events as
-// in launch_step, passes 2 and 3 empty.
-static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
+// in launch_step, passes 2 and 3 empty.  Deferred (defer_err): the err of the
+// step is finalized with the next step's energy, after its 1+2 sweep (``first``:
+// no err pending, the first step of a call), and by do_steps after the last
+// step of a call; the timed run finalizes every step on its own.
+static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
   double* p1 = c->part + c->ntiles;
+  double* p2 = c->part + 2 * c->ntiles;
+  const bool defer = defer_err(c, ev != nullptr);
   const int v12 = pairs_v(c, c->tv[1]), v34 = pairs_v(c, c->tv[4]);
+  const int n12 = c->ops->ntiles(c->prec, v12, c->g), n34 = c->ops->ntiles(c->prec, v34, c->g);
   int e = 0;
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = l1_buf(c, parity);
   b.lout2 = c->l[1];
-  b.part0 = p0;
+  b.part0 = defer ? p2 : p0;
   HIPCHK(c, c->ops->pass(c->prec, 9, v12, b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
-                     c->ops->ntiles(c->prec, v12, c->g), c->st, c->hist, c->hist_cap, c->g.killkinen, parity);
-  HIPCHK(c, hipGetLastError());
+  if (launch_fin_after12(c, defer && !first, n34, b.part0, n12, parity)) return -1;
   if (ev)
     for (int k = 0; k < 3; ++k) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   b.lout = nullptr;
   b.lout2 = nullptr;
+  b.part0 = p0;
   b.part1 = p1;
   HIPCHK(c, c->ops->pass(c->prec, 10, v34, b, c->tp, c->g, c->stream));
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
-                     c->ops->ntiles(c->prec, v34, c->g), c->st, c->hist + c->hist_cap, c->hist_cap);
-  HIPCHK(c, hipGetLastError());
+  if (!defer && launch_fin_err(c, n34)) return -1;
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
   return 0;
 }
@@ -679,33 +723,33 @@ static int launch_step_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
 // decision) -> the decision -> compensated 3+4 (after an arrest with
 // l2 = l1) -> err.  Elsewhere: pass 1 (low words, energy of hi + lo) -> the
 // decision -> pass 2 (V arrested) -> compensated 3+4 -> err.
-static int launch_step_comp_pairs(ss_ctx* c, int parity, hipEvent_t* ev) {
+static int launch_step_comp_pairs(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
   PassBufs b = bufs_for(c, parity);
   double* p0 = c->part;
   double* p1 = c->part + c->ntiles;
+  double* p2 = c->part + 2 * c->ntiles;
   const int v1 = pass1_variant(c), v34 = pairs_v(c, c->tv[4]);
   int e = 0;
   if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-  if (comp12_ok(c)) {
+  if (comp12_ok(c)) {  // err finalize deferred as in launch_step_pairs
+    const bool defer = defer_err(c, ev != nullptr);
     const int v12 = comp12_v(c, c->tv[1]);
+    const int n12 = c->ops->ntiles(c->prec, v12, c->g), n34 = c->ops->ntiles(c->prec, v34, c->g);
     b.lout = l1_buf(c, parity);
     b.lout2 = c->l[1];
-    b.part0 = p0;
+    b.part0 = defer ? p2 : p0;
     HIPCHK(c, c->ops->pass(c->prec, 12, v12, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-    hipLaunchKernelGGL(k_fin_energy, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0,
-                       c->ops->ntiles(c->prec, v12, c->g), c->st, c->hist, c->hist_cap, c->g.killkinen, parity);
-    HIPCHK(c, hipGetLastError());
+    if (launch_fin_after12(c, defer && !first, n34, b.part0, n12, parity)) return -1;
     if (ev)
       for (int k = 0; k < 3; ++k) HIPCHK(c, hipEventRecord(ev[e++], c->stream));  // no separate passes 2, 3
     b.lout = nullptr;
     b.lout2 = nullptr;
+    b.part0 = p0;
     b.part1 = p1;
     HIPCHK(c, c->ops->pass(c->prec, 11, v34, b, c->tp, c->g, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
-    hipLaunchKernelGGL(k_fin_err, dim3(1), dim3(kFinThreads), 0, c->stream, (const double*)p0, (const double*)p1,
-                       c->ops->ntiles(c->prec, v34, c->g), c->st, c->hist + c->hist_cap, c->hist_cap);
-    HIPCHK(c, hipGetLastError());
+    if (!defer && launch_fin_err(c, n34)) return -1;
     if (ev) HIPCHK(c, hipEventRecord(ev[e++], c->stream));
     return 0;
   }
@@ -737,8 +781,8 @@ static int launch_step(ss_ctx* c, int parity, hipEvent_t* ev, bool first) {
     if (ev) return fail(c, "pass timing is available for the fused reference flow only");
     return c->ops->l2_normalize ? launch_step_l2(c, parity) : launch_step_exact(c, parity);
   }
-  if (use_pairs(c)) return launch_step_pairs(c, parity, ev);
-  if (use_comp_pairs(c)) return launch_step_comp_pairs(c, parity, ev);
+  if (use_pairs(c)) return launch_step_pairs(c, parity, ev, first);
+  if (use_comp_pairs(c)) return launch_step_comp_pairs(c, parity, ev, first);
   const bool f41 = use41(c);
   const bool fold = fold_mode(c, ev != nullptr);
   PassBufs b = bufs_for(c, parity);
@@ -1259,8 +1303,9 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, c->stream, c->st, prev_energy);
   HIPCHK(c, hipGetLastError());
   int done = 0;
-  if (use41(c) && nsteps > 0) {
-    // the first step carries pass 1 and stays outside the graphs
+  const bool timed = ev != nullptr;
+  if (lead_step(c, timed) && nsteps > 0) {
+    // the first step (4+1: carries pass 1; deferred pairs: no err pending) stays outside the graphs
     if (launch_step(c, c->cur, ev, true)) return -1;
     if (ev) {
       HIPCHK(c, hipEventSynchronize(ev[6]));
@@ -1281,7 +1326,7 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
     }
   }
   for (; done < nsteps; ++done) {
-    if (launch_step(c, c->cur, ev, !use41(c))) return -1;
+    if (launch_step(c, c->cur, ev, !lead_step(c, timed))) return -1;
     if (ev) {
       HIPCHK(c, hipEventSynchronize(ev[6]));
       for (int k = 0; k < 6; ++k) {
@@ -1293,6 +1338,9 @@ static int do_steps(ss_ctx* c, double prev_energy, int nsteps, hipEvent_t* ev, d
     c->cur ^= 1;
   }
   if (fold_mode(c, ev != nullptr) && nsteps > 0 && launch_fin41(c, c->cur)) return -1;
+  if (defer_err(c, timed) && nsteps > 0 &&
+      launch_fin_err(c, c->ops->ntiles(c->prec, pairs_v(c, c->tv[4]), c->g)))  // the last step's err
+    return -1;
   if (launch_finish(c, nsteps)) return -1;
   c->last_nsteps = nsteps;
   return 0;

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import soliton_solver_amd as ssa
+from soliton_solver_amd import _lib
 from oracle import core as ocore
 from tests import helpers as H
 
@@ -636,6 +637,52 @@ def test_folded_finalize_is_bit_exact(precision, graphs):
         assert np.array_equal(a.get_field(), b.get_field())
         assert np.array_equal(a.get_velocity(), b.get_velocity())
         assert H.rel(Ea, a.compute_energy()) < (1e-12 if precision == "fp64" else 1e-6)
+        a.close()
+        b.close()
+
+
+def _history(sim, n):
+    """Per-step energies and errs of the context's last call, whatever ran it."""
+    hE, hR = np.empty(n), np.empty(n)
+    sim._check(_lib.lib().ss_step_history(sim._ctx, hE, hR, n))
+    return hE, hR
+
+
+@pytest.mark.parametrize("graphs", [0, 4])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+def test_deferred_err_finalize_is_bit_exact(precision, graphs):
+    """The paired schedules finalize the err of a step together with the next
+    step's energy (one launch after the next 1+2 sweep; the last step's err
+    after the call); the timed schedule (ss_time_passes) finalizes every step on
+    its own.  After 1, 2 and 9 steps from a rough field (arrests included) the
+    fields and velocities agree bit for bit, so do the per-step energies and
+    errs, and the call returns the last step's err."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 96, 80)
+    p_i, p_f = H.packed(th, p)
+    F = H.chiral_field(th, p_i, p_f, seed=8, rough=True)
+    for n in (1, 2, 9):
+        a = _sim(th, p, F, precision=precision)
+        b = _sim(th, p, F, precision=precision)
+        for s in (a, b):
+            s.set_tile_variants([3, 3, 3, 3])
+            s.set_fused_stages(False, False, pairs=True)
+        assert a.tile_variants()["pairs"]
+        if precision == "fp32c":
+            assert a.tile_variants()["comp12"]
+        if graphs:
+            a.use_graphs(graphs)
+        E0 = a.compute_energy()
+        Ea, ema, _ = a.steps(E0, n, history=True)
+        hEa, hRa = a.last_history
+        b.time_passes(E0, n)
+        hEb, hRb = _history(b, n)
+        rises = int(np.sum(np.diff(np.concatenate([[E0], hEa])) > 0))
+        print(f"[deferred err finalize {precision} graphs={graphs} n={n}] energy rises={rises}")
+        assert np.array_equal(a.get_field(), b.get_field())
+        assert np.array_equal(a.get_velocity(), b.get_velocity())
+        assert np.array_equal(hEa, hEb) and np.array_equal(hRa, hRb)
+        assert ema == hRa[-1]
         a.close()
         b.close()
 
