@@ -432,31 +432,46 @@ def schedule(tiles, comp):
     step, for the schedule ``tiles`` describes (the dict of
     ``Simulation.tile_variants``); ``comp``: the compensated FP32 form, whose
     field carries a second word (lo) that its pass 1 reads and its combine
-    reads and writes."""
+    reads and writes.  ``arrested_words``: the dominant launch's words in a
+    step whose velocity was arrested, where V is known to be zero and is not
+    read (and a paired 3+4 sweep takes l2 = l1, read once)."""
     pairs, f41, f23 = bool(tiles["pairs"]), bool(tiles["fused41"]), bool(tiles["fused23"])
     v4 = tiles["variants"][3]
     cw = 2 if v4 >= 6 else 1
     if comp:
         if pairs and tiles.get("comp12"):  # 1+2 (F, lo, V -> l1, l2) + 3+4 (F, lo, V, l1, l2 -> F', lo', V')
-            return {"name": "compensated paired 1+2, 3+4", "dominant_words": 8, "step_words": 13,
-                    "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
+            return {"name": "compensated paired 1+2, 3+4", "dominant_words": 8, "arrested_words": 6,
+                    "step_words": 13, "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
         if pairs:  # pass 1 (F, lo -> l1) + pass 2 (F, V -> l2) + 3+4 (F, lo, V, l1, l2 -> F', lo', V')
-            return {"name": "compensated pass 1, pass 2, paired 3+4", "dominant_words": 8, "step_words": 14,
-                    "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
+            return {"name": "compensated pass 1, pass 2, paired 3+4", "dominant_words": 8, "arrested_words": 6,
+                    "step_words": 14, "kernel": f"stream_pass_comp<ChiralMagnet, float, 34, {cw}>"}
         kern = (f"stream_pass_comp<ChiralMagnet, float, 4, {cw}>" if v4 >= 3 else
                 f"stage_pass<ChiralMagnet, float, 4, {tiles['rows'][3]}, true>")
         return {"name": "compensated pass 1, " + ("2+3, " if f23 else "2, 3, ") + "4",
-                "dominant_words": 9, "step_words": 3 + (5 if f23 else 7) + 9, "kernel": kern}
+                "dominant_words": 9, "arrested_words": 8, "step_words": 3 + (5 if f23 else 7) + 9, "kernel": kern}
     if pairs:  # 1+2 (F, V -> l1, l2) + 3+4 (F, V, l1, l2 -> F', V')
-        return {"name": "paired 1+2, 3+4", "dominant_words": 6, "step_words": 10,
+        return {"name": "paired 1+2, 3+4", "dominant_words": 6, "arrested_words": 4, "step_words": 10,
                 "kernel": f"stream_pass<ChiralMagnet, float, 34, {cw}>"}
     if f41:  # 2+3 or 2, 3, then 4+1 (F, V, l1, l2, l3 -> F', V', l1')
-        return {"name": ("2+3, " if f23 else "2, 3, ") + "4+1", "dominant_words": 9,
+        return {"name": ("2+3, " if f23 else "2, 3, ") + "4+1", "dominant_words": 9, "arrested_words": 8,
                 "step_words": (5 if f23 else 7) + 9, "kernel": f"stream_pass<ChiralMagnet, float, 41, {cw}>"}
     kern = (f"stream_pass<ChiralMagnet, float, 4, {cw}>" if v4 >= 3 else
             f"stage_pass<ChiralMagnet, float, 4, {tiles['rows'][3]}>")
-    return {"name": "1, " + ("2+3, " if f23 else "2, 3, ") + "4", "dominant_words": 7,
+    return {"name": "1, " + ("2+3, " if f23 else "2, 3, ") + "4", "dominant_words": 7, "arrested_words": 6,
             "step_words": 2 + (5 if f23 else 7) + 7, "kernel": kern}
+
+
+def arrested_steps(energies, prev_energy):
+    """Which steps of a call ran with the velocity arrested: the arrest of step
+    s >= 1 is E(F_s) > E(F_{s-1}) (core/integrator.py:338-343), with E(F_0) the
+    call's prev_energy; ``energies[k]`` = E(F_{k+1}), the per-step history of
+    the call.  Step 0 is never arrested (the decision the call starts from is
+    applied before it, on the host's side)."""
+    e = np.asarray(energies, dtype=np.float64)
+    prevs = np.concatenate([[prev_energy], e[:-1]])
+    kill = np.zeros(len(e), dtype=bool)
+    kill[1:] = e[:-1] > prevs[:-1]
+    return kill
 
 
 def cpu_baseline(args, n):
@@ -842,10 +857,15 @@ def main():
     print(f"[bench] {form} form: {t32 / args.steps * 1e3:.3f} ms/step, {ups32:.3e} updates/s", file=sys.stderr,
           flush=True)
 
-    # dominant-kernel roofline from HIP events on the context stream
-    passes = sim32.time_passes(E32, 20)
+    # dominant-kernel roofline from HIP events on the context stream; its
+    # algorithmic bytes per launch follow the arrest decisions of those steps
+    # (an arrested step reads no V)
+    nroof = 20
+    passes = sim32.time_passes(E32, nroof)
+    kills = arrested_steps(sim32.history(nroof)[0], E32)
     sched = schedule(tiles32, comp)
-    alg_dom = sched["dominant_words"] * nf * w * N
+    words_dom = ((nroof - kills.sum()) * sched["dominant_words"] + kills.sum() * sched["arrested_words"]) / nroof
+    alg_dom = words_dom * nf * w * N
     dom_gbs = alg_dom / (passes[4] * 1e-3) / 1e9
     step_ms_kernels = float(passes.sum())
     alg_step = sched["step_words"] * nf * w * N
@@ -960,7 +980,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "traffic_read_write": [traffic["read_bytes"], traffic["write_bytes"]] if traffic else None,
-                     "kernel": kernel, "alg_words_per_site": sched["dominant_words"] * nf,
+                     "kernel": kernel, "alg_words_per_site": words_dom * nf,
+                     "alg_words_per_site_note": f"{sched['dominant_words'] * nf} per site in a step with the velocity "
+                                                f"free, {sched['arrested_words'] * nf} in an arrested one (no V read); "
+                                                f"{int(kills.sum())} of the {nroof} timed steps arrested",
                      "alg_bytes_per_launch": alg_dom, "avg_launch_ms": passes[4],
                      "step_kernels_ms": step_ms_kernels, "step_kernels_gbs": step_gbs,
                      "pass_ms": [float(x) for x in passes]},

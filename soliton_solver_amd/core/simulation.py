@@ -499,6 +499,15 @@ class Simulation:
         self._check(_lib.lib().ss_variation(self._ctx, _VAR[mode], out, out.size))
         return out.reshape(self.field_shape)
 
+    def history(self, nsteps: int):
+        """(energies, errs) of the last ``nsteps`` steps of the last stepping
+        call: energies[k] is the energy after step k, errs[k] the signed max
+        of step k's stage-4 energy gradient."""
+        hE = np.zeros(int(nsteps))
+        hR = np.zeros(int(nsteps))
+        self._check(_lib.lib().ss_step_history(self._ctx, hE, hR, int(nsteps)))
+        return hE, hR
+
     def time_passes(self, prev_energy: float, nsteps: int) -> np.ndarray:
         """Mean ms of [pass1, energy finalize, pass2, pass3, pass4, err finalize]."""
         out = np.zeros(6)

@@ -906,11 +906,12 @@ struct TheoryLaunchers {
     return StreamShape<R, NF, 6>::kFits && (!StreamShape<R, 2 * NF, 2>::kFits || StreamShape<R, 2 * NF, 6>::kFits) &&
            (!StreamShape<R, 2 * NF, 1>::kFits || StreamShape<R, 2 * NF, 5>::kFits);
   }
-  // The streaming form addresses each plane with a 32-bit byte offset and the
-  // field with 32-bit element offsets; larger fields run the default tile shape.
+  // The streaming form addresses each array of nf planes through one buffer
+  // resource with 32-bit byte offsets below 2^31; larger fields run the
+  // default tile shape.
   template <typename R>
   static int effective_shape(int v, const Geom& g) {
-    const bool ok = (long long)TH::NF * g.N < (1LL << 31) && g.N * (long long)sizeof(double) < (1LL << 32);
+    const bool ok = (long long)TH::NF * g.N * (long long)sizeof(R) < (1LL << 31);
     if (v >= 3 && !ok) return 0;
     if (v >= 6 && !wide_fits<R>()) return v - 3;
     return v;

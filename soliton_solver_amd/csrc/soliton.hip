@@ -2099,10 +2099,11 @@ int ss_batch_create(int theory_id, int ninst, const int32_t* p_i_all, int n_i, c
       bfail(b, -1, "instance %d: theory %s is built for %d fields", i, ops->name, ops->nf);
       return bail(-1);
     }
-    // the batched strip kernels address an instance's planes with 32-bit
-    // byte offsets from its own base, as the single-lattice strips do
-    if (!((long long)g.nf * g.N < (1LL << 31) && g.N * (long long)sizeof(double) < (1LL << 32))) {
-      bfail(b, -1, "instance %d: %d x %d x %d values exceed the batched form's limit (nf*xlen*ylen < 2^31)", i,
+    // the batched strip kernels address an instance's arrays with 32-bit byte
+    // offsets below 2^31 from its own base, as the single-lattice strips do
+    // (FP64, the widest the batch may convert to)
+    if (!((long long)g.nf * g.N * (long long)sizeof(double) < (1LL << 31))) {
+      bfail(b, -1, "instance %d: %d x %d x %d values exceed the batched form's limit (8*nf*xlen*ylen < 2^31)", i,
             g.nf, g.X, g.Y);
       return bail(-1);
     }

@@ -116,6 +116,17 @@ struct StreamShape {
   static_assert(RB + 4 <= NR, "ring too small");
 };
 
+// A copy of v defined here, in program order.  The held prefetch values are
+// kept this way: a plain assignment is a rename, whose copy the compiler
+// places at the loop latch, after the next group's loads, so those loads
+// cannot reuse the registers of the values they replace and go to
+// temporaries copied over, which waits for them.
+template <typename T>
+__device__ __forceinline__ T pinned(T v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Column wave (0 .. CW-1) and row wave (0 .. RW-1) of the calling thread.
 template <class S>
 __device__ __forceinline__ int col_wave() {
@@ -195,6 +206,26 @@ template <typename R>
 __device__ __forceinline__ R ldnt(const char* plane_base, unsigned boff) {
   return __builtin_nontemporal_load((const R*)(plane_base + boff));
 }
+// The prefetch loads of the streaming form go through buffer resources with
+// hardware range checking: a site outside the lattice gets the offset kOutside,
+// beyond the array's size, and the load returns 0.  The value is then final as
+// loaded (no select after the load, which would make the wave wait for it at
+// once), and an array read as all zeros (V after an arrest) is a resource of
+// size 0.  A resource spans one array of nf planes, so the host keeps
+// nf*X*Y*sizeof(R) below 2^31 bytes for this form; plane a is at the scalar
+// offset a*X*Y*sizeof(R).
+constexpr unsigned kOutside = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t array_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+template <typename R>
+__device__ __forceinline__ R bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(R) == 4)
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // Outputs of a streaming pass are read by the next launch only: non-temporal
 // stores keep them from displacing the input rows (halo lines still to be read
 // by neighbouring strips) in L2.  Measured at 4096^2: pass 4 380 -> 351 us,
@@ -308,18 +339,49 @@ __device__ __forceinline__ void ring_d1_joined(const Ring<R, NFR, SC>& q, int a,
 }
 
 // Raw inputs of the stage field for the RB rows of one group, per thread: RPW
-// interior sites (its column) and, for threads < 4*RB, one halo-column site
-// (row tid/4, ring column 0, 1, TYW+2 or TYW+3).  A group starts 2 rows after
-// the first row computed in the next iteration, so the interior row of (row
-// wave w, k) is (w*RPW + k - 2) mod RB: each thread loads the sites it
-// computes next iteration, except the first two compute rows (w*RPW + k < 2),
-// whose loaders hold them one iteration longer.
-template <typename R, int NF, int RPW>
+// interior sites (its column) and HP (site, component) pairs of the group's
+// 4*RB halo-column sites (ring columns 0, 1, TYW+2, TYW+3), spread one pair
+// per thread so that each thread holds one component of a halo site, not all
+// NF.  A group starts 2 rows after the first row computed in the next
+// iteration, so the interior row of (row wave w, k) is (w*RPW + k - 2) mod RB:
+// each thread loads the sites it computes next iteration, except the first
+// two compute rows (w*RPW + k < 2), whose loaders hold them one iteration
+// longer.  Sites outside the lattice read 0 (range-checked loads), so the
+// values are used as loaded and stay in flight across the compute of the
+// current rows.
+template <typename R, int NF, int RPW, int HP>
 struct StreamPrefetch {
-  R f[RPW + 1][NF], v[RPW + 1][NF], l[RPW + 1][NF];
-  R la[RPW + 1][NF], lc[RPW + 1][NF];  // fused 4+1 only: l1 and l3
+  R f[RPW][NF], v[RPW][NF], l[RPW][NF];
+  R la[RPW][NF], lc[RPW][NF];  // fused 4+1: l1 and l3; 3+4: l2 in la
+  R hf[HP], hv[HP], hl[HP], hla[HP];  // halo pairs: F, V, l, la of one component
 };
 
+// Halo-column (site, component) pairs of a group: 4*RB sites x NF components,
+// pair p = threadIdx.x + j*kThreads (site p / NF, component p mod NF).
+template <class S, int NF>
+struct HaloPairs {
+  static constexpr int kPairs = 4 * S::RB * NF;
+  static constexpr int HP = (kPairs + kThreads - 1) / kThreads;
+};
+// Row r (in the group), ring column c and component a of pair j of this
+// thread; false when the thread has no such pair.
+template <class S, int NF>
+__device__ __forceinline__ bool halo_pair(int j, int& r, int& c, int& a) {
+  const int p = threadIdx.x + j * kThreads;
+  const int site = p / NF;
+  a = p - site * NF;
+  r = site >> 2;
+  const int qd = site & 3;
+  c = qd < 2 ? qd : S::TYW + qd;
+  return p < HaloPairs<S, NF>::kPairs;
+}
+
+template <typename R, int NF, int NFR, int SC>
+using PrefetchFor = StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW, HaloPairs<StreamShape<R, NFR, SC>, NF>::HP>;
+
+// Ring slot row r and column c of the RPW interior sites (k < RPW) and, for
+// the 4+1 sweep, of its halo-column site (k == RPW, threads < 4*RB; one site
+// with all components, loaded when stored).
 template <typename R, int NFR, int SC>
 __device__ __forceinline__ void group_site(int k, int& r, int& c) {
   using S = StreamShape<R, NFR, SC>;
@@ -334,57 +396,58 @@ __device__ __forceinline__ void group_site(int k, int& r, int& c) {
   }
 }
 
-// Issue the loads of the group of rows starting at rb.  Element offsets are
-// 32-bit (the host keeps nf*X*Y below 2^31); a site outside the lattice loads
-// element 0 and is replaced by 0, so no load sits behind a branch.  PASS
+// Issue the loads of the group of rows starting at rb (range-checked: a site
+// outside the lattice, or V after an arrest, reads 0; no load sits behind a
+// branch).  PASS
 // selects the arrays: 0/1 F; 2 and 12 F, V; 3 and 23 F, V, l1; 4 F, V, l2; 34
 // F, V, l1, l2 (into la); 41 F, V, l2, l1, l3; pass 1 of the compensated form (PASS 1 with a two-half ring) F
 // and its low words (into l); passes 1+2 of the compensated form (PASS 12 with a three-part ring) F, V and the
 // low words (into l).
 template <int PASS, typename R, int NF, int NFR, int SC>
-__device__ __forceinline__ void group_load(StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, const PassBufs& b,
-                                           const Geom& g, int rb, int y0, bool kill) {
+__device__ __forceinline__ void group_load(PrefetchFor<R, NF, NFR, SC>& p, const PassBufs& b, const Geom& g, int rb,
+                                           int y0, bool kill, bool live = true) {
   using S = StreamShape<R, NFR, SC>;
   const void* L = (PASS == 4 || PASS == 41) ? b.l2 : b.l1;
-  const unsigned NP = (unsigned)g.N;
+  // the low words of the compensated passes 1 and 1+2 travel in l
+  constexpr bool kLo = (PASS == 1 && NFR == 2 * NF) || (PASS == 12 && NFR >= 3 * NF);
+  const unsigned NB = (unsigned)g.N * (unsigned)sizeof(R), AB = NB * (unsigned)NF;
+  const auto rF = array_rsrc(b.F, AB), rV = array_rsrc(b.V, kill ? 0u : AB);
+  const auto rL = array_rsrc(kLo ? (const void*)b.Flo : L, AB);
+  const auto r1 = array_rsrc(b.l1, AB), r2 = array_rsrc(b.l2, AB), r3 = array_rsrc(b.l3, AB);
 #pragma unroll
-  for (int k = 0; k <= S::RPW; ++k) {
-    if (PASS == 41 && k == S::RPW) break;  // the 4+1 sweep loads its halo-column sites when it stores them
+  for (int k = 0; k < S::RPW; ++k) {
     int r, c;
     group_site<R, NFR, SC>(k, r, c);
     const int x = rb + r, y = y0 - 2 + c;
-    const bool mine = (k < S::RPW) || (threadIdx.x < 4 * S::RB);
-    const bool in = mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
-    const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
+    const bool in = live && x >= 0 && x < g.X && y >= 0 && y < g.Y;
+    const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : kOutside;
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
-      const R fv = ld<R>(plane<R>(b.F, a, NP), boff);
-      p.f[k][a] = in ? fv : R(0);
-      if (PASS >= 2) {
-        const R vv = ld<R>(plane<R>(b.V, a, NP), boff);
-        p.v[k][a] = (in && !kill) ? vv : R(0);
-      } else {
-        p.v[k][a] = R(0);
-      }
-      if constexpr (PASS >= 3 && PASS != 12) {
-        const R lv = ld<R>(plane<R>(L, a, NP), boff);
-        p.l[k][a] = in ? lv : R(0);
-      } else if constexpr ((PASS == 1 && NFR == 2 * NF) || (PASS == 12 && NFR >= 3 * NF)) {
-        const R lv = ld<R>(plane<R>(b.Flo, a, NP), boff);
-        p.l[k][a] = in ? lv : R(0);
-      } else {
-        p.l[k][a] = R(0);
-      }
+      const unsigned so = (unsigned)a * NB;
+      p.f[k][a] = bld<R>(rF, boff, so);
+      p.v[k][a] = (PASS >= 2) ? bld<R>(rV, boff, so) : R(0);
+      p.l[k][a] = ((PASS >= 3 && PASS != 12) || kLo) ? bld<R>(rL, boff, so) : R(0);
       if (PASS == 41) {
-        const R l1v = ld<R>(plane<R>(b.l1, a, NP), boff), l3v = ld<R>(plane<R>(b.l3, a, NP), boff);
-        p.la[k][a] = in ? l1v : R(0);
-        p.lc[k][a] = in ? l3v : R(0);
+        p.la[k][a] = bld<R>(r1, boff, so);
+        p.lc[k][a] = bld<R>(r3, boff, so);
       }
-      if (PASS == 34) {  // l2 (l1 after an arrest: the caller points b.l2 there)
-        const R l2v = ld<R>(plane<R>(b.l2, a, NP), boff);
-        p.la[k][a] = in ? l2v : R(0);
-      }
+      if (PASS == 34)  // l2 (l1 after an arrest: the caller points b.l2 there)
+        p.la[k][a] = bld<R>(r2, boff, so);
     }
+  }
+  if constexpr (PASS == 41) return;  // the 4+1 sweep loads its halo-column sites when it stores them
+#pragma unroll
+  for (int j = 0; j < HaloPairs<S, NF>::HP; ++j) {
+    int r, c, a;
+    const bool mine = halo_pair<S, NF>(j, r, c, a);
+    const int x = rb + r, y = y0 - 2 + c;
+    const bool in = live && mine && x >= 0 && x < g.X && y >= 0 && y < g.Y;
+    // component a is a runtime plane: its offset joins the site's
+    const unsigned boff = in ? (unsigned)a * NB + (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : kOutside;
+    p.hf[j] = bld<R>(rF, boff, 0u);
+    p.hv[j] = (PASS >= 2) ? bld<R>(rV, boff, 0u) : R(0);
+    p.hl[j] = ((PASS >= 3 && PASS != 12) || kLo) ? bld<R>(rL, boff, 0u) : R(0);
+    p.hla[j] = (PASS == 34) ? bld<R>(r2, boff, 0u) : R(0);
   }
 }
 
@@ -440,36 +503,47 @@ __device__ __forceinline__ R stage_value(R fo, R v, R l, R dt) {
 // Write a group into the ring; slot0 is the ring slot of its first row.  PASS
 // 23 writes both stage fields, Temp2 to components 0..NF-1 and Temp3 to
 // NF..2NF-1; PASS 12 F and Temp2; PASS 34 Temp3 and Temp4.
+// One component a of one site into ring slot ``slot``, column c.
 template <int PASS, typename R, int NF, int NFR, int SC>
-__device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
-                                            const StreamPrefetch<R, NF, StreamShape<R, NFR, SC>::RPW>& p, int slot0,
+__device__ __forceinline__ void ring_put(Ring<R, NFR, SC>& q, int slot, int c, int a, R fo, R vv, R lv, R la, R dt) {
+  using S = StreamShape<R, NFR, SC>;
+  if (PASS == 23) {
+    q.v[a][slot][c] = stage_value<2, R>(fo, vv, lv, dt);
+    q.v[NF + a][slot][c] = stage_value<3, R>(fo, vv, lv, dt);
+  } else if (PASS == 12) {
+    q.v[a][slot][c] = fo;
+    q.v[NF + a][slot][c] = stage_value<2, R>(fo, vv, R(0), dt);
+    if constexpr (NFR == 3 * NF) q.v[2 * NF + a][slot][c] = lv;  // compensated: low words
+    if constexpr (NFR == 4 * NF)  // compensated: hi + lo joined in FP64
+      joined_plane(q, a)[slot * S::RY + c] = (double)fo + (double)lv;
+  } else if (PASS == 34) {
+    q.v[a][slot][c] = stage_value<3, R>(fo, vv, lv, dt);
+    q.v[NF + a][slot][c] = stage_value<4, R>(fo, vv, la, dt);
+  } else {
+    q.v[a][slot][c] = stage_value<PASS, R>(fo, vv, lv, dt);
+    if constexpr (PASS == 1 && NFR == 2 * NF) q.v[NF + a][slot][c] = lv;  // low words
+  }
+}
+
+template <int PASS, typename R, int NF, int NFR, int SC>
+__device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q, const PrefetchFor<R, NF, NFR, SC>& p, int slot0,
                                             R dt) {
   using S = StreamShape<R, NFR, SC>;
 #pragma unroll
-  for (int k = 0; k <= S::RPW; ++k) {
-    if (k == S::RPW && threadIdx.x >= 4 * S::RB) break;
+  for (int k = 0; k < S::RPW; ++k) {
     int r, c;
     group_site<R, NFR, SC>(k, r, c);
     const int slot = (slot0 + r) & (S::NR - 1);
 #pragma unroll
-    for (int a = 0; a < NF; ++a) {
-      if (PASS == 23) {
-        q.v[a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
-        q.v[NF + a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
-      } else if (PASS == 12) {
-        q.v[a][slot][c] = p.f[k][a];
-        q.v[NF + a][slot][c] = stage_value<2, R>(p.f[k][a], p.v[k][a], R(0), dt);
-        if constexpr (NFR == 3 * NF) q.v[2 * NF + a][slot][c] = p.l[k][a];  // compensated: low words
-        if constexpr (NFR == 4 * NF)  // compensated: hi + lo joined in FP64
-          joined_plane(q, a)[slot * S::RY + c] = (double)p.f[k][a] + (double)p.l[k][a];
-      } else if (PASS == 34) {
-        q.v[a][slot][c] = stage_value<3, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
-        q.v[NF + a][slot][c] = stage_value<4, R>(p.f[k][a], p.v[k][a], p.la[k][a], dt);
-      } else {
-        q.v[a][slot][c] = stage_value<PASS, R>(p.f[k][a], p.v[k][a], p.l[k][a], dt);
-        if constexpr (PASS == 1 && NFR == 2 * NF) q.v[NF + a][slot][c] = p.l[k][a];  // low words
-      }
-    }
+    for (int a = 0; a < NF; ++a)
+      ring_put<PASS, R, NF, NFR, SC>(q, slot, c, a, p.f[k][a], p.v[k][a], p.l[k][a], p.la[k][a], dt);
+  }
+#pragma unroll
+  for (int j = 0; j < HaloPairs<S, NF>::HP; ++j) {
+    int r, c, a;
+    if (!halo_pair<S, NF>(j, r, c, a)) break;
+    const int slot = (slot0 + r) & (S::NR - 1);
+    ring_put<PASS, R, NF, NFR, SC>(q, slot, c, a, p.hf[j], p.hv[j], p.hl[j], p.hla[j], dt);
   }
 }
 
@@ -478,8 +552,7 @@ __device__ __forceinline__ void group_store(Ring<R, NFR, SC>& q,
 // the halo-column sites (threads < 4 RB) are loaded here, not prefetched,
 // which keeps five arrays of one site out of the prefetch registers.
 template <class TH, typename R, int NFR, int SC>
-__device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q,
-                                              const StreamPrefetch<R, TH::NF, StreamShape<R, NFR, SC>::RPW>& p,
+__device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q, const PrefetchFor<R, TH::NF, NFR, SC>& p,
                                               int slot0, R dt, const typename TH::TP& tp, const PassBufs& b,
                                               const Geom& g, int rb, int y0, bool kill) {
   using S = StreamShape<R, NFR, SC>;
@@ -497,18 +570,19 @@ __device__ __forceinline__ void group_store41(Ring<R, NFR, SC>& q,
     if (k < S::RPW) {
       new_field<TH, R>(tp, g, in, p.f[k], p.v[k], p.la[k], p.l[k], p.lc[k], dt, T4, Fn);
     } else {
-      const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
+      const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : kOutside;
+      const unsigned NB = NP * (unsigned)sizeof(R), AB = NB * (unsigned)NF;
+      const auto rF = array_rsrc(b.F, AB), rV = array_rsrc(b.V, kill ? 0u : AB);
+      const auto r1 = array_rsrc(b.l1, AB), r2 = array_rsrc(b.l2, AB), r3 = array_rsrc(b.l3, AB);
       R f[NF], v[NF], l1[NF], l2[NF], l3[NF];
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        const R f0 = ld<R>(plane<R>(b.F, a, NP), boff), v0 = ld<R>(plane<R>(b.V, a, NP), boff);
-        const R a1 = ld<R>(plane<R>(b.l1, a, NP), boff), a2 = ld<R>(plane<R>(b.l2, a, NP), boff);
-        const R a3 = ld<R>(plane<R>(b.l3, a, NP), boff);
-        f[a] = in ? f0 : R(0);
-        v[a] = (in && !kill) ? v0 : R(0);
-        l1[a] = in ? a1 : R(0);
-        l2[a] = in ? a2 : R(0);
-        l3[a] = in ? a3 : R(0);
+        const unsigned so = (unsigned)a * NB;
+        f[a] = bld<R>(rF, boff, so);
+        v[a] = bld<R>(rV, boff, so);
+        l1[a] = bld<R>(r1, boff, so);
+        l2[a] = bld<R>(r2, boff, so);
+        l3[a] = bld<R>(r3, boff, so);
       }
       new_field<TH, R>(tp, g, in, f, v, l1, l2, l3, dt, T4, Fn);
     }
@@ -548,7 +622,12 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
   // registers are not there to spare, and they are loaded at the combine.
   constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF == -1 && !COMP;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
-  if (kPre) {
+  // Compensated pass 4 of the 3+4 sweep: hi and lo of this thread's sites,
+  // likewise issued before the stencils (they are read once, at the combine).
+  // (The separate compensated pass 4 has no registers to spare for them.)
+  constexpr bool kPreC = PASS == 4 && COMP && FOFF == -2;
+  float phi[kPreC ? RPW : 1][NF], plo[kPreC ? RPW : 1][NF];
+  if (kPre || kPreC) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int x = c0 + w * RPW + k;
@@ -556,8 +635,14 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        pl1[k][a] = ldnt<R>(plane<R>(b.l1, a, NP), boff);
-        pl3[k][a] = ldnt<R>(plane<R>(b.l3, a, NP), boff);
+        if constexpr (kPre) {
+          pl1[k][a] = ldnt<R>(plane<R>(b.l1, a, NP), boff);
+          pl3[k][a] = ldnt<R>(plane<R>(b.l3, a, NP), boff);
+        }
+        if constexpr (kPreC) {
+          phi[k][a] = ldnt<float>(plane<float>(b.F, a, NP), boff);
+          plo[k][a] = ldnt<float>(plane<float>(b.Flo, a, NP), boff);
+        }
       }
     }
   }
@@ -638,7 +723,9 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           }
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
-          if constexpr (COMP)
+          if constexpr (kPreC)
+            Fd[a] = compensated_sum(phi[k][a], plo[k][a], dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
+          else if constexpr (COMP)
             Fd[a] = compensated_sum(ldnt<float>(plane<float>(b.F, a, NP), boff),
                                     ldnt<float>(plane<float>(b.Flo, a, NP), boff),
                                     dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
@@ -817,7 +904,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       }
     }
   }
-  StreamPrefetch<R, NF, RPW> pf;
+  PrefetchFor<R, NF, NFR, SC> pf;
   if (nit > 1) group_load<PASS, R, NF, NFR, SC>(pf, b, g, xs + RB + 2, y0, kill);
 
   PassBufs b2 = b, b3 = b;
@@ -888,9 +975,11 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
                                                             &keepL[0][0], acc0, acc1);
     }
     __syncthreads();  // every read of this iteration's rows is done
-    if (it + 1 < nit) {
+    {
       // rows c0+RB+2 .. c0+2RB+1 go to slots (it*RB + RB + 4 ..) mod NR, which
-      // no row of iteration it+1 other than themselves maps to
+      // no row of iteration it+1 other than themselves maps to.  Done in the
+      // last iteration too (nothing reads the ring after it), so that no merge
+      // of stored and unstored values follows the block.
       if constexpr (PASS == 41)
         group_store41<TH, R, NFR, SC>(q, pf, (it * RB + RB + 4) & (NR - 1), dt, tp, b, g, c0 + RB + 2, y0, kill);
       else
@@ -905,24 +994,34 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
           for (int a = 0; a < NF; ++a) {
             // 3+4: the prefetch holds l1 in l and l2 in la
             const R nl = PASS == 34 ? pf.la[k][a] : pf.l[k][a];
-            keepV[k][a] = lag ? holdV[k][a] : pf.v[k][a];
-            keepL[k][a] = lag ? holdL[k][a] : nl;
-            holdV[k][a] = pf.v[k][a];
-            holdL[k][a] = nl;
+            const R nv = pf.v[k][a];
+            keepV[k][a] = pinned(lag ? holdV[k][a] : nv);
+            keepL[k][a] = pinned(lag ? holdL[k][a] : nl);
+            holdV[k][a] = pinned(nv);
+            holdL[k][a] = pinned(nl);
             if (PASS == 34) {
-              keepA[k][a] = lag ? holdA[k][a] : pf.l[k][a];
-              holdA[k][a] = pf.l[k][a];
+              const R na = pf.l[k][a];
+              keepA[k][a] = pinned(lag ? holdA[k][a] : na);
+              holdA[k][a] = pinned(na);
             }
             if (PASS == 41) {
-              keepA[k][a] = lag ? holdA[k][a] : pf.la[k][a];
-              keepC[k][a] = lag ? holdC[k][a] : pf.lc[k][a];
-              holdA[k][a] = pf.la[k][a];
-              holdC[k][a] = pf.lc[k][a];
+              const R na = pf.la[k][a], nc = pf.lc[k][a];
+              keepA[k][a] = pinned(lag ? holdA[k][a] : na);
+              keepC[k][a] = pinned(lag ? holdC[k][a] : nc);
+              holdA[k][a] = pinned(na);
+              holdC[k][a] = pinned(nc);
             }
           }
         }
       }
-      if (it + 2 < nit) group_load<PASS, R, NF, NFR, SC>(pf, b, g, c0 + 2 * RB + 2, y0, kill);
+      // issued in every iteration (after the last group all of it reads as
+      // outside, with no memory access), so the prefetch registers carry no
+      // merge of loaded and kept values around the loop; and only after every
+      // read of the previous group (the scheduling barrier), so that the loads
+      // can land in the registers that group held instead of in temporaries
+      // copied over (the copies would wait for the loads)
+      __builtin_amdgcn_sched_barrier(0);
+      group_load<PASS, R, NF, NFR, SC>(pf, b, g, c0 + 2 * RB + 2, y0, kill, it + 2 < nit);
     }
   }
   if (PASS == 1 || PASS == 12) {

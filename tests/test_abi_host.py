@@ -100,13 +100,16 @@ def test_simulation_requires_gradient_kernel():
 
 
 def test_batch_refuses_instances_beyond_32bit_offsets():
-    """The batched strip kernels address an instance's planes with 32-bit byte
-    offsets (sites * 8 bytes < 2^32); a larger instance is refused when the batch
-    is created, before any device is touched.  (nf * sites < 2^31 already holds
-    for every instance: the packed p_i carries it as an int32.)"""
+    """The batched strip kernels address an instance's arrays (nf planes each)
+    through buffer resources with 32-bit byte offsets below 2^31, in FP64 (the
+    widest precision a batch converts to): nf * sites * 8 < 2^31.  A larger
+    instance is refused when the batch is created, before any device is
+    touched; 10000^2 baby-Skyrme sites (nf = 3: 2.4e9 bytes per array) are
+    beyond it."""
     from soliton_solver_amd.batch import BatchSimulation
     th = ssa.load_theory("Baby Skyrme model")
     small = th.params.default_params(xlen=64, ylen=64, xsize=20.0, ysize=20.0)
-    huge = th.params.default_params(xlen=24000, ylen=24000, xsize=20.0, ysize=20.0)
-    with pytest.raises(ValueError, match="exceed"):
-        BatchSimulation(th, [small, huge], precision="fp32")
+    for n in (10000, 24000):
+        huge = th.params.default_params(xlen=n, ylen=n, xsize=20.0, ysize=20.0)
+        with pytest.raises(ValueError, match="exceed"):
+            BatchSimulation(th, [small, huge], precision="fp32")
