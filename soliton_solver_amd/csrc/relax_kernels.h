@@ -731,7 +731,12 @@ __global__ void __launch_bounds__(kThreads) batch_stage_pass(BatchBufs B) {
   b.part0 = B.part0;
   b.part1 = B.part1;
   const bool kill = (PASS >= 2) ? (st.kill != 0) : false;
-  stage_tile<TH, R, PASS, TXR>(tile, red, b, I.tp, I.g, tx * S::TX, ty * TY, kill, t);
+  // geometry and couplings copied out of the descriptor: as references into
+  // global memory they would be re-read after every store of the pass (the
+  // compiler cannot rule out that a store aliases them)
+  const Geom g = I.g;
+  const typename TH::TP tp = I.tp;
+  stage_tile<TH, R, PASS, TXR>(tile, red, b, tp, g, tx * S::TX, ty * TY, kill, t);
 }
 
 // Streaming form of the batched passes: the table entries are strips of
@@ -766,7 +771,9 @@ __global__ void __launch_bounds__(kThreads) batch_stream_pass(BatchBufs B) {
   b.part1 = B.part1;
   const bool kill = (PASS >= 2 && PASS != 12) ? (st.kill != 0) : false;  // 1+2 runs before the decision
   if (PASS == 34 && kill) b.l2 = b.l1;                                      // l2 of an arrested V
-  stream_strip<TH, R, PASS, CW>(q, red, b, I.tp, I.g, sxi * B.sx, B.sx, syi * TYW, kill, t);
+  const Geom g = I.g;  // copied out of the descriptor, as in batch_stage_pass
+  const typename TH::TP tp = I.tp;
+  stream_strip<TH, R, PASS, CW>(q, red, b, tp, g, sxi * B.sx, B.sx, syi * TYW, kill, t);
 }
 
 // Energy partials of every instance (active or not) from its current field.
@@ -780,7 +787,8 @@ __global__ void __launch_bounds__(kThreads) batch_energy_pass(BatchBufs B) {
   if (t < 0) return;
   const int i = B.tile_inst[t];
   const BatchInst<TH>& I = ((const BatchInst<TH>*)B.insts)[i];
-  const Geom& g = I.g;
+  const Geom g = I.g;
+  const typename TH::TP tp = I.tp;
   const int lt = t - I.tile_begin;
   const int tx = lt / I.ntiles_y, ty = lt - tx * I.ntiles_y;
   const int x0 = tx * S::TX, y0 = ty * TY;
@@ -795,7 +803,7 @@ __global__ void __launch_bounds__(kThreads) batch_energy_pass(BatchBufs B) {
       const SiteInfo s = site_info(g, x, y);
       double fd[NF], d1d[NF][2];
       tile_d1<double, R, NF>(tile, lx, lane, g, s, fd, d1d);
-      acc += TH::template energy<double>(I.tp, g, s, fd, d1d) * g.gv;
+      acc += TH::template energy<double>(tp, g, s, fd, d1d) * g.gv;
     }
   }
   const double sum = block_sum<kThreads>(acc, red);

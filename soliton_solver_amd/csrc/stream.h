@@ -610,7 +610,7 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
                                             const Geom& g, const StencilK<R>& K, const StencilK<double>& Kd, int xs,
                                             int c0, int y, int cy, const R* kV, const R* kL, double& acc0,
                                             double& acc1, const R* kA = nullptr, const R* kC = nullptr,
-                                            R* kOut = nullptr) {
+                                            R* kOut = nullptr, const float* kHL = nullptr) {
   using S = StreamShape<R, NFR, SC>;
   constexpr int NF = TH::NF, NR = S::NR, RPW = S::RPW;
   const R dt = R(g.dt);
@@ -622,12 +622,12 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
   // registers are not there to spare, and they are loaded at the combine.
   constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF == -1 && !COMP;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
-  // Compensated pass 4 of the 3+4 sweep: hi and lo of this thread's sites,
-  // likewise issued before the stencils (they are read once, at the combine).
-  // (The separate compensated pass 4 has no registers to spare for them.)
+  // Compensated pass 4 of the 3+4 sweep: hi and lo of this thread's sites come
+  // in kHL (hi [k][a], then lo), loaded by the caller before pass 3.  (The
+  // separate compensated pass 4 has no registers to spare for them and loads
+  // them at the combine.)
   constexpr bool kPreC = PASS == 4 && COMP && FOFF == -2;
-  float phi[kPreC ? RPW : 1][NF], plo[kPreC ? RPW : 1][NF];
-  if (kPre || kPreC) {
+  if (kPre) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
       const int x = c0 + w * RPW + k;
@@ -635,14 +635,8 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
       const unsigned boff = in ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        if constexpr (kPre) {
-          pl1[k][a] = ldnt<R>(plane<R>(b.l1, a, NP), boff);
-          pl3[k][a] = ldnt<R>(plane<R>(b.l3, a, NP), boff);
-        }
-        if constexpr (kPreC) {
-          phi[k][a] = ldnt<float>(plane<float>(b.F, a, NP), boff);
-          plo[k][a] = ldnt<float>(plane<float>(b.Flo, a, NP), boff);
-        }
+        pl1[k][a] = ldnt<R>(plane<R>(b.l1, a, NP), boff);
+        pl3[k][a] = ldnt<R>(plane<R>(b.l3, a, NP), boff);
       }
     }
   }
@@ -724,7 +718,8 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
           const R a4 = -dt * eg[a];
           Vn[a] = Vc + sixth * (a1 + R(2) * a2 + R(2) * a3 + a4);
           if constexpr (kPreC)
-            Fd[a] = compensated_sum(phi[k][a], plo[k][a], dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
+            Fd[a] = compensated_sum(kHL[k * NF + a], kHL[(RPW + k) * NF + a],
+                                    dt * (Vc + R(0.5) * a2) + dt6 * (a1 - R(2) * a2 + a3));
           else if constexpr (COMP)
             Fd[a] = compensated_sum(ldnt<float>(plane<float>(b.F, a, NP), boff),
                                     ldnt<float>(plane<float>(b.Flo, a, NP), boff),
@@ -945,16 +940,33 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       for (int k = 0; k < RPW; ++k)
 #pragma unroll
         for (int a = 0; a < NF; ++a) l3r[k][a] = R(0);
+      // compensated combine: hi and lo of this iteration's sites, issued
+      // before pass 3 so that their latency hides behind both variations
+      float hl[COMP ? 2 * RPW : 1][NF];
+      if constexpr (COMP) {
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+          const int x = c0 + w * RPW + k;
+          const unsigned boff = (x < g.X && y < g.Y) ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            hl[k][a] = ldnt<float>(plane<float>(b.F, a, NP), boff);
+            hl[RPW + k][a] = ldnt<float>(plane<float>(b.Flo, a, NP), boff);
+          }
+        }
+      }
       if (interior) {
         stream_rows<TH, R, 3, true, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
                                                 nullptr, nullptr, &l3r[0][0]);
         stream_rows<TH, R, 4, true, NFR, NF, -2, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
-                                                           &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0]);
+                                                           &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0],
+                                                           nullptr, &hl[0][0]);
       } else {
         stream_rows<TH, R, 3, false, NFR, 0, -3>(q, b, tp, g, K, Kd, xs, c0, y, cy, nullptr, nullptr, unused, unused,
                                                  nullptr, nullptr, &l3r[0][0]);
         stream_rows<TH, R, 4, false, NFR, NF, -2, SC, COMP>(q, b, tp, g, K, Kd, xs, c0, y, cy, &keepV[0][0],
-                                                            &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0]);
+                                                            &keepL[0][0], acc0, acc1, &keepA[0][0], &l3r[0][0],
+                                                            nullptr, &hl[0][0]);
       }
     } else if (PASS == 41) {
       // pass 4 from Temp4 (ring 0..NF-1), then pass 1 of the next step from F' (ring NF..2NF-1)

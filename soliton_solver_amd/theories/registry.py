@@ -73,29 +73,32 @@ def load_theory(name: str) -> ModuleType:
     return mod
 
 
+def _spec_lines(spec: TheorySpec) -> list[tuple[str, str]]:
+    """(label, value) pairs of a spec, empty values left out."""
+    rows = [("Theory", spec.name), ("Import path", spec.import_path), ("Version", spec.version),
+            ("Aliases", ", ".join(spec.aliases)), ("Description", spec.description)]
+    return [(k, v) for k, v in rows if v]
+
+
 def describe_theory(name: str) -> None:
-    spec = get_theory_spec(name)
-    print(f"Theory: {spec.name}")
-    print(f"Import path: {spec.import_path}")
-    print(f"Version: {spec.version}")
-    if spec.aliases:
-        print("Aliases:", ", ".join(spec.aliases))
-    if spec.description:
-        print(f"Description: {spec.description}")
+    """Print one registered theory, one labelled field per line."""
+    for label, value in _spec_lines(get_theory_spec(name)):
+        print(f"{label}: {value}")
 
 
 def print_theory_table() -> None:
-    specs = sorted(set(_REGISTRY.values()), key=lambda s: s.name.lower())
+    """Print every registered theory once (aliases folded into their spec) as
+    a table of name, version and description, sorted by name."""
+    specs = sorted({id(s): s for s in _REGISTRY.values()}.values(), key=lambda s: s.name.lower())
     if not specs:
         print("No theories registered.")
         return
-    w1 = max(len("Theory"), *(len(s.name) for s in specs))
-    w2 = max(len("Version"), *(len(s.version) for s in specs))
-    header = f"{'Theory':<{w1}}  {'Version':<{w2}}  Description"
-    print(header)
-    print("-" * len(header))
-    for s in specs:
-        print(f"{s.name:<{w1}}  {s.version:<{w2}}  {s.description}")
+    cols = ("Theory", "Version", "Description")
+    rows = [(s.name, s.version, s.description) for s in specs]
+    widths = [max(len(r[i]) for r in [cols, *rows]) for i in range(2)]
+    fmt = lambda r: "  ".join([r[0].ljust(widths[0]), r[1].ljust(widths[1]), r[2]])  # noqa: E731
+    head = fmt(cols)
+    print("\n".join([head, "-" * len(head), *map(fmt, rows)]))
 
 
 def discover_theories(package: str = "soliton_solver_amd.theories") -> None:
