@@ -524,21 +524,16 @@ def _chunk(sim, E, nsteps):
     return E, err, ea, nsteps, time.perf_counter() - t0
 
 
-def batch_workload(args):
-    """Many baby-Skyrme relaxations of uneven sizes in one set of launches per RK4
-    pass: sizes uniform in {64, 128, 256, 512}^2, xsize 20, dt = 0.0957 h, pion
-    mass uniform in [0.1, 1], Skyrme coupling uniform in [0.5, 2], standard
-    potential, Neel hedgehog of charge 1..4 plus U(-1e-3, 1e-3), graphs of 20
-    steps.  FP32 fields with FP64 sums until every instance reaches max |EG| <
-    1e-3 (the FP32 floor) or --fp32-steps, then, for a tighter --tol, the FP64
-    retire phase (fields converted in place, every instance reactivated) until
-    each reaches --tol, within --batch-max-steps steps in all."""
-    import torch
+def make_batch(seed, ninst):
+    """The batched workload's instances (FP32, initialised, launch shape
+    autotuned): sizes uniform in {64, 128, 256, 512}^2, xsize 20, dt = 0.0957 h,
+    pion mass uniform in [0.1, 1], Skyrme coupling uniform in [0.5, 2],
+    standard potential, Neel hedgehog of charge 1..4 plus U(-1e-3, 1e-3).
+    Returns (batch, sizes, charges, autotune info)."""
     import soliton_solver_amd as ssa
     from soliton_solver_amd.batch import BatchSimulation
     th = ssa.load_theory("Baby Skyrme model")
-    rng = np.random.default_rng(args.seed)
-    ninst = args.instances
+    rng = np.random.default_rng(seed)
     sizes = rng.choice([64, 128, 256, 512], size=ninst)
     mpi = rng.uniform(0.1, 1.0, size=ninst)
     kappa = rng.uniform(0.5, 2.0, size=ninst)
@@ -552,6 +547,46 @@ def batch_workload(args):
     bs.initialize([{"mode": "initial", "ansatz": "neel", "skyrmion_number": int(q), "noise": 1e-3, "seed": i}
                    for i, q in enumerate(charge)])
     tinfo = bs.autotune_tiles(reps=5)
+    return bs, sizes, charge, tinfo
+
+
+def batch_throughput(args, steps=100, warmup=20):
+    """The batched workload's step rate with every instance active (tolerance
+    0, so none retires): ``warmup`` steps (the step graphs are captured there),
+    then ``steps`` timed steps in graphs of 20."""
+    import torch
+    bs, sizes, charge, tinfo = make_batch(args.seed, args.instances)
+    bs.use_graphs(20)
+    bs.begin()
+    bs.run(warmup, 0.0, check_every=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.run(steps, 0.0, check_every=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sites = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
+    out = {"workload": f"{args.instances} baby Skyrme lattices, sizes U{{64,128,256,512}}^2 (seed {args.seed}), FP32 "
+                       "fields with FP64 sums, every instance active",
+           "updates_per_s": sites * steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps,
+           "total_sites": int(sites), "shape_variant": tinfo["variant"], "fused_passes_2_3": bool(tinfo["fused23"]),
+           "paired_sweeps": bool(tinfo.get("pairs", False)),
+           "relaxation": "bench.py --workload batch (until max|EG| < 1e-6 or a step budget)"}
+    bs.close()
+    return out
+
+
+def batch_workload(args):
+    """Many baby-Skyrme relaxations of uneven sizes in one set of launches per RK4
+    pass: sizes uniform in {64, 128, 256, 512}^2, xsize 20, dt = 0.0957 h, pion
+    mass uniform in [0.1, 1], Skyrme coupling uniform in [0.5, 2], standard
+    potential, Neel hedgehog of charge 1..4 plus U(-1e-3, 1e-3), graphs of 20
+    steps.  FP32 fields with FP64 sums until every instance reaches max |EG| <
+    1e-3 (the FP32 floor) or --fp32-steps, then, for a tighter --tol, the FP64
+    retire phase (fields converted in place, every instance reactivated) until
+    each reaches --tol, within --batch-max-steps steps in all."""
+    import torch
+    ninst = args.instances
+    bs, sizes, charge, tinfo = make_batch(args.seed, ninst)
     print(f"[bench] batch tile variant {tinfo['variant']} ({tinfo['rows']} rows), fused 2+3: {tinfo['fused23']}; "
           "ms per variant " + " ".join(f"{x:.3f}" for x in tinfo["ms"]) + "; with 2+3 fused "
           + " ".join(f"{x:.3f}" for x in tinfo["ms_fused23"]) + "; paired sweeps "
@@ -762,6 +797,7 @@ def main():
     ap.add_argument("--demag", type=int, default=0, help="precision workload: 1 turns the demagnetising field on")
     ap.add_argument("--chunk", type=int, default=10000, help="precision workload: FP32 steps between polish checks")
     ap.add_argument("--polish", type=int, default=200, help="precision workload: FP64 steps per polish check")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload step rate of the default line")
     ap.add_argument("--no-precision", action="store_true",
                     help="throughput workload: skip the time-to-converge run of the FP32C form")
     ap.add_argument("--converge-cap", type=int, default=0,
@@ -917,6 +953,12 @@ def main():
     if not args.no_cpu and not args.profile_only:
         cpu = cpu_baseline(args, n)
 
+    batch = None
+    if not args.no_batch and not args.profile_only:
+        batch = batch_throughput(args)
+        print(f"[bench] batch: {batch['ms_per_step']:.3f} ms per step, {batch['updates_per_s']:.3e} updates/s",
+              file=sys.stderr, flush=True)
+
     # time to converge of the compensated FP32 form against the FP64 form, to
     # max|EG| < 1e-6 from one seeded field: one anti-skyrmion at 4096^2
     # (demagnetisation off; with it on, the saddle field psi keeps the flow
@@ -991,6 +1033,7 @@ def main():
         "speedup_vs_fp64": (ups32 / result64["updates_per_s"]) if result64 else None,
         ("fp32_form" if comp else "fp32c_form"): other,
         "time_to_converge": ttc,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
