@@ -113,3 +113,17 @@ def test_batch_refuses_instances_beyond_32bit_offsets():
         huge = th.params.default_params(xlen=n, ylen=n, xsize=20.0, ysize=20.0)
         with pytest.raises(ValueError, match="exceed"):
             BatchSimulation(th, [small, huge], precision="fp32")
+
+
+def test_bench_arrested_steps_follow_the_energy_history():
+    """bench.py prices the dominant launch of each timed step at its
+    algorithmic bytes, which drop when the step's velocity was arrested.  The
+    arrest of step s >= 1 is E(F_s) > E(F_{s-1}) (core/integrator.py:338-343),
+    with E(F_0) the energy the call starts from and history[k] = E(F_{k+1});
+    step 0 starts from the caller's decision and counts as free."""
+    from bench import arrested_steps
+    hist = [5.0, 4.0, 4.5, 4.4, 4.6, 4.7]
+    kills = arrested_steps(hist, 6.0)
+    # E(F_1..F_5) = 5.0, 4.0, 4.5, 4.4, 4.6 against 6.0, 5.0, 4.0, 4.5, 4.4
+    assert kills.tolist() == [False, False, False, True, False, True]
+    assert arrested_steps([7.0], 6.0).tolist() == [False]
