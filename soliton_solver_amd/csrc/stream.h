@@ -1028,11 +1028,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       }
       // issued in every iteration (after the last group all of it reads as
       // outside, with no memory access), so the prefetch registers carry no
-      // merge of loaded and kept values around the loop; and only after every
-      // read of the previous group (the scheduling barrier), so that the loads
-      // can land in the registers that group held instead of in temporaries
-      // copied over (the copies would wait for the loads)
-      __builtin_amdgcn_sched_barrier(0);
+      // merge of loaded and kept values around the loop
       group_load<PASS, R, NF, NFR, SC>(pf, b, g, c0 + 2 * RB + 2, y0, kill, it + 2 < nit);
     }
   }
