@@ -143,6 +143,32 @@ def test_kill_semantics_forced_from_host():
             assert np.abs(Vg - Vo).max() < 1e-12 and np.abs(Vo).max() > 0
 
 
+def test_kill_is_strict_on_the_device():
+    """The device arrest is a strict ``>`` (core/integrator.py:341): the same
+    step from the same state, with prev_energy set to the energy that step
+    produces (to the bit), keeps its velocity; with prev_energy one ulp below,
+    it arrests."""
+    th = ssa.load_theory("Abelian Higgs")
+    p = H.gl_params(th, 32, 32)
+    p_i, p_f = H.packed(th, p)
+    F, V = H.gl_field(th, p_i, p_f, seed=11, velocity=True)
+    sim = _sim(th, p, F, V)
+    E1, _ = sim.step(np.inf)
+    V1 = sim.get_velocity()
+    assert np.abs(V1).max() > 0
+    for prev, arrested in ((E1, False), (np.nextafter(E1, -np.inf), True)):
+        sim.set_field(F)
+        sim.set_velocity(V)
+        E2, _ = sim.step(prev)
+        V2 = sim.get_velocity()
+        assert E2 == E1
+        if arrested:
+            assert np.all(V2 == 0.0)
+        else:
+            assert np.array_equal(V2, V1)
+    sim.close()
+
+
 def test_killkinen_off_never_kills():
     th = ssa.load_theory("Abelian Higgs")
     p = H.gl_params(th, 32, 32, killkinen=False)

@@ -140,6 +140,27 @@ def test_kill_and_no_kill():
     assert np.abs(V).max() > 0
 
 
+def test_kill_is_strict():
+    """The arrest compares with a strict ``>`` (core/integrator.py:341): a step
+    whose new energy equals the caller's prev_energy to the bit keeps its
+    velocity.  The equal energy is produced by running the same step twice from
+    the same state, the second time with prev_energy set to the first result."""
+    p_i, p_f = _pp(X=8, Y=8)
+    rng = np.random.default_rng(5)
+    F0 = rng.normal(size=(1, 8, 8))
+    V0 = rng.normal(size=(1, 8, 8))
+    V = V0.copy()
+    E1, _ = ocore.arrested_newton_flow_step(_Linear, F0.copy(), V, p_i, p_f, np.inf)
+    assert np.abs(V).max() > 0
+    V = V0.copy()
+    E2, _ = ocore.arrested_newton_flow_step(_Linear, F0.copy(), V, p_i, p_f, E1)
+    assert E2 == E1
+    assert np.abs(V).max() > 0
+    V = V0.copy()
+    ocore.arrested_newton_flow_step(_Linear, F0.copy(), V, p_i, p_f, np.nextafter(E1, -np.inf))
+    assert np.all(V == 0)
+
+
 # --- variation against the energy: directional derivatives ------------------
 
 def _bump(p_i, p_f, cx=0.5, cy=0.5, w=0.18):
