@@ -229,7 +229,10 @@ def test_gl_variation_is_functional_derivative_of_energy():
         b = _bump(p_i, p_f, w=0.3)
         F[0] += 0.2 * b
         F[1] -= 0.1 * b
-        Psi = np.stack([b, 0.5 * b, -0.3 * b, 0.7 * b])
+        # a direction off the vortex centre: centred, psi.Psi integrates to zero
+        # over the angle and the potential term (lambda) would drop out
+        c = _bump(p_i, p_f, cx=0.6, cy=0.42, w=0.25)
+        Psi = np.stack([c, 0.5 * c, -0.3 * c, 0.7 * c])
         errs.append(_dir_mismatch(o_gl, F, Psi, p_i, p_f))
     print("GL directional mismatch (h, h/2):", errs)
     assert errs[1] < 5e-4
@@ -401,10 +404,13 @@ def test_bec_variation_is_functional_derivative():
         p = H.bec_params(th, n, n, xsize=6.0, beta=3.0, omega_rot=0.4)
         p_i, p_f = H.packed(th, p)
         g = H.grid_of(p_i, p_f)
-        F = np.stack([0.5 * np.exp(-((g[0] - 3) ** 2 + (g[1] - 3) ** 2) / 4) * np.cos(0.3 * g[1]),
-                      0.4 * np.exp(-((g[0] - 3) ** 2 + (g[1] - 3) ** 2) / 5) * np.sin(0.2 * g[0])])
+        # a winding field and a direction with angular structure, so that the
+        # rotation term -Omega L_z carries a good part of the derivative
+        xp, yp = g[0] - 3.0, g[1] - 3.0
+        f = np.exp(-(xp ** 2 + yp ** 2) / 3.0)
+        F = np.stack([0.6 * f * (xp + 0.2), 0.6 * f * (yp - 0.1)])
         b = _bump(p_i, p_f, w=0.35)
-        Psi = np.stack([b, 0.6 * b])
+        Psi = np.stack([b * (xp - 0.5 * yp), b * (yp + 0.3 * xp)])
         errs.append(_dir_mismatch(o_bec, F, Psi, p_i, p_f))
     print("BEC directional mismatch (h, h/2):", errs)
     assert errs[1] < 1e-3
@@ -679,7 +685,8 @@ def test_liquid_crystal_phi_equation_is_poisson_with_bound_charge():
 
 def test_aux_oracle_supercurrent_is_curl_of_flux_and_phase_difference_is_normalised_dot():
     """Supercurrent (d_y B3, -d_x B3, -lap A3) with B3 = d_x A2 - d_y A1, at 4th order
-    in the interior; cos of the phase difference = psi1.psi2/(|psi1||psi2|)."""
+    in the interior; cos of the phase difference = psi1.psi2/(|psi1||psi2|);
+    the Higgs density of a field built in polar form rho e^{i phi} is rho^2."""
     from oracle import aux as oaux
     th = ssa.load_theory("Ferromagnetic superconductor")
     errs = []
@@ -702,3 +709,8 @@ def test_aux_oracle_supercurrent_is_curl_of_flux_and_phase_difference_is_normali
     cosd = oaux.phase_difference(G)
     dot = (G[2] * G[4] + G[3] * G[5]) / np.sqrt((G[2] ** 2 + G[3] ** 2) * (G[4] ** 2 + G[5] ** 2))
     assert np.abs(cosd - dot).max() < 1e-12
+    # |psi|^2 of psi = rho e^{i phi} built in polar form is rho^2
+    rho = rng.uniform(0.1, 2.0, size=(9, 9))
+    phi = rng.uniform(-np.pi, np.pi, size=(9, 9))
+    H2 = np.stack([np.zeros((9, 9)), np.zeros((9, 9)), rho * np.cos(phi), rho * np.sin(phi)])
+    assert np.abs(oaux.higgs(H2, 2, 3) - rho ** 2).max() < 1e-14 * 4.0
