@@ -190,7 +190,7 @@ def test_chiral_variation_is_functional_derivative_of_energy():
     th = ssa.load_theory("Chiral magnet")
     errs = []
     for n in (61, 121):
-        p = H.chiral_params(th, n, n, xsize=4.0, demag=True)
+        p = H.chiral_params(th, n, n, xsize=4.0, demag=True, B=0.2)  # B != 0: the Zeeman term counts
         p_i, p_f = H.packed(th, p)
         g = H.grid_of(p_i, p_f)
         F = np.zeros((4, n, n))
@@ -599,7 +599,10 @@ def test_spin_triplet_vacuum_and_ansatz_charge():
     assert abs(ocore.reference_sum(o_st.skyrmion_density(F, D1, p_i, p_f).ravel()) + 1.0) < 2e-3
 
 
-def _lc_smooth(th, n, **kw):
+def _lc_smooth(th, n, unit=True, **kw):
+    """A smooth director (unit, or -- ``unit`` False -- of length 1 +- 0.15, as
+    the stage fields of a step are, which are not renormalised) with the
+    depolarisation potential Phi, and a direction w tangent to it."""
     p = H.lc_params(th, n, n, xsize=8.0, **kw)
     p_i, p_f = H.packed(th, p)
     g = H.grid_of(p_i, p_f)
@@ -608,11 +611,13 @@ def _lc_smooth(th, n, **kw):
     F[1] = 0.5 * np.cos(0.3 * g[1] - 0.4 * g[0])
     F[2] = 0.8 + 0.2 * np.sin(0.4 * g[0])
     F[:3] /= np.sqrt((F[:3] ** 2).sum(0))
+    if not unit:
+        F[:3] *= 1.0 + 0.15 * np.cos(0.35 * g[0] - 0.25 * g[1])
     F[3] = 0.4 * np.sin(0.3 * g[0] + 0.5 * g[1])
     b = _bump(p_i, p_f, w=0.35)
     w = np.zeros_like(F)
     w[:3] = np.stack([b * np.cos(0.6 * g[1]), 0.5 * b, b * (0.2 + 0.1 * g[0])])
-    w[:3] -= (w[:3] * F[:3]).sum(0) * F[:3]
+    w[:3] -= (w[:3] * F[:3]).sum(0) * F[:3] / (F[:3] ** 2).sum(0)
     return p_i, p_f, F, w
 
 
@@ -629,10 +634,13 @@ def test_liquid_crystal_variation_is_functional_derivative_without_depol(deforma
     assert errs[0] / errs[1] > 8.0
 
 
-def test_liquid_crystal_depolarisation_director_terms_are_twice_their_derivative():
+@pytest.mark.parametrize("unit", [True, False])
+def test_liquid_crystal_depolarisation_director_terms_are_twice_their_derivative(unit):
     """Reference quirk kept: with depol on, the director variation's
     depolarisation part (kernels.py:492-500) is 2 d/dn of the depolarisation
-    energy (kernels.py:203-209), so EG.w - dE = dE_depol at 4th order."""
+    energy (kernels.py:203-209), so EG.w - dE = dE_depol at 4th order.  Also on
+    a director that is not of unit length, where the (e3/e1) n.(d_i n) d_i Phi
+    term, zero for unit directors, contributes."""
     from oracle.theories import liquid_crystal as o_lc
     th = ssa.load_theory("Liquid crystal")
     errs = []
@@ -640,7 +648,7 @@ def test_liquid_crystal_depolarisation_director_terms_are_twice_their_derivative
     for n in (61, 121):
         d = {}
         for depol in (False, True):
-            p_i, p_f, F, w = _lc_smooth(th, n, depol=depol)
+            p_i, p_f, F, w = _lc_smooth(th, n, unit=unit, depol=depol)
             d[depol] = (ocore.theory_energy(o_lc, F + eps * w, p_i, p_f) - ocore.theory_energy(o_lc, F - eps * w, p_i, p_f)) / (2 * eps)
         D1 = ocore.first_derivatives(F, p_i, p_f)
         D2 = ocore.second_derivatives(F, p_i, p_f)
@@ -681,6 +689,34 @@ def test_liquid_crystal_phi_equation_is_poisson_with_bound_charge():
     print("liquid crystal div P vs rho (h, h/2):", errs)
     assert errs[1] < 1e-4
     assert errs[0] / errs[1] > 8.0
+
+
+@pytest.mark.parametrize("unit", [True, False])
+def test_liquid_crystal_depolarisation_energy_is_flexoelectric_polarisation(unit):
+    """The depolarisation energy density (kernels.py:203-209) equals
+    (gv/2) grad Phi . (n div n + (e3/e1) n x curl n), the flexoelectric
+    polarisation (splay e1, bend e3) with d_z = 0.  The identity
+    (n.grad) n = grad(|n|^2)/2 - n x curl n is algebraic in the stencil
+    derivatives, so it holds to rounding, also for a director whose length
+    varies, where the (e3/e1) n.(d_i n) d_i Phi / 2 term does not vanish."""
+    from oracle.theories import liquid_crystal as o_lc
+    th = ssa.load_theory("Liquid crystal")
+    p_i1, p_f1, F, _ = _lc_smooth(th, 61, unit=unit, depol=True)
+    p_i0, p_f0, _, _ = _lc_smooth(th, 61, unit=unit, depol=False)
+    D1 = ocore.first_derivatives(F, p_i1, p_f1)
+    e_depol = o_lc.energy_density(F, D1, p_i1, p_f1) - o_lc.energy_density(F, D1, p_i0, p_f0)
+    gv, r = float(p_f1[4]), float(p_f1[10]) / float(p_f1[9])
+    assert r != 1.0
+    divn = D1[0, 0] + D1[1, 1]
+    curl = np.stack([D1[2, 1], -D1[2, 0], D1[1, 0] - D1[0, 1]])
+    n_x_curl = np.cross(F[:3], curl, axis=0)
+    P = F[:2] * divn + r * n_x_curl[:2]
+    want = 0.5 * gv * (D1[3, 0] * P[0] + D1[3, 1] * P[1])
+    scale = np.abs(want).max()
+    assert np.abs(e_depol - want).max() < 1e-12 * scale
+    if not unit:   # the |n|-gradient part is a visible share of the density
+        grad_n2 = np.stack([(F[:3] * D1[:3, i]).sum(0) for i in (0, 1)])
+        assert np.abs(0.25 * r * gv * (grad_n2 * D1[3, :2]).sum(0)).max() > 0.05 * scale
 
 
 def test_aux_oracle_supercurrent_is_curl_of_flux_and_phase_difference_is_normalised_dot():
