@@ -80,11 +80,14 @@ def time_form(th, p, cfg, precision, steps, warmup, graph_steps, F0=None, autotu
     sim.synchronize()
     import torch
     torch.cuda.synchronize()
+    E_start = E
     t0 = time.perf_counter()
     E, err, ea = sim.steps(E, steps)
     sim.synchronize()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # how many timed steps ran with the velocity arrested (they read no V)
+    sim.timed_arrested = int(arrested_steps(sim.history(steps)[0], E_start).sum()) if steps > 0 else 0
     return sim, E, err, ea, dt
 
 
@@ -384,12 +387,18 @@ FETCH_SCALE = 2.0
 WRITE_SCALE = 1.0
 
 
+TRAFFIC_STEPS = 6
+
+
 def measure_traffic(child_args, kernel):
     """HBM bytes per launch of ``kernel`` (a kernel name as rocprof prints it,
     without namespace and arguments): two ``rocprofv3 --pmc`` runs (FETCH_SIZE,
-    then WRITE_SIZE) of a 3-step child ``bench.py --profile-only`` with the
-    same shapes, each scaled by the copy calibration.  None when rocprofv3 is
-    absent or a run fails."""
+    then WRITE_SIZE) of a child ``bench.py --traffic-child`` that runs
+    TRAFFIC_STEPS eager steps with the same form and shapes, each scaled by the
+    copy calibration, averaged over the child's launches of the kernel (one per
+    step).  Also returns which of the child's steps ran arrested, so that its
+    bytes are compared with the algorithmic bytes of the same launches.  None
+    when rocprofv3 is absent or a run fails."""
     import glob
     import re
     import shutil
@@ -400,30 +409,38 @@ def measure_traffic(child_args, kernel):
     if not prof:
         return None
     kib = {}
+    arrested = None
     with tempfile.TemporaryDirectory() as d:
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(d, ctr.lower())
             cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", ctr, "-d", out, "-o", "t", "--", sys.executable,
-                   os.path.abspath(__file__), "--profile-only", "--no-traffic", *child_args, "--steps", "3",
-                   "--warmup", "1", "--graph-steps", "0"]
-            print(f"[bench] traffic: rocprofv3 --pmc {ctr} (3 steps)", file=sys.stderr, flush=True)
+                   os.path.abspath(__file__), "--traffic-child", str(TRAFFIC_STEPS), *child_args]
+            print(f"[bench] traffic: rocprofv3 --pmc {ctr} ({TRAFFIC_STEPS} steps)", file=sys.stderr, flush=True)
             r = subprocess.run(cmd, capture_output=True, text=True, cwd=ROOT)
             dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
-            if r.returncode != 0 or not dbs:
+            marks = [ln for ln in r.stdout.splitlines() if ln.startswith("TRAFFIC_CHILD ")]
+            if r.returncode != 0 or not dbs or not marks:
                 print(f"[bench] traffic: rocprofv3 run failed (rc {r.returncode})", file=sys.stderr, flush=True)
                 return None
+            pattern = json.loads(marks[-1].split(" ", 1)[1])["arrested"]
+            if arrested is not None and pattern != arrested:
+                print("[bench] traffic: the two counter runs took different arrest decisions", file=sys.stderr,
+                      flush=True)
+                return None
+            arrested = pattern
             con = sqlite3.connect(dbs[0])
-            rows = con.execute("select kernel_name, avg(value) from counters_collection where counter_name = ? "
-                               "group by kernel_name", (ctr,)).fetchall()
+            rows = con.execute("select kernel_name, count(value), avg(value) from counters_collection "
+                               "where counter_name = ? group by kernel_name", (ctr,)).fetchall()
             con.close()
-            for name, v in rows:
+            for name, cnt, v in rows:
                 short = re.sub(r"\(.*$", "", re.sub(r"^void ", "", name)).replace("ss::", "")
-                if short == kernel:
+                if short == kernel and cnt == len(arrested):
                     kib[ctr] = float(v)
     if len(kib) != 2:
         return None
     return {"bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024 + kib["WRITE_SIZE"] * WRITE_SCALE * 1024,
-            "read_bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024, "write_bytes": kib["WRITE_SIZE"] * WRITE_SCALE * 1024}
+            "read_bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024, "write_bytes": kib["WRITE_SIZE"] * WRITE_SCALE * 1024,
+            "arrested": arrested}
 
 
 def schedule(tiles, comp):
@@ -472,6 +489,26 @@ def arrested_steps(energies, prev_energy):
     kill = np.zeros(len(e), dtype=bool)
     kill[1:] = e[:-1] > prevs[:-1]
     return kill
+
+
+def traffic_child(th, p, cfg, form, tiles, args):
+    """The counted child of ``measure_traffic``: ``args.traffic_child`` eager
+    steps of ``form`` from the seeded field, with the launch shapes ``tiles``
+    and the schedule of --fused23 / --fused41 / --pairs; prints which of them
+    ran arrested."""
+    import soliton_solver_amd as ssa
+    sim = ssa.Simulation(p, th, precision=form)
+    sim.initialize(cfg)
+    if tiles is not None:
+        sim.set_tile_variants(tiles)
+        sim.set_fused_stages(args.fused23 > 0, args.fused41 > 0, pairs=args.pairs > 0)
+    E0 = sim.compute_energy()
+    sim.steps(E0, args.traffic_child)
+    energies = sim.history(args.traffic_child)[0]
+    kills = arrested_steps(energies, E0)
+    sim.close()
+    print("TRAFFIC_CHILD " + json.dumps({"arrested": [bool(k) for k in kills], "E0": E0,
+                                         "energies": [float(e) for e in energies]}), flush=True)
 
 
 def cpu_baseline(args, n):
@@ -822,6 +859,9 @@ def main():
     ap.add_argument("--check-every", type=int, default=500)
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM bytes")
+    ap.add_argument("--traffic-child", type=int, default=0, metavar="K",
+                    help="only run K eager steps of the chosen form and shapes from the seeded field and print "
+                         "which of them ran arrested (the child that rocprofv3 --pmc counts bytes in)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -874,6 +914,9 @@ def main():
     comp = form == "fp32c"
     nf, w, N = 4, 4, n * n
     tiles_arg = [int(v) for v in args.tiles.split(",")] if args.tiles else None
+    if args.traffic_child:
+        traffic_child(th, p, cfg, form, tiles_arg, args)
+        return
     sim32, E32, err32, ea32, t32 = time_form(th, p, cfg, form, args.steps, args.warmup, args.graph_steps,
                                              autotune=not args.no_autotune, tiles=tiles_arg,
                                              fused23=args.fused23, fused41=args.fused41, pairs=args.pairs)
@@ -918,6 +961,7 @@ def main():
         sim64.close()
         s64 = schedule(tiles64, False)
         result64 = {"updates_per_s": ups64, "ms_per_step": t64 / args.steps * 1e3,
+                    "timed_steps_arrested": sim64.timed_arrested,
                     "hbm_frac_alg": ups64 * 16 * nf * 8 / (HBM_PEAK_GBS * 1e9),
                     "hbm_frac_schedule": ups64 * s64["step_words"] * nf * 8 / (HBM_PEAK_GBS * 1e9),
                     "words_per_update": s64["step_words"],
@@ -988,8 +1032,16 @@ def main():
                  "--reduced-form", form]
         traffic = measure_traffic(child, kernel)
         if traffic:
+            ka = np.asarray(traffic["arrested"], dtype=bool)
+            words_child = ((~ka).sum() * sched["dominant_words"] + ka.sum() * sched["arrested_words"]) / len(ka)
+            traffic["alg_bytes"] = words_child * nf * w * N
+            traffic["note"] = (f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over the {len(ka)} eager steps of a "
+                               f"child run from the seeded field ({int(ka.sum())} arrested), copy-calibrated; "
+                               f"{traffic['bytes'] / traffic['alg_bytes']:.2f}x the algorithmic bytes of those "
+                               "launches")
             print(f"[bench] traffic of {kernel}: {traffic['bytes'] / 1e9:.3f} GB per launch "
-                  f"({traffic['bytes'] / alg_dom:.2f}x algorithmic)", file=sys.stderr, flush=True)
+                  f"({traffic['bytes'] / traffic['alg_bytes']:.2f}x the algorithmic bytes of the same launches, "
+                  f"{int(ka.sum())} of {len(ka)} arrested)", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -1005,6 +1057,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "form": form,
+        "timed_steps_arrested": sim32.timed_arrested,
         "tile_variants": tiles32["variants"],
         "fused23": int(tiles32.get("fused23", False)),
         "fused41": int(tiles32["fused41"]),
@@ -1022,6 +1075,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic["bytes"] if traffic else None,
                      "traffic_read_write": [traffic["read_bytes"], traffic["write_bytes"]] if traffic else None,
+                     "traffic_alg_bytes_per_launch": traffic["alg_bytes"] if traffic else None,
+                     "traffic_note": traffic["note"] if traffic else None,
                      "kernel": kernel, "alg_words_per_site": words_dom * nf,
                      "alg_words_per_site_note": f"{sched['dominant_words'] * nf} per site in a step with the velocity "
                                                 f"free, {sched['arrested_words'] * nf} in an arrested one (no V read); "

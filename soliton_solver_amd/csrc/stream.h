@@ -835,8 +835,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
 #pragma unroll
     for (int a = 0; a < NF; ++a) {
       const R fo = ld<R>(plane<R>(b.F, a, NP), boff);
-      const R vr = (PASS >= 2) ? ld<R>(plane<R>(b.V, a, NP), boff) : R(0);
-      const R v = kill ? R(0) : vr;
+      const R v = (PASS >= 2 && !kill) ? ld<R>(plane<R>(b.V, a, NP), boff) : R(0);  // arrested: V not read
       const R l = (PASS == 3 || PASS == 23 || PASS == 34) ? ld<R>(plane<R>(b.l1, a, NP), boff)
                 : PASS == 4                               ? ld<R>(plane<R>(b.l2, a, NP), boff)
                                                           : R(0);
@@ -868,6 +867,10 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
   R keepV[RPW][NF], keepL[RPW][NF], holdV[RPW][NF], holdL[RPW][NF];
   // 4+1 sweep: l1 and l3 of the same sites (its front loads carry them already)
   R keepA[RPW][NF], keepC[RPW][NF], holdA[RPW][NF], holdC[RPW][NF];
+  // compensated 3+4 sweep: hi of the same sites, for the combine (a second
+  // read of hi there, rows after its prefetch, misses L2 about half the time)
+  constexpr bool kKeepF = PASS == 34 && COMP;
+  R keepF[kKeepF ? RPW : 1][NF], holdF[kKeepF ? RPW : 1][NF];
   if (PASS == 4 || PASS == 41 || PASS == 34) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
@@ -877,12 +880,18 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       const unsigned bh = inh ? (unsigned)(xh * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
-        const R v0 = ld<R>(plane<R>(b.V, a, NP), bk), q0 = ld<R>(plane<R>(b.l2, a, NP), bk);
-        const R v1 = ld<R>(plane<R>(b.V, a, NP), bh), q1 = ld<R>(plane<R>(b.l2, a, NP), bh);
-        keepV[k][a] = (ink && !kill) ? v0 : R(0);
+        const R v0 = kill ? R(0) : ld<R>(plane<R>(b.V, a, NP), bk);
+        const R v1 = kill ? R(0) : ld<R>(plane<R>(b.V, a, NP), bh);
+        const R q0 = ld<R>(plane<R>(b.l2, a, NP), bk), q1 = ld<R>(plane<R>(b.l2, a, NP), bh);
+        keepV[k][a] = ink ? v0 : R(0);
         keepL[k][a] = ink ? q0 : R(0);
-        holdV[k][a] = (inh && !kill) ? v1 : R(0);
+        holdV[k][a] = inh ? v1 : R(0);
         holdL[k][a] = inh ? q1 : R(0);
+        if constexpr (kKeepF) {
+          const R f0 = ld<R>(plane<R>(b.F, a, NP), bk), f1 = ld<R>(plane<R>(b.F, a, NP), bh);
+          keepF[k][a] = ink ? f0 : R(0);
+          holdF[k][a] = inh ? f1 : R(0);
+        }
         if (PASS == 34) {
           const R a0 = ld<R>(plane<R>(b.l1, a, NP), bk), a1 = ld<R>(plane<R>(b.l1, a, NP), bh);
           keepA[k][a] = ink ? a0 : R(0);
@@ -940,8 +949,9 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
       for (int k = 0; k < RPW; ++k)
 #pragma unroll
         for (int a = 0; a < NF; ++a) l3r[k][a] = R(0);
-      // compensated combine: hi and lo of this iteration's sites, issued
-      // before pass 3 so that their latency hides behind both variations
+      // compensated combine: hi of this iteration's sites (kept from the
+      // prefetch) and lo, issued before pass 3 so that its latency hides
+      // behind both variations
       float hl[COMP ? 2 * RPW : 1][NF];
       if constexpr (COMP) {
 #pragma unroll
@@ -950,7 +960,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
           const unsigned boff = (x < g.X && y < g.Y) ? (unsigned)(x * g.Y + y) * (unsigned)sizeof(R) : 0u;
 #pragma unroll
           for (int a = 0; a < NF; ++a) {
-            hl[k][a] = ldnt<float>(plane<float>(b.F, a, NP), boff);
+            hl[k][a] = keepF[k][a];
             hl[RPW + k][a] = ldnt<float>(plane<float>(b.Flo, a, NP), boff);
           }
         }
@@ -1015,6 +1025,11 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
               const R na = pf.l[k][a];
               keepA[k][a] = pinned(lag ? holdA[k][a] : na);
               holdA[k][a] = pinned(na);
+            }
+            if constexpr (kKeepF) {
+              const R nf = pf.f[k][a];
+              keepF[k][a] = pinned(lag ? holdF[k][a] : nf);
+              holdF[k][a] = pinned(nf);
             }
             if (PASS == 41) {
               const R na = pf.la[k][a], nc = pf.lc[k][a];

@@ -178,7 +178,7 @@ def test_parameter_gradient_by_backward_matches_energy_differences():
             fd = (vals[0] - 8.0 * vals[1] + 8.0 * vals[2] - vals[3]) / (12.0 * h)
             print(f"[dE/d{name}] backward {g[k]:.12e} 5-point {fd:.12e} rel {abs(g[k] - fd) / max(abs(fd), 1e-30):.1e}",
                   flush=True)
-            assert abs(g[k] - fd) <= 1e-8 * max(abs(fd), 1e-3 * abs(float(E)))
+            assert abs(g[k] - fd) <= 1e-8 * max(abs(fd), 1e-3 * abs(float(E.detach())))
     finally:
         sim.close()
 
