@@ -716,8 +716,8 @@ def differentiation_workload(args):
     t0 = time.perf_counter()
     # legs of 20 000 steps; a leg that shrinks max |EG| by less than 2x restarts the next one from V = 0
     E, nrel, restarts, prev = sim.compute_energy(), 0, 0, float("inf")
-    while nrel < args.max_steps:
-        E, ea, n = sim.minimize(tol=args.tol, max_steps=min(20000, args.max_steps - nrel), check_every=5000,
+    while nrel < args.diff_max_steps:
+        E, ea, n = sim.minimize(tol=args.tol, max_steps=min(20000, args.diff_max_steps - nrel), check_every=5000,
                                 prev_energy=E)
         nrel += n
         if ea < args.tol:
@@ -750,7 +750,7 @@ def differentiation_workload(args):
             s2 = ssa.Simulation(p.with_(**{name: v + sgn * d}), th, variation="exact_adjoint")
             s2.initialize({"mode": "array", "field": Fstar})
             s2.use_graphs(20)
-            E2, ea2, n2 = s2.minimize(tol=args.tol, max_steps=args.max_steps, check_every=5000)
+            E2, ea2, n2 = s2.minimize(tol=args.tol, max_steps=args.diff_max_steps, check_every=5000)
             Es.append(E2)
             steps.append([n2, ea2])
             s2.close()
@@ -823,6 +823,9 @@ def main():
                     help="differentiation workload: box side of the 512^2 lattice")
     ap.add_argument("--diff-dt", type=float, default=0.3,
                     help="differentiation workload: time step in units of the spacing")
+    ap.add_argument("--diff-max-steps", type=int, default=400000,
+                    help="differentiation workload: step budget of each relaxation (the first one's tail, from "
+                         "3e-6 to 1e-6, is slow: about 210 000 steps in all)")
     ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge", "differentiation"],
                     default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
