@@ -623,9 +623,9 @@ __device__ __forceinline__ void stream_rows(const Ring<R, NFR, SC>& q, const Pas
   constexpr bool kPre = PASS == 4 && sizeof(R) == 4 && FOFF == -1 && !COMP;
   R pl1[kPre ? RPW : 1][NF], pl3[kPre ? RPW : 1][NF];
   // Compensated pass 4 of the 3+4 sweep: hi and lo of this thread's sites come
-  // in kHL (hi [k][a], then lo), loaded by the caller before pass 3.  (The
-  // separate compensated pass 4 has no registers to spare for them and loads
-  // them at the combine.)
+  // in kHL (hi [k][a], then lo) from the caller: hi kept from its prefetch, lo
+  // loaded before pass 3.  (The separate compensated pass 4 has no registers
+  // to spare for them and loads both at the combine.)
   constexpr bool kPreC = PASS == 4 && COMP && FOFF == -2;
   if (kPre) {
 #pragma unroll
