@@ -566,7 +566,7 @@ def make_batch(seed, ninst):
     autotuned): sizes uniform in {64, 128, 256, 512}^2, xsize 20, dt = 0.0957 h,
     pion mass uniform in [0.1, 1], Skyrme coupling uniform in [0.5, 2],
     standard potential, Neel hedgehog of charge 1..4 plus U(-1e-3, 1e-3).
-    Returns (batch, sizes, charges, autotune info)."""
+    Returns (batch, sizes, charges, autotune info, pion masses, Skyrme couplings)."""
     import soliton_solver_amd as ssa
     from soliton_solver_amd.batch import BatchSimulation
     th = ssa.load_theory("Baby Skyrme model")
@@ -584,7 +584,7 @@ def make_batch(seed, ninst):
     bs.initialize([{"mode": "initial", "ansatz": "neel", "skyrmion_number": int(q), "noise": 1e-3, "seed": i}
                    for i, q in enumerate(charge)])
     tinfo = bs.autotune_tiles(reps=5)
-    return bs, sizes, charge, tinfo
+    return bs, sizes, charge, tinfo, mpi, kappa
 
 
 def batch_throughput(args, steps=100, warmup=20):
@@ -592,7 +592,7 @@ def batch_throughput(args, steps=100, warmup=20):
     0, so none retires): ``warmup`` steps (the step graphs are captured there),
     then ``steps`` timed steps in graphs of 20."""
     import torch
-    bs, sizes, charge, tinfo = make_batch(args.seed, args.instances)
+    bs, sizes, charge, tinfo, _, _ = make_batch(args.seed, args.instances)
     bs.use_graphs(20)
     bs.begin()
     bs.run(warmup, 0.0, check_every=warmup)
@@ -623,7 +623,7 @@ def batch_workload(args):
     each reaches --tol, within --batch-max-steps steps in all."""
     import torch
     ninst = args.instances
-    bs, sizes, charge, tinfo = make_batch(args.seed, ninst)
+    bs, sizes, charge, tinfo, mpi, kappa = make_batch(args.seed, ninst)
     print(f"[bench] batch tile variant {tinfo['variant']} ({tinfo['rows']} rows), fused 2+3: {tinfo['fused23']}; "
           "ms per variant " + " ".join(f"{x:.3f}" for x in tinfo["ms"]) + "; with 2+3 fused "
           + " ".join(f"{x:.3f}" for x in tinfo["ms_fused23"]) + "; paired sweeps "
@@ -654,6 +654,12 @@ def batch_workload(args):
                  for q in range(1, 5)}
     by_size = {int(n): {"instances": int((sizes == n).sum()), "converged": int(((sizes == n) & (conv >= 0)).sum())}
                for n in (64, 128, 256, 512)}
+    mbins = ((0.1, 0.4), (0.4, 0.7), (0.7, 1.0))
+    by_mass = {f"{lo:.1f}-{hi:.1f}": {q: {"instances": int(((mpi >= lo) & (mpi < hi) & (charge == q)).sum()),
+                                           "converged": int(((mpi >= lo) & (mpi < hi) & (charge == q)
+                                                             & (conv >= 0)).sum())}
+                                      for q in range(1, 5)}
+               for lo, hi in mbins}
     out = {
         "metric": "grid-pt updates/s, batched relaxations (sum over instances of sites x steps / wall time)",
         "value": updates / wall,
@@ -675,6 +681,11 @@ def batch_workload(args):
         "failed": res["failed"],
         "converged_by_charge": by_charge,
         "converged_by_size": by_size,
+        "converged_by_pion_mass_and_charge": by_mass,
+        "per_instance_note": "size, charge, pion mass, Skyrme coupling, step converged at (-1: not within the "
+                             "budget, -2: failed), last max|EG|",
+        "per_instance": [[int(n), int(q), round(float(m), 4), round(float(k), 4), int(c), float(f"{e:.3e}")]
+                         for n, q, m, k, c, e in zip(sizes, charge, mpi, kappa, conv, res["err_absmax"])],
         "steps_fp32_max": int(res["steps_reduced"].max()),
         "steps_fp64_max": int(res["steps_fp64"].max()),
     }
