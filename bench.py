@@ -436,6 +436,9 @@ def measure_traffic(child_args, kernel):
                 short = re.sub(r"\(.*$", "", re.sub(r"^void ", "", name)).replace("ss::", "")
                 if short == kernel and cnt == len(arrested):
                     kib[ctr] = float(v)
+                elif short == kernel:
+                    print(f"[bench] traffic: {kernel} ran {cnt} times in the child, not once per step",
+                          file=sys.stderr, flush=True)
     if len(kib) != 2:
         return None
     return {"bytes": kib["FETCH_SIZE"] * FETCH_SCALE * 1024 + kib["WRITE_SIZE"] * WRITE_SCALE * 1024,
