@@ -961,12 +961,30 @@ def main():
     print(f"[bench] {form} form: {t32 / args.steps * 1e3:.3f} ms/step, {ups32:.3e} updates/s", file=sys.stderr,
           flush=True)
 
-    # dominant-kernel roofline from HIP events on the context stream; its
-    # algorithmic bytes per launch follow the arrest decisions of those steps
-    # (an arrested step reads no V)
-    nroof = 20
-    passes = sim32.time_passes(E32, nroof)
-    kills = arrested_steps(sim32.history(nroof)[0], E32)
+    # dominant-kernel roofline from HIP events on the context stream, over the
+    # timed steps: a second context with the same seeded field, shapes and
+    # warm-up reaches the state they started from and replays them eagerly,
+    # pass by pass (the same trajectory bit for bit); its algorithmic bytes per
+    # launch follow the arrest decisions of those steps (an arrested step reads
+    # no V)
+    timed_arrested = sim32.timed_arrested
+    sim32.close()
+    simr = ssa.Simulation(p, th, precision=form)
+    simr.initialize(cfg)
+    simr.set_tile_variants(tiles32["variants"])
+    simr.set_fused_stages(bool(tiles32.get("fused23", False)), bool(tiles32["fused41"]), pairs=bool(tiles32["pairs"]))
+    if args.graph_steps:
+        simr.use_graphs(args.graph_steps)
+    E_start = simr.compute_energy()
+    E_start, _, _ = simr.steps(E_start, args.warmup)
+    nroof = args.steps
+    passes = simr.time_passes(E_start, nroof)
+    E_replay = simr.history(nroof)[0]
+    simr.close()
+    kills = arrested_steps(E_replay, E_start)
+    if E_replay[-1] != E32 or int(kills.sum()) != timed_arrested:
+        print(f"[bench] roofline replay: last energy {E_replay[-1]!r} against {E32!r}, {int(kills.sum())} "
+              f"arrests against {timed_arrested}", file=sys.stderr, flush=True)
     sched = schedule(tiles32, comp)
     words_dom = ((nroof - kills.sum()) * sched["dominant_words"] + kills.sum() * sched["arrested_words"]) / nroof
     alg_dom = words_dom * nf * w * N
@@ -974,7 +992,6 @@ def main():
     step_ms_kernels = float(passes.sum())
     alg_step = sched["step_words"] * nf * w * N
     step_gbs = alg_step / (step_ms_kernels * 1e-3) / 1e9
-    sim32.close()
 
     result64 = None
     if not args.profile_only:
@@ -1082,7 +1099,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "form": form,
-        "timed_steps_arrested": sim32.timed_arrested,
+        "timed_steps_arrested": timed_arrested,
         "tile_variants": tiles32["variants"],
         "fused23": int(tiles32.get("fused23", False)),
         "fused41": int(tiles32["fused41"]),
@@ -1106,6 +1123,11 @@ def main():
                      "alg_words_per_site_note": f"{sched['dominant_words'] * nf} per site in a step with the velocity "
                                                 f"free, {sched['arrested_words'] * nf} in an arrested one (no V read); "
                                                 f"{int(kills.sum())} of the {nroof} timed steps arrested",
+                     "launches_note": f"the {nroof} timed steps replayed in a second context from the same seeded "
+                                      "field, shapes and warm-up, eagerly, each pass between HIP events on the "
+                                      "context's stream; the replay "
+                                      + ("ends on the timed run's energy bit for bit"
+                                         if E_replay[-1] == E32 else "does NOT end on the timed run's energy"),
                      "alg_bytes_per_launch": alg_dom, "avg_launch_ms": passes[4],
                      "step_kernels_ms": step_ms_kernels, "step_kernels_gbs": step_gbs,
                      "pass_ms": [float(x) for x in passes]},
