@@ -870,7 +870,7 @@ __device__ __forceinline__ void stream_strip(typename StreamRing<TH, R, PASS, CW
   // compensated 3+4 sweep: hi of the same sites, for the combine (a second
   // read of hi there, rows after its prefetch, misses L2 about half the time)
   constexpr bool kKeepF = PASS == 34 && COMP;
-  R keepF[kKeepF ? RPW : 1][NF], holdF[kKeepF ? RPW : 1][NF];
+  R keepF[RPW][NF], holdF[RPW][NF];
   if (PASS == 4 || PASS == 41 || PASS == 34) {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
