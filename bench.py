@@ -481,6 +481,15 @@ def schedule(tiles, comp):
             "step_words": 2 + (5 if f23 else 7) + 7, "kernel": kern}
 
 
+def launch_words(sched, kills):
+    """Mean algorithmic words per site and component of the dominant launch
+    over steps whose arrest decisions are ``kills``: the schedule's
+    ``dominant_words`` in a step with the velocity free, ``arrested_words`` in
+    an arrested one (V known to be zero and not read)."""
+    k = np.asarray(kills, dtype=bool)
+    return float(((~k).sum() * sched["dominant_words"] + k.sum() * sched["arrested_words"]) / len(k))
+
+
 def arrested_steps(energies, prev_energy):
     """Which steps of a call ran with the velocity arrested: the arrest of step
     s >= 1 is E(F_s) > E(F_{s-1}) (core/integrator.py:338-343), with E(F_0) the
@@ -986,7 +995,7 @@ def main():
         print(f"[bench] roofline replay: last energy {E_replay[-1]!r} against {E32!r}, {int(kills.sum())} "
               f"arrests against {timed_arrested}", file=sys.stderr, flush=True)
     sched = schedule(tiles32, comp)
-    words_dom = ((nroof - kills.sum()) * sched["dominant_words"] + kills.sum() * sched["arrested_words"]) / nroof
+    words_dom = launch_words(sched, kills)
     alg_dom = words_dom * nf * w * N
     dom_gbs = alg_dom / (passes[4] * 1e-3) / 1e9
     step_ms_kernels = float(passes.sum())
@@ -1075,8 +1084,7 @@ def main():
         traffic = measure_traffic(child, kernel)
         if traffic:
             ka = np.asarray(traffic["arrested"], dtype=bool)
-            words_child = ((~ka).sum() * sched["dominant_words"] + ka.sum() * sched["arrested_words"]) / len(ka)
-            traffic["alg_bytes"] = words_child * nf * w * N
+            traffic["alg_bytes"] = launch_words(sched, ka) * nf * w * N
             traffic["note"] = (f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over the {len(ka)} eager steps of a "
                                f"child run from the seeded field ({int(ka.sum())} arrested), copy-calibrated; "
                                f"{traffic['bytes'] / traffic['alg_bytes']:.2f}x the algorithmic bytes of those "

@@ -127,3 +127,26 @@ def test_bench_arrested_steps_follow_the_energy_history():
     # E(F_1..F_5) = 5.0, 4.0, 4.5, 4.4, 4.6 against 6.0, 5.0, 4.0, 4.5, 4.4
     assert kills.tolist() == [False, False, False, True, False, True]
     assert arrested_steps([7.0], 6.0).tolist() == [False]
+
+
+def test_bench_schedule_words_and_launch_pricing():
+    """The words per site and component bench.py prices each schedule at.  The
+    unfused 4-pass schedule moves 2 + 3 + 4 + 7 = 16 words per update, the
+    fixed algorithmic count (16 nf w bytes); the paired sweeps move 4 + 6 (F, V
+    -> l1, l2; F, V, l1, l2 -> F', V'), the compensated ones 5 + 8 (the low
+    words: read in the 1+2 sweep, read and written in the 3+4 sweep).  An
+    arrested 3+4 sweep reads no V and takes l2 = l1: 2 words fewer."""
+    from bench import launch_words, schedule
+    base = {"rows": [64, 64, 64, 64], "fused23": 0, "fused41": 0, "pairs": 0, "variants": [3, 3, 3, 3]}
+    plain = schedule(base, False)
+    assert (plain["step_words"], plain["dominant_words"], plain["arrested_words"]) == (16, 7, 6)
+    pairs = schedule(dict(base, pairs=1), False)
+    assert (pairs["step_words"], pairs["dominant_words"], pairs["arrested_words"]) == (10, 6, 4)
+    assert pairs["kernel"] == "stream_pass<ChiralMagnet, float, 34, 1>"
+    comp = schedule(dict(base, pairs=1, comp12=1, variants=[3, 6, 6, 6]), True)
+    assert (comp["step_words"], comp["dominant_words"], comp["arrested_words"]) == (13, 8, 6)
+    assert comp["kernel"] == "stream_pass_comp<ChiralMagnet, float, 34, 2>"
+    # mean words over the launches: free steps at 8, arrested ones at 6
+    assert launch_words(comp, [False, True, False, False]) == (3 * 8 + 6) / 4
+    assert launch_words(comp, [False] * 5) == 8.0
+    assert launch_words(comp, [True] * 3) == 6.0
