@@ -281,12 +281,12 @@ constexpr int kMaxDevices = 64;
 static double* g_scratch[kMaxDevices] = {nullptr};
 static std::mutex g_scratch_mu;
 
-// Resolves the scratch array's address on device ``dev`` (once per device,
-// with that device current); 0 on success.
 // Packed p_f entries 0..5 are the lattice geometry (core/params.py:247-248);
 // the theories' coupling constants follow.
 constexpr int kCorePF = 6;
 
+// Resolves the scratch array's address on device ``dev`` (once per device,
+// with that device current); 0 on success.
 static int scratch_setup(int dev, std::string* err) {
   if (dev < 0 || dev >= kMaxDevices) {
     if (err) *err = "device id out of range for the static scratch table";
