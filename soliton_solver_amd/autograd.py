@@ -89,19 +89,22 @@ def packed_parameter_jacobian(sim, names, rel_step: float = 1e-6) -> np.ndarray:
     """J[j, k] = dp_f[j] / dtheta_k of the theory's packing
     (``pack_device_params(params.resolved())``) at the simulation's parameters,
     host-side: entries that copy a parameter are exactly 1 (the central
-    difference of an identity), derived ones (e.g. the anisotropic theory's u1,
-    u2 from alpha, beta: anisotropic_superconductor/params.py:190-243) are the
-    central difference of that closed-form algebra at step rel_step |theta_k|
-    (truncation ~ step^2, rounding ~ 1e-16/step: ~1e-10 relative)."""
+    difference of an identity, divided by the step actually taken: vp - vm is
+    exact, the two values being within a factor 2), derived ones (e.g. the
+    anisotropic theory's u1, u2 from alpha, beta:
+    anisotropic_superconductor/params.py:190-243) are the central difference
+    of that closed-form algebra at step rel_step |theta_k| (truncation ~
+    step^2, rounding ~ 1e-16/step: ~1e-10 relative)."""
     base = sim.p
     th = sim.theory
     J = np.zeros((int(sim.p_f_h.size), len(names)))
     for k, name in enumerate(names):
         v = float(getattr(base, name))
         h = rel_step * max(abs(v), 1.0)
-        _, pf_p = th.params.pack_device_params(base.with_(**{name: v + h}).resolved())
-        _, pf_m = th.params.pack_device_params(base.with_(**{name: v - h}).resolved())
-        J[:, k] = (np.asarray(pf_p, dtype=np.float64) - np.asarray(pf_m, dtype=np.float64)) / (2.0 * h)
+        vp, vm = v + h, v - h
+        _, pf_p = th.params.pack_device_params(base.with_(**{name: vp}).resolved())
+        _, pf_m = th.params.pack_device_params(base.with_(**{name: vm}).resolved())
+        J[:, k] = (np.asarray(pf_p, dtype=np.float64) - np.asarray(pf_m, dtype=np.float64)) / (vp - vm)
     J[:6, :] = 0.0  # the lattice geometry does not depend on the couplings
     return J
 

@@ -150,3 +150,32 @@ def test_bench_schedule_words_and_launch_pricing():
     assert launch_words(comp, [False, True, False, False]) == (3 * 8 + 6) / 4
     assert launch_words(comp, [False] * 5) == 8.0
     assert launch_words(comp, [True] * 3) == 6.0
+
+
+def test_packed_parameter_jacobian_copies_are_exactly_one():
+    """dp_f/dtheta of the anisotropic packing (host-side, no GPU): entries
+    that copy a parameter are exactly 1 (the difference quotient divides by
+    the step actually taken), the geometry rows are 0, and a derived entry
+    matches a central difference at a tenth of the step."""
+    import numpy as np
+    import soliton_solver_amd as ssa
+    from soliton_solver_amd import autograd as sag
+    from tests import helpers as H
+    th = ssa.load_theory("Anisotropic superconductor")
+    p = H.as_params(th, 32, 32, xsize=10.0, seed=1)
+
+    class _Sim:  # what packed_parameter_jacobian reads of a Simulation
+        pass
+    sim = _Sim()
+    sim.p, sim.theory = p, th
+    sim.p_f_h = np.asarray(th.params.pack_device_params(p.resolved())[1], dtype=np.float64)
+    names = ["alpha1", "alpha2", "beta1", "beta2", "beta3", "beta4", "kappa"]
+    J = sag.packed_parameter_jacobian(sim, names)
+    assert np.all(J[:6] == 0.0)
+    copies = np.abs(J - 1.0) < 1e-6
+    assert copies.sum() >= len(names)  # every named parameter is packed at least once as itself
+    assert np.all(J[copies] == 1.0)
+    J10 = sag.packed_parameter_jacobian(sim, names, rel_step=1e-7)
+    derived = (np.abs(J) > 1e-3) & ~copies
+    assert derived.any()
+    assert np.allclose(J[derived], J10[derived], rtol=1e-6)
