@@ -705,6 +705,19 @@ def batch_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def anisotropic_betas(seed):
+    """beta1..beta4 ~ U[0.2, 1] from ``seed`` (alpha1 = -1, alpha2 = -0.5),
+    redrawn until the quartic is bounded below with real, non-zero vacua."""
+    rng = np.random.default_rng(seed)
+    while True:
+        b1, b2, b3, b4 = rng.uniform(0.2, 1.0, size=4)
+        b3t = b3 - 2.0 * b4
+        den = b3t * b3t - 4.0 * b1 * b2
+        n1, n2 = 2.0 * b2 * -1.0 - b3t * -0.5, 2.0 * b1 * -0.5 - b3t * -1.0
+        if b3t > -2.0 * np.sqrt(b1 * b2) + 0.1 and den != 0 and n1 / den > 0 and n2 / den > 0:
+            return float(b1), float(b2), float(b3), float(b4)
+
+
 def differentiation_workload(args):
     """Anisotropic s+id Ginzburg-Landau, 512 x 512, FP64, the exact-adjoint
     flow (the variation is dE_h/dphi of the discrete energy): alpha1 = -1,
@@ -721,14 +734,7 @@ def differentiation_workload(args):
     th = ssa.load_theory("Anisotropic superconductor")
     n, xs = 512, args.diff_xsize
     h = xs / (n - 1)
-    rng = np.random.default_rng(1)
-    while True:  # beta1..beta4 ~ U[0.2, 1], redrawn until the quartic is bounded below with real vacua
-        b1, b2, b3, b4 = rng.uniform(0.2, 1.0, size=4)
-        b3t = b3 - 2.0 * b4
-        den = b3t * b3t - 4.0 * b1 * b2
-        n1, n2 = 2.0 * b2 * -1.0 - b3t * -0.5, 2.0 * b1 * -0.5 - b3t * -1.0
-        if b3t > -2.0 * np.sqrt(b1 * b2) + 0.1 and den != 0 and n1 / den > 0 and n2 / den > 0:
-            break
+    b1, b2, b3, b4 = anisotropic_betas(1)
     p = th.params.default_params(xlen=n, ylen=n, xsize=xs, ysize=xs, alpha1=-1.0, alpha2=-0.5, beta1=b1, beta2=b2,
                                  beta3=b3, beta4=b4, gamma1=1.0, gamma2=1.0, gamma12=0.3, Q=1.0, kappa=0.673274,
                                  time_step=args.diff_dt * h)
@@ -823,6 +829,155 @@ def differentiation_workload(args):
     print(json.dumps(out), flush=True)
 
 
+# The nine theories as the theories workload runs them: registry name, oracle
+# module (the CPU baseline), parameters besides the lattice, box side, time
+# step in units of the spacing (None: the theory's default 0.5) and initial
+# condition.
+THEORY_CASES = (
+    ("Chiral magnet", "chiral_magnet",
+     dict(J=40e-12, K=0.8e6, D=4e-3, M=580e3, B=0.0, dmi_term="Heusler", demag=True, unit_magnetization=True),
+     10.0, None, {"mode": "initial", "ansatz": "anti", "noise": 1e-3}),
+    ("Ginzburg-Landau superconductor", "ginzburg_landau", dict(q=1.0, Lambda=1.0, u1=1.0, vortex_number=1.0),
+     25.4, None, {"mode": "vortex", "noise": 1e-3}),
+    ("Baby Skyrme model", "baby_skyrme", dict(mpi=1.0, kappa=1.0, unit_magnetization=True),
+     20.0, 0.0957, {"mode": "initial", "ansatz": "neel", "skyrmion_number": 2, "noise": 1e-3}),
+    ("Anisotropic superconductor", "anisotropic_superconductor",
+     dict(alpha1=-1.0, alpha2=-0.5, gamma1=1.0, gamma2=1.0, gamma12=0.3, Q=1.0, kappa=0.673274),
+     25.0, 0.3, {"mode": "initial", "noise": 1e-2}),
+    ("Bose-Einstein condensate", "bose_einstein_condensate", dict(beta=50.0, omega_rot=0.5, vortex_number=2.0),
+     12.0, None, {"mode": "ground", "noise": 1e-3}),
+    ("Anyon superconductor", "maxwell_chern_simons_higgs", dict(q=1.0, Lambda=1.5, kappa=0.7, u1=1.0),
+     12.0, None, {"mode": "initial", "noise": 1e-2}),
+    ("Ferromagnetic superconductor", "ferromagnetic_superconductor",
+     dict(q=0.8, eta1=-0.1, gamma=1.2, unit_magnetization=True),
+     16.0, None, {"mode": "initial", "ansatz": "bloch", "vortex_number": 1.0, "noise": 1e-2}),
+    ("Spin triplet superconducting ferromagnet", "spin_triplet", dict(q=0.9, gamma=1.1, unit_magnetization=True),
+     16.0, None, {"mode": "initial", "ansatz": "bloch", "noise": 1e-2}),
+    ("Liquid crystal", "liquid_crystal", dict(unit_magnetization=True),
+     12.0, None, {"mode": "initial", "ansatz": "bloch", "noise": 1e-2}),
+)
+
+
+def theory_params(th, name, kw, xsize, dt_h, n):
+    extra = dict(kw)
+    if name == "Anisotropic superconductor":
+        extra.update(zip(("beta1", "beta2", "beta3", "beta4"), anisotropic_betas(1)))
+    if dt_h is not None:
+        extra["time_step"] = dt_h * xsize / (n - 1)
+    return th.params.default_params(xlen=n, ylen=n, xsize=xsize, ysize=xsize, **extra)
+
+
+def theories_workload(args):
+    """Every theory on one n x n lattice (--theories-n, default 2048): the FP64
+    form and the reduced form the theory has (compensated FP32, else plain
+    FP32; the L2-normalised BEC flow has FP64 storage only), each autotuned,
+    --warmup steps then --steps timed in graphs of 20 -> updates/s and the
+    fraction of 8 TB/s at the fixed 16 nf w bytes per update; the launch that
+    finishes a step timed by HIP events over the 20 steps that follow and
+    priced at its algorithmic words with those steps' arrest decisions; and,
+    beside it, the single-thread NumPy oracle on a 128 x 128 lattice of the
+    same theory (--theories-cpu-steps steps)."""
+    import importlib
+    import torch
+    import soliton_solver_amd as ssa
+    from oracle import core as ocore
+    n, steps, warmup = args.theories_n, args.steps, args.warmup
+    N = n * n
+    out = {}
+    for name, oname, kw, xsize, dt_h, cfg in THEORY_CASES:
+        th = ssa.load_theory(name)
+        p = theory_params(th, name, kw, xsize, dt_h, n)
+        cfg = dict(cfg, seed=1)
+        forms = {}
+        for form in ("fp64", "fp32c", "fp32"):
+            if form == "fp32" and "refused" not in forms.get("fp32c", {"refused": ""}):
+                continue  # the compensated form ran: the plain one is not needed
+            try:  # a form the theory does not have is refused at creation or at its first step
+                sim = ssa.Simulation(p, th, precision=form)
+            except (ValueError, RuntimeError) as e:
+                forms[form] = {"refused": str(e)}
+                continue
+            try:
+                sim.initialize(cfg)
+                try:
+                    tiles = sim.autotune_tiles(reps=5)
+                except ValueError:  # the L2-normalised flow has fixed launch shapes
+                    tiles = sim.tile_variants()
+                sim.use_graphs(args.graph_steps)
+                E = sim.compute_energy()
+                E, _, _ = sim.steps(E, warmup)
+                sim.prepare_graphs(steps)
+            except (ValueError, RuntimeError) as e:
+                sim.close()
+                forms[form] = {"refused": str(e)}
+                continue
+            nf = int(sim.p_i_h[4])
+            w = 8 if form == "fp64" else 4
+            sim.synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            E1, _, ea = sim.steps(E, steps)
+            sim.synchronize()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            timed_kills = arrested_steps(sim.history(steps)[0], E)
+            try:
+                passes = sim.time_passes(E1, 20)
+                kills = arrested_steps(sim.history(20)[0], E1)
+            except ValueError:  # the L2-normalised flow is timed as a whole only
+                passes, kills = np.zeros(6), None
+            sim.close()
+            ups = N * steps / dt
+            res = {"ms_per_step": dt / steps * 1e3, "updates_per_s": ups, "nf": nf,
+                   "hbm_frac_alg": ups * 16 * nf * w / (HBM_PEAK_GBS * 1e9),
+                   "timed_steps_arrested": int(timed_kills.sum()), "energy_after": E1, "err_absmax_after": ea,
+                   "tile_variants": tiles["variants"], "paired_sweeps": bool(tiles["pairs"]),
+                   "pass_ms": [round(float(x), 5) for x in passes]}
+            if kills is not None and passes[4] > 0:
+                sched = schedule(tiles, form == "fp32c")
+                words = launch_words(sched, kills)
+                gbs = words * nf * w * N / (passes[4] * 1e-3) / 1e9
+                res.update({"schedule": sched["name"], "words_per_update": sched["step_words"],
+                            "hbm_frac_schedule": ups * sched["step_words"] * nf * w / (HBM_PEAK_GBS * 1e9),
+                            "dominant_launch": {"ms": float(passes[4]), "alg_words_per_site": words * nf,
+                                                "steps_arrested": f"{int(kills.sum())} of 20",
+                                                "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}})
+            else:
+                res["schedule"] = "L2-normalised flow: stage passes plus a normalisation of every stage field"
+            forms[form] = res
+            print(f"[bench] {name} {form}: {res['ms_per_step']:.3f} ms/step, {ups:.3e} updates/s, "
+                  f"{100 * res['hbm_frac_alg']:.1f} % of 8 TB/s at 16 nf w"
+                  + (f", finishing launch {res['dominant_launch']['achieved_gbs'] / 1e3:.2f} TB/s"
+                     if "dominant_launch" in res else ""), file=sys.stderr, flush=True)
+        # the single-thread NumPy oracle on a 128 x 128 lattice of the same theory
+        o = importlib.import_module(f"oracle.theories.{oname}")
+        pc = theory_params(th, name, kw, xsize, dt_h, 128)
+        p_i, p_f = th.params.pack_device_params(pc.resolved())
+        p_i = np.ascontiguousarray(p_i, dtype=np.int32)
+        p_f = np.ascontiguousarray(p_f, dtype=np.float64)
+        X, Y, nfc = int(p_i[0]), int(p_i[1]), int(p_i[4])
+        g = np.empty((2, X, Y))
+        g[0] = p_f[2] * np.arange(X)[:, None]
+        g[1] = p_f[3] * np.arange(Y)[None, :]
+        F = np.zeros((nfc, X, Y))
+        V = np.zeros_like(F)
+        th.initial_config.initialize(Field=F, Velocity=V, grid=g, p_i_h=p_i, p_f_h=p_f, config=cfg)
+        E0 = ocore.theory_energy(o, F, p_i, p_f)
+        t0 = time.perf_counter()
+        ocore.relax(o, F, V, p_i, p_f, args.theories_cpu_steps, prev_energy=E0)
+        tc = time.perf_counter() - t0
+        cpu = {"value": X * Y * args.theories_cpu_steps / tc, "unit": "grid-point updates/s", "cores": 1,
+               "kind": "port", "sample": f"NumPy oracle, 128x128, {args.theories_cpu_steps} steps, {tc:.2f} s"}
+        print(f"[bench] {name} NumPy oracle: {cpu['value']:.3e} updates/s", file=sys.stderr, flush=True)
+        out[name] = {"forms": forms, "cpu_baseline": cpu}
+    print(json.dumps({"metric": "grid-pt updates/s + %HBM peak per theory",
+                      "unit": "grid-point updates/s", "n_gpus": 1, "data": "synthetic", "steps": steps,
+                      "warmup": warmup,
+                      "config": {"workload": f"each of the nine theories on {n}x{n}, its own parameters and "
+                                             "initial condition (bench.py THEORY_CASES)", "lattice": [n, n]},
+                      "theories": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -857,7 +1012,11 @@ def main():
     ap.add_argument("--diff-max-steps", type=int, default=400000,
                     help="differentiation workload: step budget of each relaxation (the first one's tail, from "
                          "3e-6 to 1e-6, is slow: about 210 000 steps in all)")
-    ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge", "differentiation"],
+    ap.add_argument("--theories-n", type=int, default=2048, help="theories workload: lattice side")
+    ap.add_argument("--theories-cpu-steps", type=int, default=20,
+                    help="theories workload: steps of the NumPy oracle on each theory's 128^2 sample")
+    ap.add_argument("--workload", choices=["throughput", "precision", "batch", "converge", "differentiation",
+                                           "theories"],
                     default="throughput",
                     help="precision: time-to-converge of the FP32 form + FP64 polish against the FP64 form; "
                          "batch: many baby-Skyrme relaxations of uneven sizes; converge: time-to-converge of the "
@@ -935,6 +1094,9 @@ def main():
         return
     if args.workload == "converge":
         converge_workload(args)
+        return
+    if args.workload == "theories":
+        theories_workload(args)
         return
     if args.workload == "differentiation":
         args.tol = 1e-6 if args.tol is None else args.tol
