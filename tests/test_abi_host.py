@@ -179,3 +179,30 @@ def test_packed_parameter_jacobian_copies_are_exactly_one():
     derived = (np.abs(J) > 1e-3) & ~copies
     assert derived.any()
     assert np.allclose(J[derived], J10[derived], rtol=1e-6)
+
+
+def test_bench_theory_cases_initialise_and_step_in_the_oracle():
+    """Each theory of the theories workload builds its parameters and initial
+    condition and takes one finite step of the oracle's flow (32^2 lattice)."""
+    import importlib
+    import numpy as np
+    import soliton_solver_amd as ssa
+    from bench import THEORY_CASES, theory_params
+    from oracle import core as ocore
+    assert len(THEORY_CASES) == 9
+    for name, oname, kw, xsize, dt_h, cfg in THEORY_CASES:
+        th = ssa.load_theory(name)
+        p_i, p_f = th.params.pack_device_params(theory_params(th, name, kw, xsize, dt_h, 32).resolved())
+        p_i = np.ascontiguousarray(p_i, dtype=np.int32)
+        p_f = np.ascontiguousarray(p_f, dtype=np.float64)
+        X, Y, nf = int(p_i[0]), int(p_i[1]), int(p_i[4])
+        g = np.empty((2, X, Y))
+        g[0] = p_f[2] * np.arange(X)[:, None]
+        g[1] = p_f[3] * np.arange(Y)[None, :]
+        F = np.zeros((nf, X, Y))
+        V = np.zeros_like(F)
+        th.initial_config.initialize(Field=F, Velocity=V, grid=g, p_i_h=p_i, p_f_h=p_f, config=dict(cfg, seed=1))
+        o = importlib.import_module(f"oracle.theories.{oname}")
+        E0 = ocore.theory_energy(o, F, p_i, p_f)
+        ocore.relax(o, F, V, p_i, p_f, 1, prev_energy=E0)
+        assert np.isfinite(E0) and np.isfinite(F).all(), name
