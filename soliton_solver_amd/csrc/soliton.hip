@@ -262,11 +262,6 @@ __global__ void k_fold(float* hi, const float* lo, long long n) {
     hi[i] = (float)((double)hi[i] + (double)lo[i]);
 }
 
-__global__ void k_scale_into(double* dst, const double* src, long long n, double a) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    dst[i] = a * src[i];
-}
-
 // ---------------------------------------------------------------------------
 // Library-held scratch of ss_energy_dev_noalloc: the per-tile energy partials
 // live in this file-scope device array instead of memory the library
@@ -948,13 +943,18 @@ static int launch_step_l2(ss_ctx* c, int parity) {
     if (l2_normalize_buf(c, (double*)c->temp)) return -1;
     PassBufs v{};
     v.F = c->temp;
-    v.lout = c->G;
     v.st = c->st;
-    HIPCHK(c, c->ops->pass(c->prec, 0, 0, v, c->tp, c->g, c->stream));
     if (st < 3) {
-      hipLaunchKernelGGL(k_scale_into, grid, dim3(256), 0, c->stream, (double*)c->l[st], (const double*)c->G, n,
-                         -c->g.dt);
-      HIPCHK(c, hipGetLastError());
+      // l_{st+1} = -dt EG of the normalised stage field, written by the
+      // variation pass itself (pass 1 on that field: the same EG, times -dt);
+      // its energy partials land in c->part, whose energy this step's
+      // finalize has already read
+      v.lout = c->l[st];
+      v.part0 = c->part;
+      HIPCHK(c, c->ops->pass(c->prec, 1, 0, v, c->tp, c->g, c->stream));
+    } else {  // stage 4: EG itself, which the combine takes
+      v.lout = c->G;
+      HIPCHK(c, c->ops->pass(c->prec, 0, 0, v, c->tp, c->g, c->stream));
     }
   }
   b.part0 = c->xpart;
