@@ -1057,3 +1057,41 @@ def test_observables_accept_the_reference_call_convention():
     with pytest.raises(TypeError):
         ob.compute_energy(np.zeros(3))
     sim.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32c", "fp64"])
+def test_two_contexts_from_the_same_seed_agree_bit_for_bit(precision):
+    """bench.py times its roofline on a second context that repeats the timed
+    run's set-up (the same seeded field, launch shapes and warm-up) and then
+    runs the timed steps eagerly, pass by pass.  That second run ends on the
+    same energies, field and velocity as the first, bit for bit (paired
+    sweeps; graphs of 4 steps in the first)."""
+    th = ssa.load_theory("Chiral magnet")
+    p = H.chiral_params(th, 256, 240)
+    cfg = {"mode": "initial", "ansatz": "anti", "noise": 1e-3, "seed": 5}
+
+    def setup(graphs):
+        s = ssa.Simulation(p, th, precision=precision)
+        s.initialize(cfg)
+        s.set_tile_variants([3, 3, 3, 3])
+        s.set_fused_stages(False, False, pairs=True)
+        if graphs:
+            s.use_graphs(graphs)
+        E = s.compute_energy()
+        E, _, _ = s.steps(E, 6)  # warm-up
+        return s, E
+
+    a, Ea0 = setup(4)
+    Ea, _, _ = a.steps(Ea0, 12, history=True)
+    ha = a.last_history[0]
+    b, Eb0 = setup(4)
+    b.time_passes(Eb0, 12)
+    hb = b.history(12)[0]
+    rises = int(np.sum(np.diff(np.concatenate([[Ea0], ha])) > 0))
+    print(f"[two contexts {precision}] energy rises={rises}")
+    assert Ea0 == Eb0
+    assert np.array_equal(ha, hb) and hb[-1] == Ea
+    assert np.array_equal(a.get_field(), b.get_field())
+    assert np.array_equal(a.get_velocity(), b.get_velocity())
+    a.close()
+    b.close()
