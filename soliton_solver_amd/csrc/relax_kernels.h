@@ -594,21 +594,33 @@ adjoint_b(double* G, const double* Px, const double* Py, const void* Tv, void* l
   }
 }
 
-// Unfused-flow helpers (used by the exact-adjoint flow).
-// Stage field Temp = F + ca dt (V + cb lprev), V read as 0 when killed.
+// Unfused-flow helpers (used by the exact-adjoint and L2-normalised flows).
+// Stage field Temp = F + ca dt (V + cb lprev), V read as 0 when killed.  With
+// l2part non-null (256-thread blocks) each block also writes its partial sum
+// of Temp^2 gv there, in the order of the separate partial-sum kernel of the
+// L2 normalisation launched on the same grid.
 template <typename R>
 __global__ void stage_field(R* Temp, const R* F, const R* V, const R* lp, double ca, double cb, int use_kill,
-                            const DevState* st, Geom g) {
+                            const DevState* st, Geom g, double* l2part) {
   const long long n = (long long)g.nf * g.N;
   const bool kill = use_kill && st->kill;
   const R dt = R(g.dt), a = R(ca), b = R(cb);
+  double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    R t;
     if (ca == 0.0) {
-      Temp[i] = F[i];
+      t = F[i];
     } else {
       const R v = kill ? R(0) : V[i];
-      Temp[i] = F[i] + a * (dt * (v + (cb == 0.0 ? R(0) : b * lp[i])));
+      t = F[i] + a * (dt * (v + (cb == 0.0 ? R(0) : b * lp[i])));
     }
+    Temp[i] = t;
+    if (l2part) s += (double)t * (double)t * g.gv;
+  }
+  if (l2part) {  // the same for every thread of the launch
+    __shared__ double red[kWaves];
+    const double bsum = block_sum<kThreads>(s, red);
+    if (threadIdx.x == 0) l2part[blockIdx.x] = bsum;
   }
 }
 

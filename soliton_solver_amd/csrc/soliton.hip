@@ -893,7 +893,7 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   for (int st = 0; st < 4; ++st) {
     hipLaunchKernelGGL((stage_field<double>), dim3(nb < 4096 ? nb : 4096), dim3(256), 0, c->stream,
                        (double*)c->temp, (const double*)b.F, (const double*)b.V, (const double*)lprev[st], ca[st],
-                       cb[st], st > 0 ? 1 : 0, (const DevState*)c->st, c->g);
+                       cb[st], st > 0 ? 1 : 0, (const DevState*)c->st, c->g, (double*)nullptr);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, c->ops->adjoint(c->prec, c->temp, c->G, c->Px, c->Py, c->temp, lout[st], 1, c->tp, c->g, c->stream));
   }
@@ -907,16 +907,30 @@ static int launch_step_exact(ss_ctx* c, int parity) {
   return 0;
 }
 
-static int l2_normalize_buf(ss_ctx* c, double* G) {
+// Grid of the L2 normalisation's partial sums (at most 1024 blocks, and no
+// more than the partial buffer holds).
+static int l2_blocks(const ss_ctx* c) {
   const long long n = (long long)c->g.nf * c->g.N;
   int nb = (int)((n + 255) / 256);
   if (nb > 1024) nb = 1024;
   if (nb > c->xpart_n) nb = c->xpart_n;
-  hipLaunchKernelGGL(k_l2_partial, dim3(nb), dim3(256), 0, c->stream, (const double*)G, n, c->g.gv, c->xpart);
+  return nb;
+}
+
+// The norm from the nb partial sums in c->xpart, and G divided by it.
+static int l2_finish(ss_ctx* c, double* G, int nb) {
+  const long long n = (long long)c->g.nf * c->g.N;
   hipLaunchKernelGGL(k_fin_sqrt, dim3(1), dim3(1024), 0, c->stream, (const double*)c->xpart, nb, c->scal + 1);
   hipLaunchKernelGGL(k_div_scalar, dim3(nb), dim3(256), 0, c->stream, G, n, (const double*)(c->scal + 1));
   HIPCHK(c, hipGetLastError());
   return 0;
+}
+
+static int l2_normalize_buf(ss_ctx* c, double* G) {
+  const long long n = (long long)c->g.nf * c->g.N;
+  const int nb = l2_blocks(c);
+  hipLaunchKernelGGL(k_l2_partial, dim3(nb), dim3(256), 0, c->stream, (const double*)G, n, c->g.gv, c->xpart);
+  return l2_finish(c, G, nb);
 }
 
 // One step of the unfused L2-normalised reference flow (Bose-Einstein
@@ -932,15 +946,15 @@ static int launch_step_l2(ss_ctx* c, int parity) {
   const double ca[4] = {0.0, 0.5, 0.5, 1.0};
   const double cb[4] = {0.0, 0.0, 0.5, 0.5};
   const void* lprev[4] = {nullptr, nullptr, c->l[0], c->l[1]};
-  const long long n = (long long)c->g.nf * c->g.N;
-  const int nb = (int)((n + 255) / 256);
-  const dim3 grid(nb < 4096 ? nb : 4096);
+  // each stage field is formed together with its partial sums of |.|^2 gv
+  // (on the normalisation's grid, so in its order), then normalised
+  const int nb = l2_blocks(c);
   for (int st = 0; st < 4; ++st) {
-    hipLaunchKernelGGL((stage_field<double>), grid, dim3(256), 0, c->stream, (double*)c->temp, (const double*)b.F,
-                       (const double*)b.V, (const double*)lprev[st], ca[st], cb[st], st > 0 ? 1 : 0,
-                       (const DevState*)c->st, c->g);
+    hipLaunchKernelGGL((stage_field<double>), dim3(nb), dim3(256), 0, c->stream, (double*)c->temp,
+                       (const double*)b.F, (const double*)b.V, (const double*)lprev[st], ca[st], cb[st],
+                       st > 0 ? 1 : 0, (const DevState*)c->st, c->g, c->xpart);
     HIPCHK(c, hipGetLastError());
-    if (l2_normalize_buf(c, (double*)c->temp)) return -1;
+    if (l2_finish(c, (double*)c->temp, nb)) return -1;
     PassBufs v{};
     v.F = c->temp;
     v.st = c->st;
